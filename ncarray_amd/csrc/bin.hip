@@ -1,0 +1,386 @@
+// Binning by an index map: the reference's scatter_add (kernels.cuh:660-680,
+// host/elementwise.hh:482-505) -- dest[idx[i]] += src[i] for 0 <= idx[i] < dest.size,
+// other indices skipped -- rebuilt as per-frame LDS-privatised histograms.
+//
+// nca_hip_bin: grid (chunks, frames).  A block zeroes an LDS histogram (uint32 count +
+// float64 or float32 sum per bin), streams its pixel chunk with 16-byte loads of pixels
+// and bins, folds runs of equal consecutive bins in registers (neighbouring pixels of a
+// detector bin map mostly share a bin) and adds each run to LDS with one atomic pair.
+// The block then writes its histogram to a partial slot; a combine kernel adds the
+// chunk partials of each (frame, bin) in chunk order, so counts are exact and the sums
+// do not depend on block scheduling (float64 in LDS makes the in-block order immaterial
+// at float32 output precision).
+//
+// nca_hip_bin_backward: grad_frames[f, p] = grad_sums[f, map[p]] (0 out of range): the
+// gradient row of each frame is staged in LDS and gathered.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+
+#include "nca_device.hh"
+#include "nca_internal.hh"
+
+namespace nca {
+
+template <typename T> struct BinT;
+template <> struct BinT<float> { static constexpr int32_t dt = NCA_FLOAT32; };
+template <> struct BinT<double> { static constexpr int32_t dt = NCA_FLOAT64; };
+template <> struct BinT<nca_bf16> { static constexpr int32_t dt = NCA_BFLOAT16; };
+template <> struct BinT<nca_f16> { static constexpr int32_t dt = NCA_FLOAT16; };
+template <> struct BinT<int> { static constexpr int32_t dt = NCA_INT32; };
+template <> struct BinT<unsigned short> { static constexpr int32_t dt = NCA_UINT16; };
+template <> struct BinT<unsigned int> { static constexpr int32_t dt = NCA_UINT32; };
+
+// value type folded into the histogram sums
+template <typename T> struct BinV { typedef float V; };
+template <> struct BinV<double> { typedef double V; };
+template <> struct BinV<int> { typedef double V; };
+template <> struct BinV<unsigned int> { typedef double V; };
+
+template <typename T, typename I, typename A, bool NONZERO>
+__global__ __launch_bounds__(512) void k_bin(const T* __restrict__ frames, const I* __restrict__ map, nca_i64 P, int nb,
+                                             nca_i64 chunk, unsigned* __restrict__ pcnt, A* __restrict__ psum) {
+  typedef typename BinV<T>::V V;
+  extern __shared__ unsigned char lds_raw[];
+  A* hs = reinterpret_cast<A*>(lds_raw);
+  unsigned* hc = reinterpret_cast<unsigned*>(lds_raw + sizeof(A) * (size_t)nb);
+  for (int b = threadIdx.x; b < nb; b += blockDim.x) { hs[b] = A(0); hc[b] = 0u; }
+  __syncthreads();
+  const nca_i64 f = blockIdx.y;
+  const int c = blockIdx.x;
+  const nca_i64 p0 = (nca_i64)c * chunk;
+  const nca_i64 p1 = (p0 + chunk < P) ? p0 + chunk : P;
+  const T* fr = frames + f * P;
+  auto flush = [&](int bin, V s, unsigned n) {
+    if (bin >= 0 && bin < nb && n) {
+      atomicAdd(&hs[bin], static_cast<A>(s));
+      atomicAdd(&hc[bin], n);
+    }
+  };
+  // vector body: 4 pixels per thread per step (aligned: host guarantees p0 % 4 == 0)
+  const nca_i64 nvec = (p1 - p0) / 4;
+  for (nca_i64 v = threadIdx.x; v < nvec; v += blockDim.x) {
+    const nca_i64 p = p0 + 4 * v;
+    const nca_vec<T, 4> x = *reinterpret_cast<const nca_vec<T, 4>*>(fr + p);
+    const nca_vec<I, 4> m = *reinterpret_cast<const nca_vec<I, 4>*>(map + p);
+    int cur = static_cast<int>(m.v[0]);
+    if (m.v[0] < 0 || m.v[0] >= nb) cur = -1;
+    V s = 0;
+    unsigned n = 0;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const V val = nca_cvt<V, T>::from(x.v[j]);
+      int bin = (m.v[j] < 0 || m.v[j] >= nb) ? -1 : static_cast<int>(m.v[j]);
+      if (bin != cur) { flush(cur, s, n); cur = bin; s = 0; n = 0; }
+      s += val;
+      n += NONZERO ? (val != V(0) ? 1u : 0u) : 1u;
+    }
+    flush(cur, s, n);
+  }
+  for (nca_i64 p = p0 + 4 * nvec + threadIdx.x; p < p1; p += blockDim.x) {
+    const V val = nca_cvt<V, T>::from(fr[p]);
+    const I bi = map[p];
+    if (bi >= 0 && bi < nb) flush(static_cast<int>(bi), val, NONZERO ? (val != V(0) ? 1u : 0u) : 1u);
+  }
+  __syncthreads();
+  // partial slot of this (frame, chunk)
+  const nca_i64 base = (f * gridDim.x + c) * (nca_i64)nb;
+  for (int b = threadIdx.x; b < nb; b += blockDim.x) {
+    if (psum) psum[base + b] = hs[b];
+    if (pcnt) pcnt[base + b] = hc[b];
+  }
+}
+
+template <typename A, typename CO, typename SO>
+__global__ __launch_bounds__(256) void k_bin_combine(const unsigned* __restrict__ pcnt, const A* __restrict__ psum,
+                                                     int chunks, int nb, nca_i64 F, CO* __restrict__ counts,
+                                                     nca_i64 cst0, nca_i64 cst1, SO* __restrict__ sums, nca_i64 sst0,
+                                                     nca_i64 sst1) {
+  const nca_i64 k = (nca_i64)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= F * nb) return;
+  const nca_i64 f = k / nb;
+  const int b = static_cast<int>(k - f * nb);
+  unsigned long long cnt = 0;
+  double s = 0.0;
+  for (int c = 0; c < chunks; ++c) {
+    const nca_i64 idx = (f * chunks + c) * (nca_i64)nb + b;
+    if (pcnt) cnt += pcnt[idx];
+    if (psum) s += static_cast<double>(psum[idx]);
+  }
+  if (counts) counts[f * cst0 + b * cst1] = static_cast<CO>(cnt);
+  if (sums) nca_st(&sums[f * sst0 + b * sst1], s);
+}
+
+template <typename G, typename I, typename O>
+__global__ __launch_bounds__(256) void k_bin_backward(const G* __restrict__ g, nca_i64 gst0, nca_i64 gst1,
+                                                      const I* __restrict__ map, nca_i64 P, int nb,
+                                                      O* __restrict__ out, nca_i64 chunk) {
+  extern __shared__ unsigned char lds_raw[];
+  float* gl = reinterpret_cast<float*>(lds_raw);
+  const nca_i64 f = blockIdx.y;
+  for (int b = threadIdx.x; b < nb; b += blockDim.x) gl[b] = nca_ld<float>(g + f * gst0 + b * gst1);
+  __syncthreads();
+  const nca_i64 p0 = (nca_i64)blockIdx.x * chunk;
+  const nca_i64 p1 = (p0 + chunk < P) ? p0 + chunk : P;
+  O* o = out + f * P;
+  const nca_i64 nvec = (p1 - p0) / 4;
+  for (nca_i64 v = threadIdx.x; v < nvec; v += blockDim.x) {
+    const nca_i64 p = p0 + 4 * v;
+    const nca_vec<I, 4> m = *reinterpret_cast<const nca_vec<I, 4>*>(map + p);
+    nca_vec<O, 4> r;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const bool in = m.v[j] >= 0 && m.v[j] < nb;
+      r.v[j] = nca_store_cvt<O, float>::to(in ? gl[static_cast<int>(m.v[j])] : 0.0f);
+    }
+    *reinterpret_cast<nca_vec<O, 4>*>(o + p) = r;
+  }
+  for (nca_i64 p = p0 + 4 * nvec + threadIdx.x; p < p1; p += blockDim.x) {
+    const I bi = map[p];
+    nca_st(o + p, (bi >= 0 && bi < nb) ? gl[static_cast<int>(bi)] : 0.0f);
+  }
+}
+
+// scatter_add with the reference's semantics; runs of equal indices in the 4 elements a
+// thread owns are added with one global atomic.
+template <typename D, typename I, typename S>
+__global__ __launch_bounds__(256) void k_scatter_add(D* __restrict__ dest, nca_i64 nd, const I* __restrict__ idx,
+                                                     const S* __restrict__ src, nca_i64 n) {
+  const nca_i64 gs = (nca_i64)gridDim.x * blockDim.x;
+  for (nca_i64 i = (nca_i64)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gs) {
+    const I k = idx[i];
+    if (k >= 0 && static_cast<nca_i64>(k) < nd) atomicAdd(&dest[k], nca_cvt<D, S>::from(src[i]));
+  }
+}
+
+// ---- host side ---------------------------------------------------------------------------------------------
+template <typename F>
+static void dispatch_frame_type(int32_t dt, F&& f) {
+  switch (dt) {
+    case NCA_FLOAT32: f((float*)nullptr); break;
+    case NCA_FLOAT64: f((double*)nullptr); break;
+    case NCA_BFLOAT16: f((nca_bf16*)nullptr); break;
+    case NCA_FLOAT16: f((nca_f16*)nullptr); break;
+    case NCA_INT32: f((int*)nullptr); break;
+    case NCA_UINT16: f((unsigned short*)nullptr); break;
+    case NCA_UINT32: f((unsigned int*)nullptr); break;
+    default: NCA_THROW(NCA_ERR_UNSUPPORTED, "binning: unsupported frame dtype");
+  }
+}
+
+template <typename F>
+static void dispatch_map_type(int32_t dt, F&& f) {
+  switch (dt) {
+    case NCA_INT32: f((int*)nullptr); break;
+    case NCA_INT64: f((long long*)nullptr); break;
+    default: NCA_THROW(NCA_ERR_UNSUPPORTED, "binning: the bin map must be int32 or int64");
+  }
+}
+
+static int64_t frames_layout(const nca_view& frames, const nca_view& map, int64_t& F) {
+  const int64_t P = view_size(map);
+  NCA_REQUIRE(frames.ndim >= 1, "frames must have a frame axis");
+  F = frames.shape[0];
+  int64_t pf = 1;
+  for (int d = 1; d < frames.ndim; ++d) pf *= frames.shape[d];
+  if (frames.ndim == 1) { F = 1; pf = frames.shape[0]; }
+  NCA_REQUIRE(pf == P, "each frame must have as many pixels as the bin map");
+  NCA_REQUIRE(is_c_contiguous(frames) && is_c_contiguous(map), "frames and bin map must be C-contiguous");
+  return P;
+}
+
+static void out2d(const nca_view* v, int64_t F, int nb, const char* what, int64_t& st0, int64_t& st1) {
+  if (!v) return;
+  validate_view(*v, what);
+  int64_t item = itemsize(v->dtype);
+  if (v->ndim == 2) {
+    NCA_REQUIRE(v->shape[0] == F && v->shape[1] == nb, std::string(what) + " must be [frames, n_bins]");
+    st0 = v->strides[0] / item;
+    st1 = v->strides[1] / item;
+    NCA_REQUIRE(v->strides[0] % item == 0 && v->strides[1] % item == 0, std::string(what) + " strides not element-aligned");
+  } else {
+    NCA_REQUIRE(v->ndim == 1 && F == 1 && v->shape[0] == nb, std::string(what) + " must be [frames, n_bins]");
+    st0 = 0;
+    st1 = v->strides[0] / item;
+  }
+}
+
+}  // namespace nca
+
+using namespace nca;
+
+extern "C" {
+
+int nca_hip_bin(const nca_view* frames, const nca_view* binmap, int32_t n_bins, int32_t count_pred,
+                const nca_view* counts, const nca_view* sums, void* stream) {
+  return api_call([&] {
+    NCA_REQUIRE(frames && binmap, "null frames/binmap");
+    NCA_REQUIRE(counts || sums, "nothing to compute");
+    NCA_REQUIRE(n_bins > 0, "n_bins must be positive");
+    NCA_REQUIRE(count_pred == NCA_COUNT_ALL || count_pred == NCA_COUNT_NONZERO, "bad count predicate");
+    validate_view(*frames, "frames");
+    validate_view(*binmap, "binmap");
+    int64_t F = 0;
+    const int64_t P = frames_layout(*frames, *binmap, F);
+    int64_t cst0 = 0, cst1 = 0, sst0 = 0, sst1 = 0;
+    out2d(counts, F, n_bins, "counts", cst0, cst1);
+    out2d(sums, F, n_bins, "sums", sst0, sst1);
+    if (counts) NCA_REQUIRE(counts->dtype == NCA_INT32 || counts->dtype == NCA_INT64, "counts must be int32 or int64");
+    if (sums) NCA_REQUIRE(sums->dtype == NCA_FLOAT32 || sums->dtype == NCA_FLOAT64, "sums must be float32 or float64");
+    hipStream_t s = as_stream(stream);
+    DeviceState& dev = current_device();
+    const size_t lds = (sizeof(double) + sizeof(unsigned)) * static_cast<size_t>(n_bins);
+    NCA_REQUIRE(lds <= static_cast<size_t>(dev.lds_per_block), "too many bins for an LDS histogram");
+    // chunks per frame: about 4 blocks per CU overall, chunk a multiple of 4 pixels
+    int64_t chunks = std::max<int64_t>(1, (static_cast<int64_t>(dev.n_cus) * 4 + F - 1) / std::max<int64_t>(F, 1));
+    chunks = std::min<int64_t>(chunks, std::max<int64_t>(1, P / 4096));
+    int64_t chunk = (P + chunks - 1) / chunks;
+    chunk = (chunk + 3) / 4 * 4;
+    chunks = (P + chunk - 1) / chunk;
+    NCA_REQUIRE(F < 65536, "at most 65535 frames per call");
+    const size_t slots = static_cast<size_t>(F) * chunks * n_bins;
+    Workspace& w = workspace(s, 1, slots * (sizeof(double) + sizeof(unsigned)));
+    double* psum = static_cast<double*>(w.big);
+    unsigned* pcnt = reinterpret_cast<unsigned*>(psum + slots);
+    if (P == 0 || F == 0) {
+      // empty frames: zero outputs through the combine kernel with zero chunks
+    }
+    dispatch_frame_type(frames->dtype, [&](auto* tp) {
+      typedef std::remove_pointer_t<decltype(tp)> T;
+      dispatch_map_type(binmap->dtype, [&](auto* ip) {
+        typedef std::remove_pointer_t<decltype(ip)> I;
+        NCA_REQUIRE(reinterpret_cast<uintptr_t>(frames->data) % (4 * sizeof(T)) == 0 &&
+                        reinterpret_cast<uintptr_t>(binmap->data) % (4 * sizeof(I)) == 0 && (P % 4 == 0 || F == 1),
+                    "binning needs 4-element aligned frames and map (and pixels per frame % 4 == 0)");
+        if (P > 0 && F > 0) {
+          dim3 grid(static_cast<unsigned>(chunks), static_cast<unsigned>(F));
+          if (count_pred == NCA_COUNT_NONZERO)
+            hipLaunchKernelGGL((k_bin<T, I, double, true>), grid, dim3(512), lds, s, (const T*)frames->data,
+                               (const I*)binmap->data, (nca_i64)P, n_bins, (nca_i64)chunk, pcnt, psum);
+          else
+            hipLaunchKernelGGL((k_bin<T, I, double, false>), grid, dim3(512), lds, s, (const T*)frames->data,
+                               (const I*)binmap->data, (nca_i64)P, n_bins, (nca_i64)chunk, pcnt, psum);
+        }
+      });
+    });
+    const int nchunks = (P > 0) ? static_cast<int>(chunks) : 0;
+    dim3 g2(static_cast<unsigned>((F * n_bins + 255) / 256));
+    auto launch_combine = [&](auto* cp, auto* sp) {
+      typedef std::remove_pointer_t<decltype(cp)> CO;
+      typedef std::remove_pointer_t<decltype(sp)> SO;
+      hipLaunchKernelGGL((k_bin_combine<double, CO, SO>), g2, dim3(256), 0, s, counts ? pcnt : nullptr,
+                         sums ? psum : nullptr, nchunks, n_bins, (nca_i64)F, counts ? (CO*)counts->data : nullptr,
+                         (nca_i64)cst0, (nca_i64)cst1, sums ? (SO*)sums->data : nullptr, (nca_i64)sst0, (nca_i64)sst1);
+    };
+    const bool c64 = counts && counts->dtype == NCA_INT64;
+    const bool s64 = sums && sums->dtype == NCA_FLOAT64;
+    if (c64 && s64) launch_combine((long long*)nullptr, (double*)nullptr);
+    else if (c64) launch_combine((long long*)nullptr, (float*)nullptr);
+    else if (s64) launch_combine((int*)nullptr, (double*)nullptr);
+    else launch_combine((int*)nullptr, (float*)nullptr);
+    check_launch("bin");
+    finish_call(s);
+  });
+}
+
+int nca_hip_bin_backward(const nca_view* grad_sums, const nca_view* binmap, const nca_view* grad_frames, void* stream) {
+  return api_call([&] {
+    NCA_REQUIRE(grad_sums && binmap && grad_frames, "null argument");
+    validate_view(*grad_sums, "grad_sums");
+    validate_view(*binmap, "binmap");
+    validate_view(*grad_frames, "grad_frames");
+    int64_t F = 0;
+    const int64_t P = frames_layout(*grad_frames, *binmap, F);
+    NCA_REQUIRE(grad_sums->ndim >= 1, "grad_sums must be [frames, n_bins]");
+    const int nb = static_cast<int>(grad_sums->shape[grad_sums->ndim - 1]);
+    int64_t gst0 = 0, gst1 = 0;
+    out2d(grad_sums, F, nb, "grad_sums", gst0, gst1);
+    hipStream_t s = as_stream(stream);
+    DeviceState& dev = current_device();
+    const size_t lds = sizeof(float) * static_cast<size_t>(nb);
+    NCA_REQUIRE(lds <= static_cast<size_t>(dev.lds_per_block), "too many bins");
+    int64_t chunks = std::max<int64_t>(1, (static_cast<int64_t>(dev.n_cus) * 8 + F - 1) / std::max<int64_t>(F, 1));
+    chunks = std::min<int64_t>(chunks, std::max<int64_t>(1, P / 4096));
+    int64_t chunk = ((P + chunks - 1) / chunks + 3) / 4 * 4;
+    chunks = (P + chunk - 1) / chunk;
+    if (P == 0 || F == 0) return;
+    NCA_REQUIRE(F < 65536, "at most 65535 frames per call");
+    auto launch = [&](auto* gp, auto* ip, auto* op) {
+      typedef std::remove_pointer_t<decltype(gp)> G;
+      typedef std::remove_pointer_t<decltype(ip)> I;
+      typedef std::remove_pointer_t<decltype(op)> O;
+      NCA_REQUIRE(reinterpret_cast<uintptr_t>(grad_frames->data) % (4 * sizeof(O)) == 0 &&
+                      reinterpret_cast<uintptr_t>(binmap->data) % (4 * sizeof(I)) == 0 && (P % 4 == 0 || F == 1),
+                  "bin backward needs 4-element aligned outputs and map");
+      hipLaunchKernelGGL((k_bin_backward<G, I, O>), dim3(static_cast<unsigned>(chunks), static_cast<unsigned>(F)),
+                         dim3(256), lds, s, (const G*)grad_sums->data, (nca_i64)gst0, (nca_i64)gst1,
+                         (const I*)binmap->data, (nca_i64)P, nb, (O*)grad_frames->data, (nca_i64)chunk);
+    };
+    auto with_out = [&](auto* gp, auto* ip) {
+      switch (grad_frames->dtype) {
+        case NCA_FLOAT32: launch(gp, ip, (float*)nullptr); break;
+        case NCA_FLOAT64: launch(gp, ip, (double*)nullptr); break;
+        case NCA_BFLOAT16: launch(gp, ip, (nca_bf16*)nullptr); break;
+        case NCA_FLOAT16: launch(gp, ip, (nca_f16*)nullptr); break;
+        default: NCA_THROW(NCA_ERR_UNSUPPORTED, "bin backward: gradient dtype must be floating point");
+      }
+    };
+    auto with_map = [&](auto* gp) {
+      dispatch_map_type(binmap->dtype, [&](auto* ip) { with_out(gp, ip); });
+    };
+    switch (grad_sums->dtype) {
+      case NCA_FLOAT32: with_map((float*)nullptr); break;
+      case NCA_FLOAT64: with_map((double*)nullptr); break;
+      default: NCA_THROW(NCA_ERR_UNSUPPORTED, "grad_sums must be float32 or float64");
+    }
+    check_launch("bin_backward");
+    finish_call(s);
+  });
+}
+
+int nca_hip_scatter_add(const nca_view* dest, const nca_view* idx, const nca_view* src, void* stream) {
+  return api_call([&] {
+    NCA_REQUIRE(dest && idx && src, "null argument");
+    validate_view(*dest, "dest");
+    validate_view(*idx, "idx");
+    validate_view(*src, "src");
+    NCA_REQUIRE(dest->ndim == 1 && is_c_contiguous(*dest), "dest must be 1-D contiguous");
+    NCA_REQUIRE(is_c_contiguous(*idx) && is_c_contiguous(*src) && view_size(*idx) == view_size(*src),
+                "idx and src must be contiguous with the same size");
+    const int64_t n = view_size(*src), nd = dest->shape[0];
+    hipStream_t s = as_stream(stream);
+    DeviceState& dev = current_device();
+    if (n == 0) return;
+    int blocks = static_cast<int>(std::min<int64_t>((n + 255) / 256, dev.n_cus * 16));
+    auto launch = [&](auto* dp, auto* ip, auto* sp) {
+      typedef std::remove_pointer_t<decltype(dp)> D;
+      typedef std::remove_pointer_t<decltype(ip)> I;
+      typedef std::remove_pointer_t<decltype(sp)> S;
+      hipLaunchKernelGGL((k_scatter_add<D, I, S>), dim3(blocks), dim3(256), 0, s, (D*)dest->data, (nca_i64)nd,
+                         (const I*)idx->data, (const S*)src->data, (nca_i64)n);
+    };
+    auto with_src = [&](auto* dp, auto* ip) {
+      switch (src->dtype) {
+        case NCA_FLOAT32: launch(dp, ip, (float*)nullptr); break;
+        case NCA_FLOAT64: launch(dp, ip, (double*)nullptr); break;
+        case NCA_INT32: launch(dp, ip, (int*)nullptr); break;
+        case NCA_UINT16: launch(dp, ip, (unsigned short*)nullptr); break;
+        case NCA_BFLOAT16: launch(dp, ip, (nca_bf16*)nullptr); break;
+        default: NCA_THROW(NCA_ERR_UNSUPPORTED, "scatter_add: unsupported src dtype");
+      }
+    };
+    auto with_idx = [&](auto* dp) { dispatch_map_type(idx->dtype, [&](auto* ip) { with_src(dp, ip); }); };
+    switch (dest->dtype) {
+      case NCA_FLOAT32: with_idx((float*)nullptr); break;
+      case NCA_FLOAT64: with_idx((double*)nullptr); break;
+      case NCA_INT32: with_idx((int*)nullptr); break;
+      case NCA_UINT32: with_idx((unsigned int*)nullptr); break;
+      case NCA_UINT64: with_idx((unsigned long long*)nullptr); break;
+      default: NCA_THROW(NCA_ERR_UNSUPPORTED, "scatter_add: unsupported dest dtype");
+    }
+    check_launch("scatter_add");
+    finish_call(s);
+  });
+}
+
+}  // extern "C"

@@ -1,0 +1,194 @@
+// Device-side helpers shared by the ahead-of-time kernels (compiled by hipcc) and by the
+// kernels generated at run time (this file's text is prepended to every hipRTC source,
+// so it must not include any header).  gfx950: 64-lane wavefronts.
+#ifndef NCA_DEVICE_HH
+#define NCA_DEVICE_HH
+
+typedef long long nca_i64;
+typedef unsigned long long nca_u64;
+
+#define NCA_WAVE 64
+
+// ---- storage-only float types ------------------------------------------------------
+// bfloat16: the top 16 bits of an IEEE binary32.  float16: IEEE binary16.
+struct nca_bf16 { unsigned short bits; };
+struct nca_f16 { unsigned short bits; };
+
+__device__ __forceinline__ float nca_bf16_to_f32(unsigned short b) {
+  return __uint_as_float(static_cast<unsigned>(b) << 16);
+}
+// Round to nearest, ties to even; NaN stays a (quiet) NaN.
+__device__ __forceinline__ unsigned short nca_f32_to_bf16(float f) {
+  unsigned u = __float_as_uint(f);
+  if ((u & 0x7fffffffu) > 0x7f800000u) return static_cast<unsigned short>((u >> 16) | 0x40u);
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return static_cast<unsigned short>(u >> 16);
+}
+__device__ __forceinline__ float nca_f16_to_f32(unsigned short b) {
+  _Float16 h;
+  __builtin_memcpy(&h, &b, 2);
+  return static_cast<float>(h);
+}
+__device__ __forceinline__ unsigned short nca_f32_to_f16(float f) {
+  _Float16 h = static_cast<_Float16>(f);
+  unsigned short b;
+  __builtin_memcpy(&b, &h, 2);
+  return b;
+}
+
+// Load one element of storage type T as compute type W (static_cast semantics, as the
+// reference's op_traits<T>::cast, op_traits.hh:93-117).
+template <typename W, typename T> struct nca_cvt {
+  __device__ __forceinline__ static W from(T v) { return static_cast<W>(v); }
+};
+template <typename W> struct nca_cvt<W, nca_bf16> {
+  __device__ __forceinline__ static W from(nca_bf16 v) { return static_cast<W>(nca_bf16_to_f32(v.bits)); }
+};
+template <typename W> struct nca_cvt<W, nca_f16> {
+  __device__ __forceinline__ static W from(nca_f16 v) { return static_cast<W>(nca_f16_to_f32(v.bits)); }
+};
+// Store compute value W into storage type T.
+template <typename T, typename W> struct nca_store_cvt {
+  __device__ __forceinline__ static T to(W v) { return static_cast<T>(v); }
+};
+template <typename W> struct nca_store_cvt<nca_bf16, W> {
+  __device__ __forceinline__ static nca_bf16 to(W v) {
+    nca_bf16 r; r.bits = nca_f32_to_bf16(static_cast<float>(v)); return r;
+  }
+};
+template <typename W> struct nca_store_cvt<nca_f16, W> {
+  __device__ __forceinline__ static nca_f16 to(W v) {
+    nca_f16 r; r.bits = nca_f32_to_f16(static_cast<float>(v)); return r;
+  }
+};
+
+template <typename W, typename T> __device__ __forceinline__ W nca_ld(const T* p) {
+  return nca_cvt<W, T>::from(*p);
+}
+template <typename T, typename W> __device__ __forceinline__ void nca_st(T* p, W v) {
+  *p = nca_store_cvt<T, W>::to(v);
+}
+
+// ---- numeric limits ------------------------------------------------------------------
+template <typename T> struct nca_lim;
+template <> struct nca_lim<bool> { __device__ static bool lowest() { return false; } __device__ static bool max() { return true; } };
+template <> struct nca_lim<signed char> { __device__ static signed char lowest() { return -128; } __device__ static signed char max() { return 127; } };
+template <> struct nca_lim<unsigned char> { __device__ static unsigned char lowest() { return 0; } __device__ static unsigned char max() { return 255; } };
+template <> struct nca_lim<short> { __device__ static short lowest() { return -32768; } __device__ static short max() { return 32767; } };
+template <> struct nca_lim<unsigned short> { __device__ static unsigned short lowest() { return 0; } __device__ static unsigned short max() { return 65535; } };
+template <> struct nca_lim<int> { __device__ static int lowest() { return -2147483647 - 1; } __device__ static int max() { return 2147483647; } };
+template <> struct nca_lim<unsigned int> { __device__ static unsigned int lowest() { return 0u; } __device__ static unsigned int max() { return 4294967295u; } };
+template <> struct nca_lim<long long> { __device__ static long long lowest() { return -9223372036854775807LL - 1; } __device__ static long long max() { return 9223372036854775807LL; } };
+template <> struct nca_lim<unsigned long long> { __device__ static unsigned long long lowest() { return 0ull; } __device__ static unsigned long long max() { return 18446744073709551615ull; } };
+template <> struct nca_lim<float> { __device__ static float lowest() { return -3.402823466e+38f; } __device__ static float max() { return 3.402823466e+38f; } };
+template <> struct nca_lim<double> { __device__ static double lowest() { return -1.7976931348623157e+308; } __device__ static double max() { return 1.7976931348623157e+308; } };
+
+// ---- compensated summation ----------------------------------------------------------
+// Neumaier's variant of Kahan summation: s is the running sum, c the running
+// compensation; the represented value is s + c.
+template <typename F>
+__device__ __forceinline__ void nca_neumaier_add(F& s, F& c, F x) {
+  F t = s + x;
+  F ax = x < F(0) ? -x : x;
+  F as = s < F(0) ? -s : s;
+  F e = (as >= ax) ? ((s - t) + x) : ((x - t) + s);
+  c += e;
+  s = t;
+}
+// Knuth's TwoSum: s + e == a + b exactly.
+template <typename F>
+__device__ __forceinline__ void nca_two_sum(F a, F b, F& s, F& e) {
+  s = a + b;
+  F bb = s - a;
+  e = (a - (s - bb)) + (b - bb);
+}
+// Merge compensated pair (s2, c2) into (s1, c1).
+template <typename F>
+__device__ __forceinline__ void nca_comp_merge(F& s1, F& c1, F s2, F c2) {
+  F s, e;
+  nca_two_sum(s1, s2, s, e);
+  s1 = s;
+  c1 = c1 + c2 + e;
+}
+
+// ---- wave / block reductions (64-lane butterflies; every lane ends with the result) --
+template <typename T>
+__device__ __forceinline__ T nca_shfl_xor(T v, int m) {
+  if constexpr (sizeof(T) < 4) return static_cast<T>(__shfl_xor(static_cast<int>(v), m, NCA_WAVE));
+  else return __shfl_xor(v, m, NCA_WAVE);
+}
+__device__ __forceinline__ bool nca_shfl_xor(bool v, int m) { return __shfl_xor(static_cast<int>(v), m, NCA_WAVE) != 0; }
+
+template <typename T>
+__device__ __forceinline__ T nca_wave_sum(T v) {
+#pragma unroll
+  for (int m = 32; m > 0; m >>= 1) v = v + nca_shfl_xor(v, m);
+  return v;
+}
+template <typename F>
+__device__ __forceinline__ void nca_wave_comp(F& s, F& c) {
+#pragma unroll
+  for (int m = 32; m > 0; m >>= 1) {
+    F s2 = nca_shfl_xor(s, m), c2 = nca_shfl_xor(c, m);
+    nca_comp_merge(s, c, s2, c2);
+  }
+}
+// max/min with the host fold's rule: a candidate replaces the running value only when
+// it is strictly greater (less); NaN never wins (host/reductions.hh:130-146).
+template <typename T> __device__ __forceinline__ T nca_maxop(T a, T b) { return (b > a) ? b : a; }
+template <typename T> __device__ __forceinline__ T nca_minop(T a, T b) { return (b < a) ? b : a; }
+template <typename T>
+__device__ __forceinline__ T nca_wave_max(T v) {
+#pragma unroll
+  for (int m = 32; m > 0; m >>= 1) v = nca_maxop(v, nca_shfl_xor(v, m));
+  return v;
+}
+template <typename T>
+__device__ __forceinline__ T nca_wave_min(T v) {
+#pragma unroll
+  for (int m = 32; m > 0; m >>= 1) v = nca_minop(v, nca_shfl_xor(v, m));
+  return v;
+}
+// (value, index) pairs for argmax/argmin: idx < 0 means "nothing selected yet"; on
+// equal values the lower index wins (first occurrence, host/reductions.hh:148-169).
+template <typename T, bool IsMax>
+__device__ __forceinline__ void nca_arg_merge(T& v, nca_i64& i, T v2, nca_i64 i2) {
+  if (i2 < 0) return;
+  bool take;
+  if (i < 0) take = true;
+  else if (IsMax ? (v2 > v) : (v2 < v)) take = true;
+  else if (IsMax ? (v > v2) : (v < v2)) take = false;
+  else take = i2 < i;
+  if (take) { v = v2; i = i2; }
+}
+template <typename T, bool IsMax>
+__device__ __forceinline__ void nca_wave_arg(T& v, nca_i64& i) {
+#pragma unroll
+  for (int m = 32; m > 0; m >>= 1) {
+    T v2 = nca_shfl_xor(v, m);
+    nca_i64 i2 = nca_shfl_xor(i, m);
+    nca_arg_merge<T, IsMax>(v, i, v2, i2);
+  }
+}
+
+// Grid-completion ticket for the "last block finishes" pattern: returns true in the
+// block that arrives last; that block also re-arms the counter for the next launch.
+__device__ __forceinline__ bool nca_last_block(unsigned* counter, unsigned nblocks) {
+  __shared__ unsigned nca_ticket;
+  __threadfence();
+  __syncthreads();
+  if (threadIdx.x == 0) nca_ticket = atomicAdd(counter, 1u);
+  __syncthreads();
+  bool last = (nca_ticket == nblocks - 1u);
+  if (last) {
+    __threadfence();
+    if (threadIdx.x == 0) atomicExch(counter, 0u);
+  }
+  return last;
+}
+
+// 128-bit vector of N elements of T (N * sizeof(T) == 16 or smaller power of two).
+template <typename T, int N>
+struct __attribute__((aligned(sizeof(T) * N))) nca_vec { T v[N]; };
+
+#endif  // NCA_DEVICE_HH

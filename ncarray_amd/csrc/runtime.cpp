@@ -1,0 +1,278 @@
+// libnca_hip runtime: initialisation, errors, device memory, scratch workspaces and view
+// helpers.  Replaces the CUDA context / DevOwnerPolicy / CircularDevicePool plumbing of
+// the reference (jit_utils.cc:28-46, storage.hh:176-186, 242-252, mem_pool.cuh:47-123).
+#include <cstdio>
+#include <cstdlib>
+
+#include "nca_internal.hh"
+
+namespace nca {
+
+namespace {
+thread_local std::string t_last_error;
+thread_local int t_device = -1;
+std::mutex g_devices_mu;
+std::map<int, std::unique_ptr<DeviceState>> g_devices;
+}  // namespace
+
+void set_last_error(const std::string& msg) { t_last_error = msg; }
+
+bool device_available() { return t_device >= 0; }
+
+DeviceState& current_device() {
+  if (t_device < 0) NCA_THROW(NCA_ERR_NOT_INIT, "nca_hip_init() was not called on this thread");
+  std::lock_guard<std::mutex> lk(g_devices_mu);
+  auto it = g_devices.find(t_device);
+  if (it == g_devices.end()) NCA_THROW(NCA_ERR_NOT_INIT, "device state missing");
+  return *it->second;
+}
+
+static void init_device(int device) {
+  int count = 0;
+  HIP_CHECK(hipGetDeviceCount(&count));
+  NCA_REQUIRE(device >= 0 && device < count,
+              "device " + std::to_string(device) + " out of range (" + std::to_string(count) + " devices)");
+  HIP_CHECK(hipSetDevice(device));
+  std::lock_guard<std::mutex> lk(g_devices_mu);
+  if (g_devices.count(device) == 0) {
+    auto st = std::make_unique<DeviceState>();
+    hipDeviceProp_t p;
+    HIP_CHECK(hipGetDeviceProperties(&p, device));
+    st->id = device;
+    st->n_cus = p.multiProcessorCount;
+    st->lds_per_block = static_cast<int>(p.sharedMemPerBlock);
+    st->max_threads = p.maxThreadsPerBlock;
+    st->total_mem = static_cast<int64_t>(p.totalGlobalMem);
+    std::string arch = p.gcnArchName;
+    auto colon = arch.find(':');
+    st->arch = colon == std::string::npos ? arch : arch.substr(0, colon);
+    g_devices[device] = std::move(st);
+  }
+  t_device = device;
+}
+
+Workspace& workspace(hipStream_t s, size_t partial_bytes, size_t big_bytes) {
+  DeviceState& dev = current_device();
+  std::lock_guard<std::mutex> lk(dev.mu);
+  auto& slot = dev.workspaces[s];
+  if (!slot) slot = std::make_unique<Workspace>();
+  Workspace& w = *slot;
+  auto capturing = [&]() {
+    hipStreamCaptureStatus st = hipStreamCaptureStatusNone;
+    if (s != nullptr) HIP_CHECK(hipStreamIsCapturing(s, &st));
+    return st != hipStreamCaptureStatusNone;
+  };
+  const size_t min_partials = size_t(1) << 20;
+  if (partial_bytes < min_partials) partial_bytes = min_partials;
+  if (w.partials_bytes < partial_bytes || w.counters == nullptr ||
+      (big_bytes > 0 && w.big_bytes < big_bytes)) {
+    if (capturing())
+      NCA_THROW(NCA_ERR_INVALID,
+                "workspace must grow during hipGraph capture; run the call once before capturing");
+    // Grow: the device must be idle for this stream before old buffers are freed.
+    HIP_CHECK(hipStreamSynchronize(s));
+  }
+  if (w.partials_bytes < partial_bytes) {
+    if (w.partials) HIP_CHECK(hipFree(w.partials));
+    w.partials = nullptr;
+    HIP_CHECK(hipMalloc(&w.partials, partial_bytes));
+    w.partials_bytes = partial_bytes;
+  }
+  if (w.counters == nullptr) {
+    w.n_counters = kWorkspaceCounters;
+    HIP_CHECK(hipMalloc(reinterpret_cast<void**>(&w.counters), w.n_counters * sizeof(unsigned)));
+    HIP_CHECK(hipMemset(w.counters, 0, w.n_counters * sizeof(unsigned)));
+    HIP_CHECK(hipDeviceSynchronize());
+  }
+  if (big_bytes > 0 && w.big_bytes < big_bytes) {
+    if (w.big) HIP_CHECK(hipFree(w.big));
+    w.big = nullptr;
+    HIP_CHECK(hipMalloc(&w.big, big_bytes));
+    HIP_CHECK(hipMemset(w.big, 0, big_bytes));
+    HIP_CHECK(hipDeviceSynchronize());
+    w.big_bytes = big_bytes;
+  }
+  return w;
+}
+
+void finish_call(hipStream_t s) {
+  if (s == nullptr) HIP_CHECK(hipStreamSynchronize(nullptr));
+}
+
+void check_launch(const char* what) {
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) NCA_THROW(NCA_ERR_HIP, std::string(what) + " launch failed: " + hipGetErrorString(e));
+}
+
+// ---- views ---------------------------------------------------------------------------
+int64_t view_size(const nca_view& v) {
+  int64_t n = 1;
+  for (int d = 0; d < v.ndim; ++d) n *= v.shape[d];
+  return n;
+}
+
+void validate_view(const nca_view& v, const char* what, bool allow_null_data) {
+  NCA_REQUIRE(v.ndim >= 0 && v.ndim <= NCA_MAX_NDIM, std::string(what) + ": ndim out of range");
+  if (!is_supported(v.dtype))
+    NCA_THROW(NCA_ERR_UNSUPPORTED,
+              std::string(what) + ": dtype " +
+                  (v.dtype >= 0 && v.dtype <= NCA_FLOAT16 ? dtype_info(v.dtype).name : "?") +
+                  " has no device kernels");
+  for (int d = 0; d < v.ndim; ++d)
+    NCA_REQUIRE(v.shape[d] >= 0, std::string(what) + ": negative extent");
+  if (v.pointer_axis >= 0)
+    NCA_THROW(NCA_ERR_UNSUPPORTED, std::string(what) + ": NC pointer-axis layouts are not supported by the device kernels");
+  if (!allow_null_data && view_size(v) > 0)
+    NCA_REQUIRE(v.data != nullptr, std::string(what) + ": null data pointer");
+}
+
+bool is_c_contiguous(const nca_view& v) {
+  int64_t expect = itemsize(v.dtype);
+  for (int d = v.ndim - 1; d >= 0; --d) {
+    if (v.shape[d] == 1) continue;
+    if (v.strides[d] != expect) return false;
+    expect *= v.shape[d];
+  }
+  return true;
+}
+
+bool same_shape(const nca_view& a, const nca_view& b) {
+  if (a.ndim != b.ndim) return false;
+  for (int d = 0; d < a.ndim; ++d)
+    if (a.shape[d] != b.shape[d]) return false;
+  return true;
+}
+
+bool is_broadcast_scalar(const nca_view& v) {
+  for (int d = 0; d < v.ndim; ++d)
+    if (v.shape[d] != 1 && v.strides[d] != 0) return false;
+  return true;
+}
+
+int coalesce(const int64_t* shape, const int64_t* strides, int ndim, int64_t* oshape,
+             int64_t* ostrides) {
+  int n = 0;
+  for (int d = 0; d < ndim; ++d) {
+    if (shape[d] == 1) continue;
+    if (n > 0 && ostrides[n - 1] == strides[d] * shape[d]) {
+      oshape[n - 1] *= shape[d];
+      ostrides[n - 1] = strides[d];
+    } else {
+      oshape[n] = shape[d];
+      ostrides[n] = strides[d];
+      ++n;
+    }
+  }
+  return n;
+}
+
+}  // namespace nca
+
+using namespace nca;
+
+extern "C" {
+
+int nca_hip_init(int device) {
+  return api_call([&] { init_device(device); });
+}
+
+int nca_hip_shutdown(void) {
+  return api_call([&] {
+    std::lock_guard<std::mutex> lk(g_devices_mu);
+    for (auto& kv : g_devices) {
+      HIP_CHECK(hipSetDevice(kv.first));
+      HIP_CHECK(hipDeviceSynchronize());
+      for (auto& w : kv.second->workspaces) {
+        if (w.second->partials) (void)hipFree(w.second->partials);
+        if (w.second->counters) (void)hipFree(w.second->counters);
+        if (w.second->big) (void)hipFree(w.second->big);
+      }
+      kv.second->workspaces.clear();
+    }
+  });
+}
+
+int nca_hip_last_error(char* buf, size_t n) {
+  if (buf && n > 0) {
+    size_t k = t_last_error.size() < n - 1 ? t_last_error.size() : n - 1;
+    std::memcpy(buf, t_last_error.data(), k);
+    buf[k] = '\0';
+  }
+  return static_cast<int>(t_last_error.size());
+}
+
+const char* nca_hip_version(void) { return "ncarray-hip 0.1.0 gfx950"; }
+
+int nca_hip_device_info(int32_t* n_cus, int32_t* lds_per_block, int64_t* total_mem,
+                        char* arch_name, size_t arch_len) {
+  return api_call([&] {
+    DeviceState& d = current_device();
+    if (n_cus) *n_cus = d.n_cus;
+    if (lds_per_block) *lds_per_block = d.lds_per_block;
+    if (total_mem) *total_mem = d.total_mem;
+    if (arch_name && arch_len > 0) {
+      std::snprintf(arch_name, arch_len, "%s", d.arch.c_str());
+    }
+  });
+}
+
+int nca_hip_malloc(void** ptr, size_t bytes) {
+  return api_call([&] {
+    NCA_REQUIRE(ptr != nullptr, "nca_hip_malloc: null out pointer");
+    current_device();
+    *ptr = nullptr;
+    if (bytes == 0) bytes = 256;
+    HIP_CHECK(hipMalloc(ptr, bytes));
+  });
+}
+
+int nca_hip_free(void* ptr) {
+  return api_call([&] {
+    if (ptr) HIP_CHECK(hipFree(ptr));
+  });
+}
+
+int nca_hip_memcpy_h2d(void* dst, const void* src, size_t bytes, void* stream) {
+  return api_call([&] {
+    if (bytes == 0) return;
+    HIP_CHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, as_stream(stream)));
+    finish_call(as_stream(stream));
+  });
+}
+
+int nca_hip_memcpy_d2h(void* dst, const void* src, size_t bytes, void* stream) {
+  return api_call([&] {
+    if (bytes == 0) return;
+    HIP_CHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, as_stream(stream)));
+    finish_call(as_stream(stream));
+  });
+}
+
+int nca_hip_memcpy_d2d(void* dst, const void* src, size_t bytes, void* stream) {
+  return api_call([&] {
+    if (bytes == 0) return;
+    HIP_CHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, as_stream(stream)));
+    finish_call(as_stream(stream));
+  });
+}
+
+int nca_hip_synchronize(void* stream) {
+  return api_call([&] { HIP_CHECK(hipStreamSynchronize(as_stream(stream))); });
+}
+
+int nca_hip_reduce_result_dtype(int32_t op, int32_t src_dtype, int32_t* out_dtype) {
+  return api_call([&] {
+    NCA_REQUIRE(out_dtype != nullptr, "null out_dtype");
+    *out_dtype = reduce_result_dtype(op, src_dtype);
+  });
+}
+
+int nca_hip_op_result_dtype(int32_t op, int32_t work_dtype, int32_t* out_dtype) {
+  return api_call([&] {
+    NCA_REQUIRE(out_dtype != nullptr, "null out_dtype");
+    (void)dtype_info(work_dtype);
+    *out_dtype = op_result_dtype(op, work_dtype);
+  });
+}
+
+}  // extern "C"

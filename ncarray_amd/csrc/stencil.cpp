@@ -1,0 +1,738 @@
+// Stencils: Stencil<NDIM>::create/apply (stencil.hh:59-255) and the NVRTC stencil
+// source generator (rtcompiler.hh:187-385), rebuilt for gfx950 with hipRTC.
+//
+// dest[c] = f(src[c*step + off_k] for each offset k, extra_j[c]) with
+// step = src.shape / dest.shape (integer division, rtcompiler.hh:363-368).
+//
+// Two generated kernels:
+//  * march (3-D, step 1, |offsets| <= 3, C-contiguous, 16-byte aligned rows): a block owns
+//    a 64 x 16 (x, y) tile and a run of z planes.  Each plane of the tile plus its halo is
+//    staged once in an LDS ring of 2*Rz+2 planes (one barrier per plane), the next plane
+//    is prefetched into registers while the current one is computed, and every thread
+//    produces 4 consecutive x outputs (16-byte stores).  Reductions of the stored values
+//    (sum along axis 0, global sum/max/min) ride along in registers.  Blocks are mapped so
+//    that neighbouring tiles land on the same XCD (shared L2 for halos).
+//  * direct (any rank, any step, pro/epilogue hooks): one output per thread, one
+//    predicated load per offset, the reference's launch shape (ceil(n/256) x 256).
+#include <cstdio>
+#include <sstream>
+#include <unordered_map>
+
+#include "expr_internal.hh"
+#include "nca_internal.hh"
+#include "tape.hh"
+
+namespace nca {
+
+struct StencilProgram {
+  int ndim = 0;
+  int n_off = 0;
+  std::vector<std::array<int64_t, NCA_MAX_NDIM>> offsets;
+  int32_t src_dtype = NCA_FLOAT32;
+  std::vector<int32_t> extra_dtypes;
+  Tape tape;
+  std::string extra_params, prologue, epilogue;
+  bool has_ext = false;
+  uint64_t forward = 0;  // for adjoints: the forward stencil
+  std::mutex mu;
+  std::unordered_map<std::string, hipFunction_t> kernels;
+  std::unordered_map<std::string, std::string> sources;
+};
+
+namespace {
+std::mutex g_st_mu;
+std::unordered_map<uint64_t, std::unique_ptr<StencilProgram>> g_stencils;
+uint64_t g_next_stencil = 1ull << 40;
+}  // namespace
+
+static StencilProgram& stencil_program(uint64_t h) {
+  std::lock_guard<std::mutex> lk(g_st_mu);
+  auto it = g_stencils.find(h);
+  NCA_REQUIRE(it != g_stencils.end(), "unknown stencil handle " + std::to_string(h));
+  return *it->second;
+}
+
+static uint64_t stencil_register(std::unique_ptr<StencilProgram> p) {
+  std::lock_guard<std::mutex> lk(g_st_mu);
+  uint64_t h = g_next_stencil++;
+  g_stencils[h] = std::move(p);
+  return h;
+}
+
+// The reference's stencil source evaluates the tape with plain C++ operators in work_t
+// (rtcompiler.hh:204-234): division has no zero-divisor rule here.
+static Tape stencil_tape(const nca_expr_desc& d) {
+  Tape t = Tape::from_desc(d);
+  t.eval_dtype = t.work_dtype;
+  t.compute_dtype = compute_dtype(t.work_dtype);
+  for (auto& n : t.nodes)
+    if (n.op == NCA_OP_DIV) n.op = NODE_DIVRAW;
+  return t;
+}
+
+struct MarchGeom {
+  int rz = 0, ry = 0, rx = 0;
+};
+
+static MarchGeom geom_of(const StencilProgram& p) {
+  MarchGeom g;
+  for (auto& o : p.offsets) {
+    g.rz = std::max<int>(g.rz, static_cast<int>(std::llabs(o[0])));
+    g.ry = std::max<int>(g.ry, static_cast<int>(std::llabs(o[1])));
+    g.rx = std::max<int>(g.rx, static_cast<int>(std::llabs(o[2])));
+  }
+  return g;
+}
+
+static std::vector<std::string> names_u(int n) {
+  std::vector<std::string> v;
+  for (int k = 0; k < n; ++k) v.push_back("u" + std::to_string(k));
+  return v;
+}
+static std::vector<std::string> names_s(int n) {
+  std::vector<std::string> v;
+  for (int k = 0; k < n; ++k) v.push_back("s" + std::to_string(k));
+  return v;
+}
+
+struct StencilFuse {
+  bool axis0 = false;
+  int fuse = 0;  // NCA_FUSE_* of global reductions
+  int mode = NCA_ACCUM_PLAIN;
+  bool with_dst = true;
+  int32_t axis0_dtype = NCA_FLOAT32;
+};
+
+// ---- march kernel ---------------------------------------------------------------------------------
+static std::string gen_march(const StencilProgram& p, const StencilFuse& fz, int32_t dst_dtype) {
+  const Tape& t = p.tape;
+  const MarchGeom g = geom_of(p);
+  const std::string C = ctype_of(t.compute_dtype);
+  const std::string S = ctype_of(p.src_dtype);
+  const std::string D = ctype_of(dst_dtype);
+  const int n_arr = static_cast<int>(t.arr_dtypes.size());
+  std::vector<int> alias(n_arr);
+  for (int k = 0; k < n_arr; ++k) alias[k] = k;
+  std::vector<std::string> roots;
+  std::string body = emit_nodes(t, {t.root}, alias, names_u(n_arr), names_s(static_cast<int>(t.scalar_dtypes.size())),
+                                "lin", "t", roots);
+  const int32_t rcd = compute_dtype(dst_dtype);
+  const std::string RC = ctype_of(rcd);
+  const bool comp = fz.mode == NCA_ACCUM_COMPENSATED;
+  std::ostringstream os;
+  os << "#define RZ " << g.rz << "\n#define RY " << g.ry << "\n#define RX " << g.rx << "\n"
+     << "#define TX 64\n#define TY 16\n#define NS (2 * RZ + 2)\n#define LW (TX + 2 * RX)\n#define LH (TY + 2 * RY)\n";
+  os << "extern \"C\" __global__ __launch_bounds__(256) void nca_stencil_march(const " << S << "* __restrict__ src";
+  for (size_t j = 0; j < p.extra_dtypes.size(); ++j) os << ", const " << ctype_of(p.extra_dtypes[j]) << "* __restrict__ e" << j;
+  os << ", " << D << "* __restrict__ dst, nca_i64 NZ, nca_i64 NY, nca_i64 NX, int ntx, int nty, int zlen"
+     << ", double* __restrict__ colpart, void* __restrict__ colout, void* __restrict__ psum, void* __restrict__ pmax"
+     << ", void* __restrict__ pmin, unsigned* __restrict__ counters, int nzc, void* __restrict__ sum_out"
+     << ", void* __restrict__ max_out, void* __restrict__ min_out";
+  for (size_t i = 0; i < t.scalar_dtypes.size(); ++i) os << ", " << C << " s" << i;
+  os << ") {\n";
+  os << "  __shared__ " << C << " lds[NS][LH][LW];\n";
+  // XCD-aware tile order: consecutive tiles on one XCD (blocks are dealt round-robin)
+  os << "  const unsigned nb = gridDim.x, b = blockIdx.x;\n"
+     << "  const unsigned tix = (nb % 8u == 0u) ? (b % 8u) * (nb / 8u) + b / 8u : b;\n"
+     << "  const int txi = tix % ntx, tyi = (tix / ntx) % nty, tzi = tix / (ntx * nty);\n"
+     << "  const nca_i64 x0 = (nca_i64)txi * TX, y0 = (nca_i64)tyi * TY, z0 = (nca_i64)tzi * zlen;\n"
+     << "  const nca_i64 z1 = (z0 + zlen < NZ) ? z0 + zlen : NZ;\n"
+     << "  const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;\n"
+     << "  const nca_i64 gx = x0 + 4 * tx, gy = y0 + ty;\n"
+     << "  const bool valid = (gx < NX) && (gy < NY);\n";
+  // per-thread staging registers for one plane
+  os << "  " << C << " ri[4], rh[4], rc;\n";
+  os << "  const int hid = threadIdx.x;\n"
+     << "  const int nhr = 2 * RY * 16;\n"  // halo-row vectors
+     << "  const int nhc = 2 * RX * LH;\n";  // halo-column scalars
+  // load plane z into registers (zero outside the array)
+  os << "  auto gload = [&](nca_i64 z) {\n"
+     << "    const bool zin = (z >= 0) && (z < NZ);\n"
+     << "    const " << S << "* pl = src + (zin ? z : 0) * NY * NX;\n"
+     << "    // interior: 4 consecutive x of row gy\n"
+     << "    if (zin && gy < NY && gx + 3 < NX) {\n"
+     << "      const nca_vec<" << S << ", 4> v = *reinterpret_cast<const nca_vec<" << S << ", 4>*>(pl + gy * NX + gx);\n"
+     << "#pragma unroll\n      for (int j = 0; j < 4; ++j) ri[j] = nca_cvt<" << C << ", " << S << ">::from(v.v[j]);\n"
+     << "    } else {\n"
+     << "#pragma unroll\n      for (int j = 0; j < 4; ++j) ri[j] = (zin && gy < NY && gx + j < NX) ? nca_ld<" << C << ">(pl + gy * NX + gx + j) : (" << C << ")0;\n"
+     << "    }\n"
+     << "    if (hid < nhr) {\n"
+     << "      const int hr = hid >> 4, hx = hid & 15;\n"
+     << "      const nca_i64 yy = y0 + ((hr < RY) ? (hr - RY) : (TY + hr - RY));\n"
+     << "      const nca_i64 xx = x0 + 4 * hx;\n"
+     << "      if (zin && yy >= 0 && yy < NY && xx + 3 < NX) {\n"
+     << "        const nca_vec<" << S << ", 4> v = *reinterpret_cast<const nca_vec<" << S << ", 4>*>(pl + yy * NX + xx);\n"
+     << "#pragma unroll\n        for (int j = 0; j < 4; ++j) rh[j] = nca_cvt<" << C << ", " << S << ">::from(v.v[j]);\n"
+     << "      } else {\n"
+     << "#pragma unroll\n        for (int j = 0; j < 4; ++j) rh[j] = (zin && yy >= 0 && yy < NY && xx + j < NX) ? nca_ld<" << C << ">(pl + yy * NX + xx + j) : (" << C << ")0;\n"
+     << "      }\n"
+     << "    }\n"
+     << "    if (hid >= nhr && hid < nhr + nhc) {\n"
+     << "      const int q = hid - nhr, hc = q / LH, hrow = q - hc * LH;\n"
+     << "      const nca_i64 xx = x0 + ((hc < RX) ? (hc - RX) : (TX + hc - RX));\n"
+     << "      const nca_i64 yy = y0 + hrow - RY;\n"
+     << "      rc = (zin && yy >= 0 && yy < NY && xx >= 0 && xx < NX) ? nca_ld<" << C << ">(pl + yy * NX + xx) : (" << C << ")0;\n"
+     << "    }\n"
+     << "  };\n";
+  os << "  auto lstore = [&](int slot) {\n"
+     << "#pragma unroll\n    for (int j = 0; j < 4; ++j) lds[slot][ty + RY][RX + 4 * tx + j] = ri[j];\n"
+     << "    if (hid < nhr) {\n"
+     << "      const int hr = hid >> 4, hx = hid & 15;\n"
+     << "      const int row = (hr < RY) ? hr : (TY + hr);\n"
+     << "#pragma unroll\n      for (int j = 0; j < 4; ++j) lds[slot][row][RX + 4 * hx + j] = rh[j];\n"
+     << "    }\n"
+     << "    if (hid >= nhr && hid < nhr + nhc) {\n"
+     << "      const int q = hid - nhr, hc = q / LH, hrow = q - hc * LH;\n"
+     << "      const int col = (hc < RX) ? hc : (TX + hc);\n"
+     << "      lds[slot][hrow][col] = rc;\n"
+     << "    }\n"
+     << "  };\n";
+  os << "  auto slot_of = [](nca_i64 pz) { int s = (int)(pz % NS); return s < 0 ? s + NS : s; };\n";
+  // accumulators
+  if (fz.axis0) {
+    os << "  float cs[4] = {0.f, 0.f, 0.f, 0.f};\n";
+    if (comp) os << "  float cc[4] = {0.f, 0.f, 0.f, 0.f};\n";
+  }
+  if (fz.fuse & NCA_FUSE_SUM) {
+    os << "  " << RC << " acc_s = 0;\n";
+    if (comp) os << "  " << RC << " acc_c = 0;\n";
+  }
+  if (fz.fuse & NCA_FUSE_MAX) os << "  " << RC << " acc_mx = nca_lim<" << RC << ">::lowest();\n";
+  if (fz.fuse & NCA_FUSE_MIN) os << "  " << RC << " acc_mn = nca_lim<" << RC << ">::max();\n";
+  // prologue: planes z0-RZ .. z0+RZ-1
+  os << "  for (nca_i64 pz = z0 - RZ; pz < z0 + RZ; ++pz) { gload(pz); lstore(slot_of(pz)); }\n";
+  os << "  gload(z0 + RZ);\n";
+  os << "  for (nca_i64 z = z0; z < z1; ++z) {\n"
+     << "    lstore(slot_of(z + RZ));\n"
+     << "    __syncthreads();\n"
+     << "    if (z + 1 < z1) gload(z + 1 + RZ);\n";
+  // extras at the output coordinate
+  for (size_t j = 0; j < p.extra_dtypes.size(); ++j) {
+    const std::string E = ctype_of(p.extra_dtypes[j]);
+    os << "    " << C << " ex" << j << "[4];\n"
+       << "    if (valid) { const nca_vec<" << E << ", 4> v = *reinterpret_cast<const nca_vec<" << E << ", 4>*>(e" << j
+       << " + (z * NY + gy) * NX + gx);\n"
+       << "#pragma unroll\n      for (int j = 0; j < 4; ++j) ex" << j << "[j] = nca_cvt<" << C << ", " << E << ">::from(v.v[j]); }\n"
+       << "    else {\n#pragma unroll\n      for (int j = 0; j < 4; ++j) ex" << j << "[j] = (" << C << ")0; }\n";
+  }
+  // slots for each z offset
+  std::vector<int64_t> dzs;
+  for (auto& o : p.offsets)
+    if (std::find(dzs.begin(), dzs.end(), o[0]) == dzs.end()) dzs.push_back(o[0]);
+  for (int64_t dz : dzs) os << "    const int sl" << (dz < 0 ? "m" : "p") << std::llabs(dz) << " = slot_of(z + (" << dz << "));\n";
+  os << "    nca_vec<" << D << ", 4> r;\n";
+  os << "#pragma unroll\n    for (int j = 0; j < 4; ++j) {\n";
+  for (int k = 0; k < p.n_off; ++k) {
+    const auto& o = p.offsets[k];
+    os << "      const " << C << " u" << k << " = lds[sl" << (o[0] < 0 ? "m" : "p") << std::llabs(o[0]) << "][ty + RY + (" << o[1]
+       << ")][RX + 4 * tx + j + (" << o[2] << ")];\n";
+  }
+  for (size_t j = 0; j < p.extra_dtypes.size(); ++j)
+    os << "      const " << C << " u" << (p.n_off + j) << " = ex" << j << "[j];\n";
+  if (t.uses_idx) os << "      const nca_i64 lin = (z * NY + gy) * NX + gx + j;\n";
+  os << body;
+  os << "      const " << D << " st = nca_store_cvt<" << D << ", " << C << ">::to(" << roots[0] << ");\n"
+     << "      r.v[j] = st;\n"
+     << "      if (valid) {\n"
+     << "        const " << RC << " rv = nca_cvt<" << RC << ", " << D << ">::from(st);\n";
+  if (fz.axis0) {
+    if (comp) os << "        nca_neumaier_add(cs[j], cc[j], (float)rv);\n";
+    else os << "        cs[j] += (float)rv;\n";
+  }
+  if (fz.fuse & NCA_FUSE_SUM) {
+    if (comp) os << "        nca_neumaier_add(acc_s, acc_c, rv);\n";
+    else os << "        acc_s += rv;\n";
+  }
+  if (fz.fuse & NCA_FUSE_MAX) os << "        acc_mx = nca_maxop(acc_mx, rv);\n";
+  if (fz.fuse & NCA_FUSE_MIN) os << "        acc_mn = nca_minop(acc_mn, rv);\n";
+  os << "      }\n    }\n";
+  if (fz.with_dst) os << "    if (valid) *reinterpret_cast<nca_vec<" << D << ", 4>*>(dst + (z * NY + gy) * NX + gx) = r;\n";
+  os << "  }\n";
+  // epilogue: column sums.  With several z runs per tile, each run writes float64
+  // partials; the last run of a tile to finish (per-tile ticket) adds them in z order.
+  if (fz.axis0) {
+    const std::string CT = ctype_of(fz.axis0_dtype);
+    os << "  if (nzc == 1) {\n"
+       << "    if (valid) {\n#pragma unroll\n      for (int j = 0; j < 4; ++j) nca_st(reinterpret_cast<" << CT
+       << "*>(colout) + gy * NX + gx + j, (double)cs[j]" << (comp ? " + (double)cc[j]" : "") << ");\n    }\n"
+       << "  } else {\n"
+       << "    if (valid) {\n#pragma unroll\n      for (int j = 0; j < 4; ++j) colpart[((nca_i64)tzi * NY + gy) * NX + gx + j] = (double)cs[j]"
+       << (comp ? " + (double)cc[j]" : "") << ";\n    }\n"
+       << "    if (nca_last_block(counters + (txi + ntx * tyi), (unsigned)nzc)) {\n"
+       << "      if (valid) {\n#pragma unroll\n        for (int j = 0; j < 4; ++j) {\n"
+       << "          double t = 0.0;\n"
+       << "          for (int q = 0; q < nzc; ++q) t += __builtin_nontemporal_load(&colpart[((nca_i64)q * NY + gy) * NX + gx + j]);\n"
+       << "          nca_st(reinterpret_cast<" << CT << "*>(colout) + gy * NX + gx + j, t);\n"
+       << "        }\n      }\n    }\n"
+       << "  }\n";
+  }
+  if (fz.fuse) {
+    const int32_t sum_dt = reduce_result_dtype(NCA_REDUCE_SUM, dst_dtype);
+    os << "  __shared__ double red_sd[4];\n  __shared__ " << RC << " red_mx[4];\n  __shared__ " << RC << " red_mn[4];\n"
+       << "  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;\n";
+    if (fz.fuse & NCA_FUSE_SUM) {
+      if (comp) os << "  nca_wave_comp(acc_s, acc_c);\n  if (lane == 0) red_sd[wid] = (double)acc_s + (double)acc_c;\n";
+      else os << "  acc_s = nca_wave_sum(acc_s);\n  if (lane == 0) red_sd[wid] = (double)acc_s;\n";
+    }
+    if (fz.fuse & NCA_FUSE_MAX) os << "  acc_mx = nca_wave_max(acc_mx);\n  if (lane == 0) red_mx[wid] = acc_mx;\n";
+    if (fz.fuse & NCA_FUSE_MIN) os << "  acc_mn = nca_wave_min(acc_mn);\n  if (lane == 0) red_mn[wid] = acc_mn;\n";
+    os << "  __syncthreads();\n  if (threadIdx.x == 0) {\n";
+    if (fz.fuse & NCA_FUSE_SUM) os << "    ((double*)psum)[b] = red_sd[0] + red_sd[1] + red_sd[2] + red_sd[3];\n";
+    if (fz.fuse & NCA_FUSE_MAX)
+      os << "    " << RC << " m = red_mx[0];\n    for (int w = 1; w < 4; ++w) m = nca_maxop(m, red_mx[w]);\n    ((" << RC
+         << "*)pmax)[b] = m;\n";
+    if (fz.fuse & NCA_FUSE_MIN)
+      os << "    " << RC << " m2 = red_mn[0];\n    for (int w = 1; w < 4; ++w) m2 = nca_minop(m2, red_mn[w]);\n    ((" << RC
+         << "*)pmin)[b] = m2;\n";
+    os << "  }\n";
+    // the last block of the grid combines the per-block partials in block order
+    os << "  if (nca_last_block(counters + ntx * nty, nb)) {\n";
+    if (fz.fuse & NCA_FUSE_SUM)
+      os << "    double ts = 0.0;\n    for (unsigned k = threadIdx.x; k < nb; k += blockDim.x) ts += __builtin_nontemporal_load(&((const double*)psum)[k]);\n"
+         << "    ts = nca_wave_sum(ts);\n    if (lane == 0) red_sd[wid] = ts;\n";
+    if (fz.fuse & NCA_FUSE_MAX)
+      os << "    " << RC << " tm = nca_lim<" << RC << ">::lowest();\n    for (unsigned k = threadIdx.x; k < nb; k += blockDim.x) tm = nca_maxop(tm, __builtin_nontemporal_load(&((const "
+         << RC << "*)pmax)[k]));\n    tm = nca_wave_max(tm);\n    if (lane == 0) red_mx[wid] = tm;\n";
+    if (fz.fuse & NCA_FUSE_MIN)
+      os << "    " << RC << " tn = nca_lim<" << RC << ">::max();\n    for (unsigned k = threadIdx.x; k < nb; k += blockDim.x) tn = nca_minop(tn, __builtin_nontemporal_load(&((const "
+         << RC << "*)pmin)[k]));\n    tn = nca_wave_min(tn);\n    if (lane == 0) red_mn[wid] = tn;\n";
+    os << "    __syncthreads();\n    if (threadIdx.x == 0) {\n";
+    if (fz.fuse & NCA_FUSE_SUM)
+      os << "      nca_st(reinterpret_cast<" << ctype_of(sum_dt) << "*>(sum_out), red_sd[0] + red_sd[1] + red_sd[2] + red_sd[3]);\n";
+    if (fz.fuse & NCA_FUSE_MAX)
+      os << "      " << RC << " m = red_mx[0];\n      for (int w = 1; w < 4; ++w) m = nca_maxop(m, red_mx[w]);\n      nca_st(reinterpret_cast<"
+         << D << "*>(max_out), m);\n";
+    if (fz.fuse & NCA_FUSE_MIN)
+      os << "      " << RC << " m2 = red_mn[0];\n      for (int w = 1; w < 4; ++w) m2 = nca_minop(m2, red_mn[w]);\n      nca_st(reinterpret_cast<"
+         << D << "*>(min_out), m2);\n";
+    os << "    }\n  }\n";
+  }
+  os << "}\n";
+  return os.str();
+}
+
+// ---- direct kernel ----------------------------------------------------------------------------------------
+static std::string gen_direct(const StencilProgram& p, int32_t dst_dtype, bool zero_boundary, int fuse, int mode) {
+  const Tape& t = p.tape;
+  const int nd = p.ndim;
+  const std::string C = ctype_of(t.compute_dtype);
+  const std::string S = ctype_of(p.src_dtype);
+  const std::string D = ctype_of(dst_dtype);
+  const int n_arr = static_cast<int>(t.arr_dtypes.size());
+  std::vector<int> alias(n_arr);
+  for (int k = 0; k < n_arr; ++k) alias[k] = k;
+  std::vector<std::string> roots;
+  std::string body = emit_nodes(t, {t.root}, alias, names_u(n_arr), names_s(static_cast<int>(t.scalar_dtypes.size())),
+                                "lin", "t", roots);
+  const int32_t rcd = compute_dtype(dst_dtype);
+  const std::string RC = ctype_of(rcd);
+  const bool comp = mode == NCA_ACCUM_COMPENSATED;
+  std::ostringstream os;
+  os << "#define NDIM_VAL " << nd << "\nstruct nca_dims { nca_i64 v[NDIM_VAL]; };\n";
+  os << "extern \"C\" __global__ __launch_bounds__(256) void nca_stencil_direct(const void* src_data, nca_dims sshape, nca_dims sstr";
+  for (size_t j = 0; j < p.extra_dtypes.size(); ++j) os << ", const char* __restrict__ e" << j << ", nca_dims est" << j;
+  os << ", void* dest_data, nca_dims dshape, nca_dims dstr, nca_dims step, nca_i64 n";
+  if (fuse) os << ", void* __restrict__ psum, void* __restrict__ pmax, void* __restrict__ pmin";
+  os << p.extra_params;
+  for (size_t i = 0; i < t.scalar_dtypes.size(); ++i) os << ", " << C << " s" << i;
+  os << ") {\n";
+  if (fuse) {
+    if (fuse & NCA_FUSE_SUM) { os << "  " << RC << " acc_s = 0;\n"; if (comp) os << "  " << RC << " acc_c = 0;\n"; }
+    if (fuse & NCA_FUSE_MAX) os << "  " << RC << " acc_mx = nca_lim<" << RC << ">::lowest();\n";
+    if (fuse & NCA_FUSE_MIN) os << "  " << RC << " acc_mn = nca_lim<" << RC << ">::max();\n";
+  }
+  // with hooks: one output per thread (a `return` in the prologue ends the thread, as in
+  // the reference); without: grid-stride loop
+  if (p.has_ext) {
+    os << "  const nca_i64 b_first = (nca_i64)blockIdx.x * blockDim.x + threadIdx.x;\n"
+       << "  for (nca_i64 b_idx = b_first; b_idx < n && b_idx == b_first; ++b_idx) {\n";
+  } else {
+    os << "  const nca_i64 gstride = (nca_i64)gridDim.x * blockDim.x;\n"
+       << "  for (nca_i64 b_idx = (nca_i64)blockIdx.x * blockDim.x + threadIdx.x; b_idx < n; b_idx += gstride) {\n";
+  }
+  os << p.prologue << "\n";
+  os << "    nca_i64 c_out[NDIM_VAL], c_src[NDIM_VAL];\n"
+     << "    { nca_i64 rem = b_idx;\n#pragma unroll\n      for (int d = NDIM_VAL - 1; d >= 0; --d) { c_out[d] = rem % dshape.v[d]; rem /= dshape.v[d]; c_src[d] = c_out[d] * step.v[d]; } }\n";
+  for (int k = 0; k < p.n_off; ++k) {
+    os << "    " << C << " u" << k << ";\n    {\n      nca_i64 off = 0; bool inb = true;\n";
+    for (int d = 0; d < nd; ++d) {
+      os << "      { const nca_i64 c = c_src[" << d << "] + (" << p.offsets[k][d] << "); ";
+      if (zero_boundary) os << "inb = inb && (c >= 0) && (c < sshape.v[" << d << "]); ";
+      os << "off += c * sstr.v[" << d << "]; }\n";
+    }
+    os << "      u" << k << " = inb ? nca_ld<" << C << ">(reinterpret_cast<const " << S
+       << "*>(reinterpret_cast<const char*>(src_data) + off)) : (" << C << ")0;\n    }\n";
+  }
+  for (size_t j = 0; j < p.extra_dtypes.size(); ++j) {
+    os << "    " << C << " u" << (p.n_off + j) << ";\n    { nca_i64 off = 0;\n";
+    for (int d = 0; d < nd; ++d) os << "      off += c_out[" << d << "] * est" << j << ".v[" << d << "];\n";
+    os << "      u" << (p.n_off + j) << " = nca_ld<" << C << ">(reinterpret_cast<const " << ctype_of(p.extra_dtypes[j]) << "*>(e" << j
+       << " + off)); }\n";
+  }
+  if (t.uses_idx) os << "    const nca_i64 lin = b_idx;\n";
+  os << body;
+  os << "    const " << D << " st = nca_store_cvt<" << D << ", " << C << ">::to(" << roots[0] << ");\n";
+  os << "    if (dest_data) { nca_i64 off = 0;\n";
+  for (int d = 0; d < nd; ++d) os << "      off += c_out[" << d << "] * dstr.v[" << d << "];\n";
+  os << "      *reinterpret_cast<" << D << "*>(reinterpret_cast<char*>(dest_data) + off) = st; }\n";
+  if (fuse) {
+    os << "    { const " << RC << " rv = nca_cvt<" << RC << ", " << D << ">::from(st);\n";
+    if (fuse & NCA_FUSE_SUM) os << (comp ? "      nca_neumaier_add(acc_s, acc_c, rv);\n" : "      acc_s += rv;\n");
+    if (fuse & NCA_FUSE_MAX) os << "      acc_mx = nca_maxop(acc_mx, rv);\n";
+    if (fuse & NCA_FUSE_MIN) os << "      acc_mn = nca_minop(acc_mn, rv);\n";
+    os << "    }\n";
+  }
+  os << p.epilogue << "\n";
+  os << "  }\n";
+  if (fuse) {
+    os << "  __shared__ double red_sd[4];\n  __shared__ " << RC << " red_mx[4];\n  __shared__ " << RC << " red_mn[4];\n"
+       << "  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;\n";
+    if (fuse & NCA_FUSE_SUM) {
+      if (comp) os << "  nca_wave_comp(acc_s, acc_c);\n  if (lane == 0) red_sd[wid] = (double)acc_s + (double)acc_c;\n";
+      else os << "  acc_s = nca_wave_sum(acc_s);\n  if (lane == 0) red_sd[wid] = (double)acc_s;\n";
+    }
+    if (fuse & NCA_FUSE_MAX) os << "  acc_mx = nca_wave_max(acc_mx);\n  if (lane == 0) red_mx[wid] = acc_mx;\n";
+    if (fuse & NCA_FUSE_MIN) os << "  acc_mn = nca_wave_min(acc_mn);\n  if (lane == 0) red_mn[wid] = acc_mn;\n";
+    os << "  __syncthreads();\n  if (threadIdx.x == 0) {\n";
+    if (fuse & NCA_FUSE_SUM) os << "    ((double*)psum)[blockIdx.x] = red_sd[0] + red_sd[1] + red_sd[2] + red_sd[3];\n";
+    if (fuse & NCA_FUSE_MAX)
+      os << "    " << RC << " m = red_mx[0];\n    for (int w = 1; w < 4; ++w) m = nca_maxop(m, red_mx[w]);\n    *(" << RC
+         << "*)((char*)pmax + 8 * (size_t)blockIdx.x) = m;\n";
+    if (fuse & NCA_FUSE_MIN)
+      os << "    " << RC << " m2 = red_mn[0];\n    for (int w = 1; w < 4; ++w) m2 = nca_minop(m2, red_mn[w]);\n    *(" << RC
+         << "*)((char*)pmin + 8 * (size_t)blockIdx.x) = m2;\n";
+    os << "  }\n";
+  }
+  os << "}\n";
+  return os.str();
+}
+
+static hipFunction_t stencil_kernel(StencilProgram& p, const std::string& key, const std::function<std::string()>& gen,
+                                    const std::string& kname) {
+  {
+    std::lock_guard<std::mutex> lk(p.mu);
+    auto it = p.kernels.find(key);
+    if (it != p.kernels.end()) return it->second;
+  }
+  std::string src = gen();
+  hipFunction_t fn = jit_get(src, kname);
+  std::lock_guard<std::mutex> lk(p.mu);
+  p.kernels[key] = fn;
+  p.sources[key] = src;
+  return fn;
+}
+
+// ---- apply ------------------------------------------------------------------------------------------------
+static void stencil_apply_impl(uint64_t handle, const nca_view* src, const nca_view* extras, int n_extra,
+                               const nca_scalar* scalars, int n_scalars, const nca_view* dst, int boundary,
+                               void* const* ext_args, int n_ext_args, const nca_view* axis0, int fuse, int mode,
+                               void* sum_out, void* max_out, void* min_out, hipStream_t s) {
+  StencilProgram& p = stencil_program(handle);
+  const Tape& t = p.tape;
+  NCA_REQUIRE(src && dst, "null src/dst");
+  validate_view(*src, "src");
+  const bool with_dst = dst->data != nullptr;
+  validate_view(*dst, "dst", true);
+  NCA_REQUIRE(src->ndim == p.ndim && dst->ndim == p.ndim, "src/dst rank differs from the stencil's NDIM");
+  NCA_REQUIRE(src->dtype == p.src_dtype, std::string("src dtype must be ") + dtype_info(p.src_dtype).name);
+  NCA_REQUIRE(dst->dtype == t.dest_dtype, std::string("dst dtype must be ") + dtype_info(t.dest_dtype).name);
+  NCA_REQUIRE(n_extra == static_cast<int>(p.extra_dtypes.size()), "wrong number of extra arrays");
+  NCA_REQUIRE(boundary == NCA_BOUNDARY_NONE || boundary == NCA_BOUNDARY_ZERO, "bad boundary mode");
+  for (int j = 0; j < n_extra; ++j) {
+    validate_view(extras[j], "extra");
+    NCA_REQUIRE(extras[j].dtype == p.extra_dtypes[j], "extra dtype mismatch");
+    NCA_REQUIRE(same_shape(extras[j], *dst), "extra arrays must have the destination shape");
+  }
+  ScalarSlots sc = make_scalar_slots(t, scalars, n_scalars);
+  int64_t step[NCA_MAX_NDIM];
+  for (int d = 0; d < p.ndim; ++d) {
+    NCA_REQUIRE(dst->shape[d] > 0 || view_size(*dst) == 0, "bad dst shape");
+    step[d] = dst->shape[d] > 0 ? src->shape[d] / dst->shape[d] : 1;
+  }
+  const int64_t n = view_size(*dst);
+  if (boundary == NCA_BOUNDARY_NONE && n > 0) {
+    for (int d = 0; d < p.ndim; ++d)
+      for (auto& o : p.offsets) {
+        const int64_t lo = o[d], hi = (dst->shape[d] - 1) * step[d] + o[d];
+        NCA_REQUIRE(lo >= 0 && hi < src->shape[d],
+                    "stencil reads outside the source along axis " + std::to_string(d) +
+                        " (use zero boundary or pad the source)");
+      }
+  }
+  NCA_REQUIRE(p.has_ext || n_ext_args == 0, "extension arguments given to a stencil without extra_params");
+  DeviceState& dev = current_device();
+  bool unit = true;
+  for (int d = 0; d < p.ndim; ++d) unit = unit && step[d] == 1;
+  MarchGeom g = p.ndim == 3 ? geom_of(p) : MarchGeom{};
+  bool march = p.ndim == 3 && unit && !p.has_ext && g.rz <= 3 && g.ry <= 3 && g.rx <= 3 && is_c_contiguous(*src) &&
+               same_shape(*src, *dst) && (!with_dst || is_c_contiguous(*dst)) && dst->shape[2] % 4 == 0 &&
+               reinterpret_cast<uintptr_t>(src->data) % (4 * itemsize(src->dtype)) == 0 &&
+               (!with_dst || reinterpret_cast<uintptr_t>(dst->data) % (4 * itemsize(dst->dtype)) == 0) &&
+               (itemsize(src->dtype) == 4 || itemsize(src->dtype) == 2);
+  for (int j = 0; j < n_extra && march; ++j)
+    march = is_c_contiguous(extras[j]) && reinterpret_cast<uintptr_t>(extras[j].data) % (4 * itemsize(extras[j].dtype)) == 0;
+  if (axis0) {
+    NCA_REQUIRE(march, "the fused axis-0 sum needs the march kernel: 3-D, step 1, C-contiguous, last extent % 4 == 0");
+    validate_view(*axis0, "axis0_sum");
+    NCA_REQUIRE(axis0->ndim == 2 && axis0->shape[0] == dst->shape[1] && axis0->shape[1] == dst->shape[2] &&
+                    is_c_contiguous(*axis0),
+                "axis0_sum must be C-contiguous with shape dst.shape[1:]");
+    NCA_REQUIRE(axis0->dtype == NCA_FLOAT32 || axis0->dtype == NCA_FLOAT64, "axis0_sum dtype must be float32 or float64");
+  }
+  if (n == 0) return;
+  if (march) {
+    const int64_t NZ = dst->shape[0], NY = dst->shape[1], NX = dst->shape[2];
+    const int ntx = static_cast<int>((NX + 63) / 64), nty = static_cast<int>((NY + 15) / 16);
+    const int64_t tiles = static_cast<int64_t>(ntx) * nty;
+    // z-run length: enough blocks for ~8 per CU, runs of at least 16 planes
+    const int64_t target = static_cast<int64_t>(dev.n_cus) * 8;
+    int64_t nzc = std::max<int64_t>(1, std::min<int64_t>((target + tiles - 1) / tiles, NZ / 16));
+    int zlen = static_cast<int>((NZ + nzc - 1) / nzc);
+    nzc = (NZ + zlen - 1) / zlen;
+    const int64_t blocks = tiles * nzc;
+    NCA_REQUIRE(blocks < (int64_t(1) << 31), "grid too large");
+    StencilFuse fz;
+    fz.axis0 = axis0 != nullptr;
+    fz.fuse = fuse;
+    fz.mode = mode;
+    fz.with_dst = with_dst;
+    fz.axis0_dtype = axis0 ? axis0->dtype : NCA_FLOAT32;
+    std::string key = "m|" + std::to_string(fz.axis0) + std::to_string(fz.axis0_dtype) + "|" + std::to_string(fuse) + "|" +
+                      std::to_string(mode) + "|" + std::to_string(with_dst);
+    hipFunction_t fn = stencil_kernel(p, key, [&]() { return gen_march(p, fz, dst->dtype); }, "nca_stencil_march");
+    size_t colbytes = (axis0 && nzc > 1) ? static_cast<size_t>(nzc * NY * NX) * sizeof(double) : 0;
+    NCA_REQUIRE(tiles + 1 <= kWorkspaceCounters, "too many stencil tiles for the ticket counters");
+    Workspace& w = workspace(s, static_cast<size_t>(blocks) * 24, colbytes);
+    std::deque<const void*> ptrs;
+    std::vector<void*> params;
+    ptrs.push_back(src->data);
+    params.push_back(&ptrs.back());
+    for (int j = 0; j < n_extra; ++j) { ptrs.push_back(extras[j].data); params.push_back(&ptrs.back()); }
+    ptrs.push_back(with_dst ? dst->data : nullptr);
+    params.push_back(&ptrs.back());
+    int64_t dims[3] = {NZ, NY, NX};
+    params.push_back(&dims[0]);
+    params.push_back(&dims[1]);
+    params.push_back(&dims[2]);
+    int ints[3] = {ntx, nty, zlen};
+    params.push_back(&ints[0]);
+    params.push_back(&ints[1]);
+    params.push_back(&ints[2]);
+    void* colpart = (axis0 && nzc > 1) ? w.big : nullptr;
+    void* colout = axis0 ? axis0->data : nullptr;
+    void* psum = w.partials;
+    void* pmax = static_cast<char*>(w.partials) + static_cast<size_t>(blocks) * 8;
+    void* pmin = static_cast<char*>(w.partials) + static_cast<size_t>(blocks) * 16;
+    void* counters = w.counters;
+    int nzc_i = static_cast<int>(nzc);
+    params.push_back(&colpart);
+    params.push_back(&colout);
+    params.push_back(&psum);
+    params.push_back(&pmax);
+    params.push_back(&pmin);
+    params.push_back(&counters);
+    params.push_back(&nzc_i);
+    params.push_back(&sum_out);
+    params.push_back(&max_out);
+    params.push_back(&min_out);
+    for (auto& x : sc.slots) params.push_back(&x);
+    jit_launch(fn, dim3(static_cast<unsigned>(blocks)), dim3(256), 0, s, params);
+    return;
+  }
+  // direct
+  const bool zero = boundary == NCA_BOUNDARY_ZERO;
+  std::string key = "d|" + std::to_string(zero) + "|" + std::to_string(fuse) + "|" + std::to_string(mode);
+  hipFunction_t fn = stencil_kernel(p, key, [&]() { return gen_direct(p, dst->dtype, zero, fuse, mode); }, "nca_stencil_direct");
+  int blocks;
+  if (p.has_ext) {
+    NCA_REQUIRE((n + 255) / 256 < (int64_t(1) << 31), "grid too large");
+    blocks = static_cast<int>((n + 255) / 256);
+    NCA_REQUIRE(!fuse, "fused reductions are not available with extension hooks");
+  } else {
+    blocks = grid_blocks(n, 256, fuse ? dev.n_cus * 4 : dev.n_cus * 16);
+  }
+  Workspace& w = workspace(s, static_cast<size_t>(blocks) * 24);
+  std::deque<const void*> ptrs;
+  std::deque<std::array<int64_t, NCA_MAX_NDIM>> dims;
+  std::vector<void*> params;
+  auto push_dims = [&](const int64_t* v) {
+    std::array<int64_t, NCA_MAX_NDIM> a{};
+    for (int d = 0; d < p.ndim; ++d) a[d] = v[d];
+    dims.push_back(a);
+    params.push_back(dims.back().data());
+  };
+  ptrs.push_back(src->data);
+  params.push_back(&ptrs.back());
+  push_dims(src->shape);
+  push_dims(src->strides);
+  for (int j = 0; j < n_extra; ++j) {
+    ptrs.push_back(extras[j].data);
+    params.push_back(&ptrs.back());
+    push_dims(extras[j].strides);
+  }
+  ptrs.push_back(with_dst ? dst->data : nullptr);
+  params.push_back(&ptrs.back());
+  push_dims(dst->shape);
+  push_dims(dst->strides);
+  push_dims(step);
+  int64_t nn = n;
+  params.push_back(&nn);
+  void* psum = w.partials;
+  void* pmax = static_cast<char*>(w.partials) + static_cast<size_t>(blocks) * 8;
+  void* pmin = static_cast<char*>(w.partials) + static_cast<size_t>(blocks) * 16;
+  if (fuse) {
+    params.push_back(&psum);
+    params.push_back(&pmax);
+    params.push_back(&pmin);
+  }
+  for (int k = 0; k < n_ext_args; ++k) params.push_back(ext_args[k]);
+  for (auto& x : sc.slots) params.push_back(&x);
+  jit_launch(fn, dim3(blocks), dim3(256), 0, s, params);
+  if ((fuse & NCA_FUSE_SUM) && sum_out) launch_finalize_scalar(NCA_FUSE_SUM, dst->dtype, psum, blocks, sum_out, s);
+  if ((fuse & NCA_FUSE_MAX) && max_out) launch_finalize_scalar(NCA_FUSE_MAX, dst->dtype, pmax, blocks, max_out, s);
+  if ((fuse & NCA_FUSE_MIN) && min_out) launch_finalize_scalar(NCA_FUSE_MIN, dst->dtype, pmin, blocks, min_out, s);
+}
+
+}  // namespace nca
+
+using namespace nca;
+
+extern "C" {
+
+int nca_hip_stencil_create(const nca_stencil_desc* desc, uint64_t* handle) {
+  return api_call([&] {
+    NCA_REQUIRE(desc && handle && desc->expr, "null argument");
+    NCA_REQUIRE(desc->ndim >= 1 && desc->ndim <= NCA_MAX_NDIM, "stencil NDIM out of range");
+    NCA_REQUIRE(desc->n_offsets >= 1 && desc->offsets, "stencil needs at least one offset");
+    NCA_REQUIRE(desc->n_extra >= 0 && (desc->n_extra == 0 || desc->extra_dtypes), "bad extra arrays");
+    auto p = std::make_unique<StencilProgram>();
+    p->ndim = desc->ndim;
+    p->n_off = desc->n_offsets;
+    p->src_dtype = desc->src_dtype;
+    NCA_REQUIRE(is_supported(p->src_dtype), "unsupported source dtype");
+    for (int k = 0; k < desc->n_offsets; ++k) {
+      std::array<int64_t, NCA_MAX_NDIM> o{};
+      for (int d = 0; d < desc->ndim; ++d) o[d] = desc->offsets[k * desc->ndim + d];
+      p->offsets.push_back(o);
+    }
+    p->extra_dtypes.assign(desc->extra_dtypes, desc->extra_dtypes + desc->n_extra);
+    NCA_REQUIRE(desc->expr->n_arrays == desc->n_offsets + desc->n_extra,
+                "the expression must load n_offsets + n_extra arrays");
+    for (int k = 0; k < desc->expr->n_arrays; ++k) {
+      int32_t want = k < desc->n_offsets ? desc->src_dtype : desc->extra_dtypes[k - desc->n_offsets];
+      NCA_REQUIRE(desc->expr->arr_dtypes[k] == want, "expression array dtypes disagree with src/extra dtypes");
+    }
+    p->tape = stencil_tape(*desc->expr);
+    p->extra_params = desc->extra_params ? desc->extra_params : "";
+    p->prologue = desc->prologue ? desc->prologue : "";
+    p->epilogue = desc->epilogue ? desc->epilogue : "";
+    p->has_ext = !p->extra_params.empty() || !p->prologue.empty() || !p->epilogue.empty();
+    *handle = stencil_register(std::move(p));
+  });
+}
+
+int nca_hip_stencil_apply(uint64_t handle, const nca_view* src, const nca_view* extras, int32_t n_extra,
+                          const nca_scalar* scalars, int32_t n_scalars, const nca_view* dst, int32_t boundary,
+                          void* const* ext_args, int32_t n_ext_args, void* stream) {
+  return api_call([&] {
+    NCA_REQUIRE(dst && dst->data, "stencil_apply needs a destination");
+    stencil_apply_impl(handle, src, extras, n_extra, scalars, n_scalars, dst, boundary, ext_args, n_ext_args, nullptr,
+                       0, NCA_ACCUM_PLAIN, nullptr, nullptr, nullptr, as_stream(stream));
+    check_launch("stencil");
+    finish_call(as_stream(stream));
+  });
+}
+
+int nca_hip_stencil_apply_reduce(uint64_t handle, const nca_view* src, const nca_view* extras, int32_t n_extra,
+                                 const nca_scalar* scalars, int32_t n_scalars, const nca_view* dst, int32_t boundary,
+                                 const nca_view* axis0_sum, int32_t fuse, int32_t accum_mode, void* sum_out,
+                                 void* max_out, void* min_out, void* stream) {
+  return api_call([&] {
+    NCA_REQUIRE((fuse & ~(NCA_FUSE_SUM | NCA_FUSE_MAX | NCA_FUSE_MIN)) == 0, "bad fuse flags");
+    NCA_REQUIRE(!(fuse & NCA_FUSE_SUM) || sum_out, "sum requested without sum_out");
+    NCA_REQUIRE(!(fuse & NCA_FUSE_MAX) || max_out, "max requested without max_out");
+    NCA_REQUIRE(!(fuse & NCA_FUSE_MIN) || min_out, "min requested without min_out");
+    stencil_apply_impl(handle, src, extras, n_extra, scalars, n_scalars, dst, boundary, nullptr, 0, axis0_sum, fuse,
+                       accum_mode, sum_out, max_out, min_out, as_stream(stream));
+    check_launch("fused stencil");
+    finish_call(as_stream(stream));
+  });
+}
+
+int nca_hip_stencil_adjoint_create(uint64_t stencil_handle, uint64_t* handle) {
+  return api_call([&] {
+    NCA_REQUIRE(handle != nullptr, "null handle");
+    StencilProgram& f = stencil_program(stencil_handle);
+    NCA_REQUIRE(!f.has_ext, "cannot differentiate a stencil with extension hooks");
+    auto p = std::make_unique<StencilProgram>();
+    p->ndim = f.ndim;
+    p->n_off = f.n_off;
+    p->src_dtype = f.tape.dest_dtype;  // the adjoint reads gradients of the destination
+    p->forward = stencil_handle;
+    Tape base = f.tape;
+    std::vector<int> alias(base.arr_dtypes.size());
+    for (size_t k = 0; k < alias.size(); ++k) alias[k] = static_cast<int>(k);
+    // c_k = d f / d v_k must not depend on any load (linear stencil)
+    std::vector<int> coef;
+    for (int k = 0; k < f.n_off; ++k) {
+      int dk = differentiate(base, base.root, k, alias);
+      NCA_REQUIRE(!depends_on_loads(base, dk),
+                  "the stencil is not linear in its source loads; its adjoint is not a stencil");
+      coef.push_back(dk);
+    }
+    // adjoint tape: sum_k c_k * LOAD k  (offsets negated)
+    Tape t = base;
+    t.arr_dtypes.assign(f.n_off, p->src_dtype);
+    int acc = -1;
+    for (int k = 0; k < f.n_off; ++k) {
+      if (coef[k] < 0) continue;
+      Node ld;
+      ld.op = NCA_OP_LOAD_NCARR;
+      ld.index = k;
+      int term = t.bin(NCA_OP_MUL, coef[k], t.add(ld));
+      acc = acc < 0 ? term : t.bin(NCA_OP_ADD, acc, term);
+    }
+    NCA_REQUIRE(acc >= 0, "the stencil does not depend on its source");
+    t.root = acc;
+    t.uses_idx = false;
+    t.dest_dtype = f.src_dtype;
+    t.signature = "adjoint(" + f.tape.signature + ")";
+    p->tape = std::move(t);
+    for (auto& o : f.offsets) {
+      std::array<int64_t, NCA_MAX_NDIM> no{};
+      for (int d = 0; d < f.ndim; ++d) no[d] = -o[d];
+      p->offsets.push_back(no);
+    }
+    *handle = stencil_register(std::move(p));
+  });
+}
+
+int nca_hip_stencil_source(uint64_t handle, int32_t variant, char* buf, size_t n) {
+  std::string src;
+  int rc = api_call([&] {
+    StencilProgram& p = stencil_program(handle);
+    if (variant == 0) {
+      NCA_REQUIRE(p.ndim == 3, "the march kernel is 3-D");
+      StencilFuse fz;
+      src = gen_march(p, fz, p.tape.dest_dtype);
+    } else if (variant == 1) {
+      StencilFuse fz;
+      fz.axis0 = true;
+      fz.fuse = NCA_FUSE_MAX;
+      NCA_REQUIRE(p.ndim == 3, "the march kernel is 3-D");
+      src = gen_march(p, fz, p.tape.dest_dtype);
+    } else {
+      src = gen_direct(p, p.tape.dest_dtype, variant == 3, 0, 0);
+    }
+    src = device_prelude() + "\n" + src;
+  });
+  if (rc != NCA_OK) return -rc;
+  if (buf && n > 0) {
+    size_t k = src.size() < n - 1 ? src.size() : n - 1;
+    std::memcpy(buf, src.data(), k);
+    buf[k] = '\0';
+  }
+  return static_cast<int>(src.size());
+}
+
+}  // extern "C"
