@@ -173,6 +173,13 @@ int nca_hip_memcpy_d2h(void* dst, const void* src, size_t bytes, void* stream);
 int nca_hip_memcpy_d2d(void* dst, const void* src, size_t bytes, void* stream);
 int nca_hip_synchronize(void* stream);
 
+/* Kernel timing: when enabled, every kernel the library launches is bracketed by a pair
+ * of HIP events on its stream.  nca_hip_profile_read synchronises on the recorded events,
+ * writes one "kernel_name milliseconds" line per launch since the last read into buf,
+ * clears the record, and returns the text length (negative status on error). */
+int nca_hip_profile_enable(int32_t on);
+int nca_hip_profile_read(char* buf, size_t n);
+
 /* Compile a complete generated source (as returned by nca_hip_expr_source or
  * nca_hip_stencil_source) with hipRTC for `arch` (NULL: gfx950) without loading it.
  * Needs no GPU.  On failure the compiler log is copied into log. */

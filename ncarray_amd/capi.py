@@ -107,6 +107,8 @@ _SIGS = {
     "nca_hip_memcpy_d2d": [_P, _P, _SZ, _P],
     "nca_hip_synchronize": [_P],
     "nca_hip_rtc_compile": [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p, _SZ],
+    "nca_hip_profile_enable": [_I32],
+    "nca_hip_profile_read": [ctypes.c_char_p, _SZ],
     "nca_hip_reduce_result_dtype": [_I32, _I32, ctypes.POINTER(_I32)],
     "nca_hip_op_result_dtype": [_I32, _I32, ctypes.POINTER(_I32)],
     "nca_hip_expr_compile": [ctypes.POINTER(ExprDesc), ctypes.POINTER(_U64)],
@@ -352,6 +354,26 @@ def init(device=0):
     if not _initialised.get(device):
         call("nca_hip_init", device)
         _initialised[device] = True
+
+
+def profile_enable(on=True):
+    call("nca_hip_profile_enable", 1 if on else 0)
+
+
+def profile_read():
+    """[(kernel name, ms)] for every library kernel launched since the last read."""
+    n = lib.nca_hip_profile_read(None, 0)
+    if n < 0:
+        check(-n)
+    buf = ctypes.create_string_buffer(n + 1)
+    n2 = lib.nca_hip_profile_read(buf, len(buf))
+    if n2 < 0:
+        check(-n2)
+    out = []
+    for line in buf.value.decode().splitlines():
+        name, ms = line.rsplit(" ", 1)
+        out.append((name, float(ms)))
+    return out
 
 
 def reduce_result_dtype(op, src_dtype):

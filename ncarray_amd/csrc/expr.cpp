@@ -492,12 +492,24 @@ void expr_launch_impl(uint64_t handle, const nca_view* ins, int n_in, const nca_
     la.params.push_back(&psum);
     la.params.push_back(&pmax);
     la.params.push_back(&pmin);
-    jit_launch(fn, dim3(blocks), dim3(256), 0, s, la.params);
-    if ((fuse & NCA_FUSE_SUM) && sum_out) launch_finalize_scalar(NCA_FUSE_SUM, out.dtype, psum, blocks, sum_out, s);
-    if ((fuse & NCA_FUSE_MAX) && max_out) launch_finalize_scalar(NCA_FUSE_MAX, out.dtype, pmax, blocks, max_out, s);
-    if ((fuse & NCA_FUSE_MIN) && min_out) launch_finalize_scalar(NCA_FUSE_MIN, out.dtype, pmin, blocks, min_out, s);
+    {
+      ProfScope ps("nca_expr_reduce", s);
+      jit_launch(fn, dim3(blocks), dim3(256), 0, s, la.params);
+    }
+    FinalizeArgs fa;
+    fa.value_dtype = out.dtype;
+    fa.fuse = fuse;
+    fa.psum = psum;
+    fa.pmax = pmax;
+    fa.pmin = pmin;
+    fa.nblocks = blocks;
+    fa.sum_out = sum_out;
+    fa.max_out = max_out;
+    fa.min_out = min_out;
+    launch_finalize_fused(fa, s);
   } else {
     if (ops.n == 0) return;
+    ProfScope ps(strided ? "nca_expr_strided" : "nca_expr_contig", s);
     jit_launch(fn, dim3(blocks), dim3(256), 0, s, la.params);
   }
 }

@@ -136,6 +136,7 @@ __device__ __forceinline__ void nca_wave_comp(F& s, F& c) {
 // max/min with the host fold's rule: a candidate replaces the running value only when
 // it is strictly greater (less); NaN never wins (host/reductions.hh:130-146).
 template <typename T> __device__ __forceinline__ T nca_maxop(T a, T b) { return (b > a) ? b : a; }
+template <typename T> __device__ __forceinline__ T nca_add2(T a, T b) { return a + b; }
 template <typename T> __device__ __forceinline__ T nca_minop(T a, T b) { return (b < a) ? b : a; }
 template <typename T>
 __device__ __forceinline__ T nca_wave_max(T v) {
@@ -171,24 +172,33 @@ __device__ __forceinline__ void nca_wave_arg(T& v, nca_i64& i) {
   }
 }
 
-// Grid-completion ticket for the "last block finishes" pattern: returns true in the
-// block that arrives last; that block also re-arms the counter for the next launch.
-__device__ __forceinline__ bool nca_last_block(unsigned* counter, unsigned nblocks) {
-  __shared__ unsigned nca_ticket;
-  __threadfence();
-  __syncthreads();
-  if (threadIdx.x == 0) nca_ticket = atomicAdd(counter, 1u);
-  __syncthreads();
-  bool last = (nca_ticket == nblocks - 1u);
-  if (last) {
-    __threadfence();
-    if (threadIdx.x == 0) atomicExch(counter, 0u);
-  }
-  return last;
-}
-
 // 128-bit vector of N elements of T (N * sizeof(T) == 16 or smaller power of two).
 template <typename T, int N>
 struct __attribute__((aligned(sizeof(T) * N))) nca_vec { T v[N]; };
+
+typedef unsigned int nca_u32x4 __attribute__((ext_vector_type(4)));
+typedef unsigned int nca_u32x2 __attribute__((ext_vector_type(2)));
+
+// Streaming (non-temporal) store of a whole vector: the data is not re-read by this
+// kernel, so it should not displace cached inputs.
+template <typename T, int N>
+__device__ __forceinline__ void nca_store_nt(nca_vec<T, N>* p, const nca_vec<T, N>& v) {
+  constexpr int B = static_cast<int>(sizeof(T)) * N;
+  if constexpr (B == 16) {
+    nca_u32x4 w;
+    __builtin_memcpy(&w, &v, 16);
+    __builtin_nontemporal_store(w, reinterpret_cast<nca_u32x4*>(p));
+  } else if constexpr (B == 8) {
+    nca_u32x2 w;
+    __builtin_memcpy(&w, &v, 8);
+    __builtin_nontemporal_store(w, reinterpret_cast<nca_u32x2*>(p));
+  } else if constexpr (B == 4) {
+    unsigned w;
+    __builtin_memcpy(&w, &v, 4);
+    __builtin_nontemporal_store(w, reinterpret_cast<unsigned*>(p));
+  } else {
+    *p = v;
+  }
+}
 
 #endif  // NCA_DEVICE_HH

@@ -126,7 +126,38 @@ std::string hiprtc_options_string();
 void jit_launch(hipFunction_t fn, dim3 grid, dim3 block, size_t shmem, hipStream_t s,
                 std::vector<void*>& params);
 
+// ---- profiling -------------------------------------------------------------------------
+// When enabled (nca_hip_profile_enable), each library kernel launch is bracketed by a pair
+// of HIP events on its stream; nca_hip_profile_read reports the durations.
+struct ProfScope {
+  ProfScope(const char* name, hipStream_t s);
+  ~ProfScope();
+  const char* name;
+  hipStream_t stream;
+  void* start = nullptr;
+};
+
 // ---- AOT kernels (reduce.hip) used by the JIT paths --------------------------------------
+// One launch that finishes fused reductions: column sums from float64 z-run partials
+// (colpart[nzc][K] -> colout, when K > 0) and the global sum/max/min from per-block
+// partials (8-byte slots: float64 sums, compute-type extrema).
+struct FinalizeArgs {
+  const double* colpart = nullptr;
+  int nzc = 0;
+  int64_t K = 0;
+  void* colout = nullptr;
+  int32_t col_dtype = NCA_FLOAT64;
+  int32_t value_dtype = NCA_FLOAT32;
+  int fuse = 0;
+  const void* psum = nullptr;
+  const void* pmax = nullptr;
+  const void* pmin = nullptr;
+  int nblocks = 0;
+  void* sum_out = nullptr;
+  void* max_out = nullptr;
+  void* min_out = nullptr;
+};
+void launch_finalize_fused(const FinalizeArgs& a, hipStream_t s);
 // Combine `nblocks` per-block partials of a fused scalar reduction into *out.
 // kind: NCA_FUSE_SUM / MAX / MIN; dtype: dtype of the reduced values (sum partials
 // are float64 for floating values, the integer sum type otherwise).

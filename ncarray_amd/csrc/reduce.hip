@@ -4,8 +4,9 @@
 // deterministic, atomics-free two-level reductions:
 //   * full: a grid of at most 4 blocks/CU streams the input with 16-byte loads, each
 //     thread folds its elements, 64-lane butterflies + LDS combine a block, every block
-//     writes one partial, and the last block to finish combines the partials in a fixed
-//     order (float64 for floating-point sums).
+//     writes one partial, and a one-block kernel combines the partials in block order
+//     (float64 for floating-point sums).  No grid-wide fences: on gfx950 a device-scope
+//     fence writes back the issuing XCD's L2, which costs more than a second launch.
 //   * axes: the input is viewed as [outer kept][reduced][inner kept] when it can be;
 //     "column" kernels give each thread a 16-byte group of inner outputs and march the
 //     reduced axis (split into S chunks with a fixed-order combine when the outputs alone
@@ -302,7 +303,13 @@ __global__ __launch_bounds__(256) void k_reduce_full(const char* __restrict__ ba
   __shared__ Acc<DT, OP, COMP> lds[16];
   block_combine(a, lds);
   if (threadIdx.x == 0) a.to_slot(partials[blockIdx.x]);
-  if (nca_last_block(counter, gridDim.x)) combine_slots_and_store<DT, OP>(partials, gridDim.x, n, out);
+}
+
+// One block folds the per-block partials of k_reduce_full in block order.
+template <int DT, int OP>
+__global__ __launch_bounds__(256) void k_reduce_full_combine(const Slot* __restrict__ partials, int nblocks, nca_i64 n,
+                                                             void* __restrict__ out) {
+  combine_slots_and_store<DT, OP>(partials, nblocks, n, out);
 }
 
 // ---- axis reductions -------------------------------------------------------------------------------------
@@ -522,7 +529,7 @@ __global__ __launch_bounds__(256) void k_reduce_generic(const char* __restrict__
 // Fused kernels write one 8-byte slot per block: float64 sums for floating values, the
 // 64-bit integer sum otherwise, and the value (compute type) for max/min.
 template <int DT, int KIND>
-__global__ __launch_bounds__(256) void k_finalize_scalar(const void* __restrict__ partials, int n, void* __restrict__ out) {
+__device__ void finalize_scalar_body(const void* __restrict__ partials, int n, void* __restrict__ out) {
   typedef typename RT<DT>::V V;
   const int tid = threadIdx.x;
   __shared__ double ld[16];
@@ -569,6 +576,35 @@ __global__ __launch_bounds__(256) void k_finalize_scalar(const void* __restrict_
       nca_st(reinterpret_cast<typename RT<DT>::T*>(out), m);
     }
   }
+}
+
+template <int DT, int KIND>
+__global__ __launch_bounds__(256) void k_finalize_scalar(const void* __restrict__ partials, int n, void* __restrict__ out) {
+  finalize_scalar_body<DT, KIND>(partials, n, out);
+}
+
+// Column sums from z-run partials (blocks [0, ceil(K/256))) and the global reductions
+// (the last block): one launch for everything a fused stencil/expression leaves over.
+template <int DT>
+__global__ __launch_bounds__(256) void k_finalize_fused(const double* __restrict__ colpart, int nzc, nca_i64 K,
+                                                        void* __restrict__ colout, int col_f64, int fuse,
+                                                        const void* __restrict__ psum, const void* __restrict__ pmax,
+                                                        const void* __restrict__ pmin, int nblocks, void* sum_out,
+                                                        void* max_out, void* min_out) {
+  const nca_i64 ncb = (K + 255) / 256;
+  if ((nca_i64)blockIdx.x < ncb) {
+    const nca_i64 k = (nca_i64)blockIdx.x * 256 + threadIdx.x;
+    if (k < K) {
+      double t = 0.0;
+      for (int q = 0; q < nzc; ++q) t += colpart[(nca_i64)q * K + k];
+      if (col_f64) reinterpret_cast<double*>(colout)[k] = t;
+      else reinterpret_cast<float*>(colout)[k] = static_cast<float>(t);
+    }
+    return;
+  }
+  if ((fuse & NCA_FUSE_SUM) && sum_out) finalize_scalar_body<DT, NCA_FUSE_SUM>(psum, nblocks, sum_out);
+  if ((fuse & NCA_FUSE_MAX) && max_out) finalize_scalar_body<DT, NCA_FUSE_MAX>(pmax, nblocks, max_out);
+  if ((fuse & NCA_FUSE_MIN) && min_out) finalize_scalar_body<DT, NCA_FUSE_MIN>(pmin, nblocks, min_out);
 }
 
 template <typename OutT>
@@ -654,12 +690,18 @@ template <int DT> struct RunFull {
       constexpr int OP = decltype(opc)::value;
       with_comp<DT, OP>(a.mode, [&](auto cc) {
         constexpr bool COMP = decltype(cc)::value;
-        if (contig)
-          hipLaunchKernelGGL((k_reduce_full<DT, OP, COMP, true>), dim3(blocks), dim3(256), 0, a.s,
-                             (const char*)a.in->data, d, (nca_i64)n, (Slot*)w.partials, w.counters, a.out);
-        else
-          hipLaunchKernelGGL((k_reduce_full<DT, OP, COMP, false>), dim3(blocks), dim3(256), 0, a.s,
-                             (const char*)a.in->data, d, (nca_i64)n, (Slot*)w.partials, w.counters, a.out);
+        {
+          ProfScope ps("nca_reduce_full", a.s);
+          if (contig)
+            hipLaunchKernelGGL((k_reduce_full<DT, OP, COMP, true>), dim3(blocks), dim3(256), 0, a.s,
+                               (const char*)a.in->data, d, (nca_i64)n, (Slot*)w.partials, w.counters, a.out);
+          else
+            hipLaunchKernelGGL((k_reduce_full<DT, OP, COMP, false>), dim3(blocks), dim3(256), 0, a.s,
+                               (const char*)a.in->data, d, (nca_i64)n, (Slot*)w.partials, w.counters, a.out);
+        }
+        ProfScope ps2("nca_reduce_full_combine", a.s);
+        hipLaunchKernelGGL((k_reduce_full_combine<DT, OP>), dim3(1), dim3(256), 0, a.s, (const Slot*)w.partials,
+                           blocks, (nca_i64)n, a.out);
       });
     });
   }
@@ -704,6 +746,7 @@ template <int DT> struct RunAxes {
       with_comp<DT, OP>(a.mode, [&](auto cc) {
         constexpr bool COMP = decltype(cc)::value;
         if (nout == 0) return;
+        ProfScope ps("nca_reduce_axes", a.s);
         if (fast && KI > 1 && KI % E == 0 && reinterpret_cast<uintptr_t>(in.data) % 16 == 0 && R > 0) {
           const int64_t threads = KO * (KI / E);
           int S = 1;
@@ -756,6 +799,25 @@ template <int DT> struct RunFinalizeScalar {
 void launch_finalize_scalar(int kind, int32_t value_dtype, const void* partials, int nblocks, void* out,
                             hipStream_t s) {
   dispatch_dtype<RunFinalizeScalar>(value_dtype, kind, partials, nblocks, out, s);
+}
+
+template <int DT> struct RunFinalizeFused {
+  static void run(const FinalizeArgs& a, hipStream_t s) {
+    const int64_t ncb = (a.K + 255) / 256;
+    const bool scal = a.fuse != 0;
+    const unsigned grid = static_cast<unsigned>(ncb + (scal ? 1 : 0));
+    if (grid == 0) return;
+    ProfScope ps("nca_finalize_fused", s);
+    hipLaunchKernelGGL((k_finalize_fused<DT>), dim3(grid), dim3(256), 0, s, a.colpart, a.nzc, (nca_i64)a.K, a.colout,
+                       a.col_dtype == NCA_FLOAT64 ? 1 : 0, a.fuse, a.psum, a.pmax, a.pmin, a.nblocks, a.sum_out,
+                       a.max_out, a.min_out);
+  }
+};
+
+void launch_finalize_fused(const FinalizeArgs& a, hipStream_t s) {
+  NCA_REQUIRE(a.K == 0 || a.col_dtype == NCA_FLOAT32 || a.col_dtype == NCA_FLOAT64,
+              "column sums are written as float32 or float64");
+  dispatch_dtype<RunFinalizeFused>(a.value_dtype, a, s);
 }
 
 void launch_finalize_columns(const double* partials, int S, int64_t K, void* out, int32_t out_dtype,

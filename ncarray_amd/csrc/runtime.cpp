@@ -99,6 +99,35 @@ void finish_call(hipStream_t s) {
   if (s == nullptr) HIP_CHECK(hipStreamSynchronize(nullptr));
 }
 
+// ---- profiling ---------------------------------------------------------------------------
+namespace {
+std::mutex g_prof_mu;
+bool g_prof_on = false;
+struct ProfRecord {
+  std::string name;
+  hipEvent_t a, b;
+};
+std::vector<ProfRecord> g_prof;
+}  // namespace
+
+ProfScope::ProfScope(const char* n, hipStream_t s) : name(n), stream(s) {
+  std::lock_guard<std::mutex> lk(g_prof_mu);
+  if (!g_prof_on) return;
+  hipEvent_t e;
+  if (hipEventCreate(&e) != hipSuccess) return;
+  if (hipEventRecord(e, s) != hipSuccess) { (void)hipEventDestroy(e); return; }
+  start = e;
+}
+
+ProfScope::~ProfScope() {
+  if (!start) return;
+  hipEvent_t e;
+  if (hipEventCreate(&e) != hipSuccess) return;
+  (void)hipEventRecord(e, stream);
+  std::lock_guard<std::mutex> lk(g_prof_mu);
+  g_prof.push_back({name, static_cast<hipEvent_t>(start), e});
+}
+
 void check_launch(const char* what) {
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) NCA_THROW(NCA_ERR_HIP, std::string(what) + " launch failed: " + hipGetErrorString(e));
@@ -258,6 +287,46 @@ int nca_hip_memcpy_d2d(void* dst, const void* src, size_t bytes, void* stream) {
 
 int nca_hip_synchronize(void* stream) {
   return api_call([&] { HIP_CHECK(hipStreamSynchronize(as_stream(stream))); });
+}
+
+int nca_hip_profile_enable(int32_t on) {
+  return api_call([&] {
+    std::lock_guard<std::mutex> lk(g_prof_mu);
+    g_prof_on = on != 0;
+  });
+}
+
+int nca_hip_profile_read(char* buf, size_t n) {
+  // Completed records are turned into text and kept until a buffer large enough to hold
+  // all of it is supplied; a NULL or short buffer only reports the needed length.
+  static std::string pending;
+  int rc = api_call([&] {
+    std::vector<ProfRecord> recs;
+    {
+      std::lock_guard<std::mutex> lk(g_prof_mu);
+      recs.swap(g_prof);
+    }
+    for (auto& r : recs) {
+      HIP_CHECK(hipEventSynchronize(r.b));
+      float ms = 0.f;
+      HIP_CHECK(hipEventElapsedTime(&ms, r.a, r.b));
+      char line[160];
+      std::snprintf(line, sizeof line, "%s %.6f\n", r.name.c_str(), ms);
+      std::lock_guard<std::mutex> lk(g_prof_mu);
+      pending += line;
+      (void)hipEventDestroy(r.a);
+      (void)hipEventDestroy(r.b);
+    }
+  });
+  if (rc != NCA_OK) return -rc;
+  std::lock_guard<std::mutex> lk(g_prof_mu);
+  const int len = static_cast<int>(pending.size());
+  if (buf && n > pending.size()) {
+    std::memcpy(buf, pending.data(), pending.size());
+    buf[pending.size()] = '\0';
+    pending.clear();
+  }
+  return len;
 }
 
 int nca_hip_reduce_result_dtype(int32_t op, int32_t src_dtype, int32_t* out_dtype) {

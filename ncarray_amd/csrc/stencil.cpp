@@ -70,6 +70,9 @@ static Tape stencil_tape(const nca_expr_desc& d) {
   return t;
 }
 
+// elements per thread along x in the march kernel: one 16-byte load of the source
+static int march_vx(int32_t src_dtype) { return itemsize(src_dtype) == 2 ? 8 : 4; }
+
 struct MarchGeom {
   int rz = 0, ry = 0, rx = 0;
 };
@@ -121,13 +124,12 @@ static std::string gen_march(const StencilProgram& p, const StencilFuse& fz, int
   const bool comp = fz.mode == NCA_ACCUM_COMPENSATED;
   std::ostringstream os;
   os << "#define RZ " << g.rz << "\n#define RY " << g.ry << "\n#define RX " << g.rx << "\n"
-     << "#define TX 64\n#define TY 16\n#define NS (2 * RZ + 2)\n#define LW (TX + 2 * RX)\n#define LH (TY + 2 * RY)\n";
+     << "#define VX " << march_vx(p.src_dtype) << "\n#define TX (16 * VX)\n#define TY 16\n#define NS (2 * RZ + 2)\n#define LW (TX + 2 * RX)\n#define LH (TY + 2 * RY)\n";
   os << "extern \"C\" __global__ __launch_bounds__(256) void nca_stencil_march(const " << S << "* __restrict__ src";
   for (size_t j = 0; j < p.extra_dtypes.size(); ++j) os << ", const " << ctype_of(p.extra_dtypes[j]) << "* __restrict__ e" << j;
   os << ", " << D << "* __restrict__ dst, nca_i64 NZ, nca_i64 NY, nca_i64 NX, int ntx, int nty, int zlen"
      << ", double* __restrict__ colpart, void* __restrict__ colout, void* __restrict__ psum, void* __restrict__ pmax"
-     << ", void* __restrict__ pmin, unsigned* __restrict__ counters, int nzc, void* __restrict__ sum_out"
-     << ", void* __restrict__ max_out, void* __restrict__ min_out";
+     << ", void* __restrict__ pmin";
   for (size_t i = 0; i < t.scalar_dtypes.size(); ++i) os << ", " << C << " s" << i;
   os << ") {\n";
   os << "  __shared__ " << C << " lds[NS][LH][LW];\n";
@@ -138,48 +140,64 @@ static std::string gen_march(const StencilProgram& p, const StencilFuse& fz, int
      << "  const nca_i64 x0 = (nca_i64)txi * TX, y0 = (nca_i64)tyi * TY, z0 = (nca_i64)tzi * zlen;\n"
      << "  const nca_i64 z1 = (z0 + zlen < NZ) ? z0 + zlen : NZ;\n"
      << "  const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;\n"
-     << "  const nca_i64 gx = x0 + 4 * tx, gy = y0 + ty;\n"
-     << "  const bool valid = (gx < NX) && (gy < NY);\n";
-  // per-thread staging registers for one plane
-  os << "  " << C << " ri[4], rh[4], rc;\n";
+     << "  const nca_i64 gx = x0 + VX * tx, gy = y0 + ty;\n"
+     << "  const bool valid = (gx < NX) && (gy < NY);\n"
+     << "  const nca_i64 plane = NY * NX;\n  const int NZi = (int)NZ;\n";
+  // per-thread staging registers: two planes in flight (sets A and B)
+  os << "  " << C << " riA[VX], rhA[VX], rcA, riB[VX], rhB[VX], rcB;\n";
   os << "  const int hid = threadIdx.x;\n"
      << "  const int nhr = 2 * RY * 16;\n"  // halo-row vectors
      << "  const int nhc = 2 * RX * LH;\n";  // halo-column scalars
+  // per-thread constants of the plane loads: 32-bit offsets within a plane (the host
+  // requires NY*NX < 2^31) and the in-range tests that do not depend on z
+  os << "  const unsigned ioff = (unsigned)(gy * NX + gx);\n"
+     << "  const bool ifull = (gy < NY) && (gx + VX - 1 < NX);\n"
+     << "  bool hrin = false, hrfull = false;\n  unsigned hroff = 0u;\n  nca_i64 hrx = 0;\n"
+     << "  if (hid < nhr) {\n"
+     << "    const int hr = hid >> 4, hx = hid & 15;\n"
+     << "    const nca_i64 yy = y0 + ((hr < RY) ? (hr - RY) : (TY + hr - RY));\n"
+     << "    hrx = x0 + VX * hx;\n"
+     << "    hrin = (yy >= 0) && (yy < NY);\n"
+     << "    hrfull = hrin && (hrx + VX - 1 < NX);\n"
+     << "    hroff = hrin ? (unsigned)(yy * NX + hrx) : 0u;\n"
+     << "  }\n"
+     << "  bool hcin = false;\n  unsigned hcoff = 0u;\n"
+     << "  if (hid >= nhr && hid < nhr + nhc) {\n"
+     << "    const int q = hid - nhr, hc = q / LH, hrow = q - hc * LH;\n"
+     << "    const nca_i64 xx = x0 + ((hc < RX) ? (hc - RX) : (TX + hc - RX));\n"
+     << "    const nca_i64 yy = y0 + hrow - RY;\n"
+     << "    hcin = (yy >= 0) && (yy < NY) && (xx >= 0) && (xx < NX);\n"
+     << "    hcoff = hcin ? (unsigned)(yy * NX + xx) : 0u;\n"
+     << "  }\n";
   // load plane z into registers (zero outside the array)
-  os << "  auto gload = [&](nca_i64 z) {\n"
-     << "    const bool zin = (z >= 0) && (z < NZ);\n"
-     << "    const " << S << "* pl = src + (zin ? z : 0) * NY * NX;\n"
-     << "    // interior: 4 consecutive x of row gy\n"
-     << "    if (zin && gy < NY && gx + 3 < NX) {\n"
-     << "      const nca_vec<" << S << ", 4> v = *reinterpret_cast<const nca_vec<" << S << ", 4>*>(pl + gy * NX + gx);\n"
-     << "#pragma unroll\n      for (int j = 0; j < 4; ++j) ri[j] = nca_cvt<" << C << ", " << S << ">::from(v.v[j]);\n"
-     << "    } else {\n"
-     << "#pragma unroll\n      for (int j = 0; j < 4; ++j) ri[j] = (zin && gy < NY && gx + j < NX) ? nca_ld<" << C << ">(pl + gy * NX + gx + j) : (" << C << ")0;\n"
+  // Loads are branch-free where possible: an out-of-range load reads a safe address
+  // (the first source element) and its value is replaced by zero.  Rows are all-or-nothing
+  // because NX is a multiple of VX.
+  os << "  auto gload = [&](int z, " << C << " (&ri)[VX], " << C << " (&rh)[VX], " << C << "& rc) {\n"
+     << "    const bool zin = (unsigned)z < (unsigned)NZi;\n"
+     << "    const " << S << "* __restrict__ pl = src + (size_t)(zin ? z : 0) * plane;\n"
+     << "    {\n"
+     << "      const bool ok = zin && ifull;\n"
+     << "      const nca_vec<" << S << ", VX> v = *reinterpret_cast<const nca_vec<" << S << ", VX>*>(ok ? pl + ioff : src);\n"
+     << "#pragma unroll\n      for (int j = 0; j < VX; ++j) ri[j] = ok ? nca_cvt<" << C << ", " << S << ">::from(v.v[j]) : (" << C << ")0;\n"
      << "    }\n"
      << "    if (hid < nhr) {\n"
-     << "      const int hr = hid >> 4, hx = hid & 15;\n"
-     << "      const nca_i64 yy = y0 + ((hr < RY) ? (hr - RY) : (TY + hr - RY));\n"
-     << "      const nca_i64 xx = x0 + 4 * hx;\n"
-     << "      if (zin && yy >= 0 && yy < NY && xx + 3 < NX) {\n"
-     << "        const nca_vec<" << S << ", 4> v = *reinterpret_cast<const nca_vec<" << S << ", 4>*>(pl + yy * NX + xx);\n"
-     << "#pragma unroll\n        for (int j = 0; j < 4; ++j) rh[j] = nca_cvt<" << C << ", " << S << ">::from(v.v[j]);\n"
-     << "      } else {\n"
-     << "#pragma unroll\n        for (int j = 0; j < 4; ++j) rh[j] = (zin && yy >= 0 && yy < NY && xx + j < NX) ? nca_ld<" << C << ">(pl + yy * NX + xx + j) : (" << C << ")0;\n"
-     << "      }\n"
+     << "      const bool ok = zin && hrfull;\n"
+     << "      const nca_vec<" << S << ", VX> v = *reinterpret_cast<const nca_vec<" << S << ", VX>*>(ok ? pl + hroff : src);\n"
+     << "#pragma unroll\n      for (int j = 0; j < VX; ++j) rh[j] = ok ? nca_cvt<" << C << ", " << S << ">::from(v.v[j]) : (" << C << ")0;\n"
      << "    }\n"
      << "    if (hid >= nhr && hid < nhr + nhc) {\n"
-     << "      const int q = hid - nhr, hc = q / LH, hrow = q - hc * LH;\n"
-     << "      const nca_i64 xx = x0 + ((hc < RX) ? (hc - RX) : (TX + hc - RX));\n"
-     << "      const nca_i64 yy = y0 + hrow - RY;\n"
-     << "      rc = (zin && yy >= 0 && yy < NY && xx >= 0 && xx < NX) ? nca_ld<" << C << ">(pl + yy * NX + xx) : (" << C << ")0;\n"
+     << "      const bool ok = zin && hcin;\n"
+     << "      const " << C << " v = nca_ld<" << C << ">(ok ? pl + hcoff : src);\n"
+     << "      rc = ok ? v : (" << C << ")0;\n"
      << "    }\n"
      << "  };\n";
-  os << "  auto lstore = [&](int slot) {\n"
-     << "#pragma unroll\n    for (int j = 0; j < 4; ++j) lds[slot][ty + RY][RX + 4 * tx + j] = ri[j];\n"
+  os << "  auto lstore = [&](int slot, const " << C << " (&ri)[VX], const " << C << " (&rh)[VX], const " << C << "& rc) {\n"
+     << "#pragma unroll\n    for (int j = 0; j < VX; ++j) lds[slot][ty + RY][RX + VX * tx + j] = ri[j];\n"
      << "    if (hid < nhr) {\n"
      << "      const int hr = hid >> 4, hx = hid & 15;\n"
      << "      const int row = (hr < RY) ? hr : (TY + hr);\n"
-     << "#pragma unroll\n      for (int j = 0; j < 4; ++j) lds[slot][row][RX + 4 * hx + j] = rh[j];\n"
+     << "#pragma unroll\n      for (int j = 0; j < VX; ++j) lds[slot][row][RX + VX * hx + j] = rh[j];\n"
      << "    }\n"
      << "    if (hid >= nhr && hid < nhr + nhc) {\n"
      << "      const int q = hid - nhr, hc = q / LH, hrow = q - hc * LH;\n"
@@ -187,11 +205,12 @@ static std::string gen_march(const StencilProgram& p, const StencilFuse& fz, int
      << "      lds[slot][hrow][col] = rc;\n"
      << "    }\n"
      << "  };\n";
-  os << "  auto slot_of = [](nca_i64 pz) { int s = (int)(pz % NS); return s < 0 ? s + NS : s; };\n";
+  // ring slots: plane z0 - RZ + k lives in slot k % NS; slots advance by one per plane
+  os << "  auto wrap = [](int s) { return s < 0 ? s + NS : (s >= NS ? s - NS : s); };\n";
   // accumulators
   if (fz.axis0) {
-    os << "  float cs[4] = {0.f, 0.f, 0.f, 0.f};\n";
-    if (comp) os << "  float cc[4] = {0.f, 0.f, 0.f, 0.f};\n";
+    os << "  float cs[VX] = {};\n";
+    if (comp) os << "  float cc[VX] = {};\n";
   }
   if (fz.fuse & NCA_FUSE_SUM) {
     os << "  " << RC << " acc_s = 0;\n";
@@ -199,75 +218,104 @@ static std::string gen_march(const StencilProgram& p, const StencilFuse& fz, int
   }
   if (fz.fuse & NCA_FUSE_MAX) os << "  " << RC << " acc_mx = nca_lim<" << RC << ">::lowest();\n";
   if (fz.fuse & NCA_FUSE_MIN) os << "  " << RC << " acc_mn = nca_lim<" << RC << ">::max();\n";
-  // prologue: planes z0-RZ .. z0+RZ-1
-  os << "  for (nca_i64 pz = z0 - RZ; pz < z0 + RZ; ++pz) { gload(pz); lstore(slot_of(pz)); }\n";
-  os << "  gload(z0 + RZ);\n";
-  os << "  for (nca_i64 z = z0; z < z1; ++z) {\n"
-     << "    lstore(slot_of(z + RZ));\n"
-     << "    __syncthreads();\n"
-     << "    if (z + 1 < z1) gload(z + 1 + RZ);\n";
+  // prologue: planes z0-RZ .. z0+RZ-1 straight to LDS, planes z0+RZ and z0+RZ+1 in flight
+  os << "  const int zb = (int)z0, ze = (int)z1;\n";
+  os << "#pragma unroll\n  for (int k = 0; k < 2 * RZ; ++k) { gload(zb - RZ + k, riA, rhA, rcA); lstore(k, riA, rhA, rcA); }\n";
+  os << "  gload(zb + RZ, riA, rhA, rcA);\n  gload(zb + RZ + 1, riB, rhB, rcB);\n";
+  os << "  auto compute = [&](int z, int sz) {\n";
   // extras at the output coordinate
   for (size_t j = 0; j < p.extra_dtypes.size(); ++j) {
     const std::string E = ctype_of(p.extra_dtypes[j]);
-    os << "    " << C << " ex" << j << "[4];\n"
-       << "    if (valid) { const nca_vec<" << E << ", 4> v = *reinterpret_cast<const nca_vec<" << E << ", 4>*>(e" << j
-       << " + (z * NY + gy) * NX + gx);\n"
-       << "#pragma unroll\n      for (int j = 0; j < 4; ++j) ex" << j << "[j] = nca_cvt<" << C << ", " << E << ">::from(v.v[j]); }\n"
-       << "    else {\n#pragma unroll\n      for (int j = 0; j < 4; ++j) ex" << j << "[j] = (" << C << ")0; }\n";
+    os << "    " << C << " ex" << j << "[VX];\n"
+       << "    if (valid) { const nca_vec<" << E << ", VX> v = *reinterpret_cast<const nca_vec<" << E << ", VX>*>(e" << j
+       << " + (size_t)z * plane + ioff);\n"
+       << "#pragma unroll\n      for (int j = 0; j < VX; ++j) ex" << j << "[j] = nca_cvt<" << C << ", " << E << ">::from(v.v[j]); }\n"
+       << "    else {\n#pragma unroll\n      for (int j = 0; j < VX; ++j) ex" << j << "[j] = (" << C << ")0; }\n";
   }
   // slots for each z offset
   std::vector<int64_t> dzs;
   for (auto& o : p.offsets)
     if (std::find(dzs.begin(), dzs.end(), o[0]) == dzs.end()) dzs.push_back(o[0]);
-  for (int64_t dz : dzs) os << "    const int sl" << (dz < 0 ? "m" : "p") << std::llabs(dz) << " = slot_of(z + (" << dz << "));\n";
-  os << "    nca_vec<" << D << ", 4> r;\n";
-  os << "#pragma unroll\n    for (int j = 0; j < 4; ++j) {\n";
+  for (int64_t dz : dzs)
+    os << "    const int sl" << (dz < 0 ? "m" : "p") << std::llabs(dz) << " = wrap(sz + (" << dz << "));\n";
+  os << "    nca_vec<" << D << ", VX> r;\n";
+  os << "#pragma unroll\n    for (int j = 0; j < VX; ++j) {\n";
   for (int k = 0; k < p.n_off; ++k) {
     const auto& o = p.offsets[k];
     os << "      const " << C << " u" << k << " = lds[sl" << (o[0] < 0 ? "m" : "p") << std::llabs(o[0]) << "][ty + RY + (" << o[1]
-       << ")][RX + 4 * tx + j + (" << o[2] << ")];\n";
+       << ")][RX + VX * tx + j + (" << o[2] << ")];\n";
   }
   for (size_t j = 0; j < p.extra_dtypes.size(); ++j)
     os << "      const " << C << " u" << (p.n_off + j) << " = ex" << j << "[j];\n";
-  if (t.uses_idx) os << "      const nca_i64 lin = (z * NY + gy) * NX + gx + j;\n";
+  if (t.uses_idx) os << "      const nca_i64 lin = (nca_i64)z * plane + ioff + j;\n";
   os << body;
-  os << "      const " << D << " st = nca_store_cvt<" << D << ", " << C << ">::to(" << roots[0] << ");\n"
-     << "      r.v[j] = st;\n"
-     << "      if (valid) {\n"
-     << "        const " << RC << " rv = nca_cvt<" << RC << ", " << D << ">::from(st);\n";
-  if (fz.axis0) {
-    if (comp) os << "        nca_neumaier_add(cs[j], cc[j], (float)rv);\n";
-    else os << "        cs[j] += (float)rv;\n";
+  os << "      r.v[j] = nca_store_cvt<" << D << ", " << C << ">::to(" << roots[0] << ");\n"
+     << "    }\n";
+  // reductions of the stored values after the element loop (short live ranges)
+  if (fz.axis0 || fz.fuse) {
+    const int vx = march_vx(p.src_dtype);
+    os << "    if (valid) {\n";
+    for (int j = 0; j < vx; ++j)
+      os << "      const " << RC << " rv" << j << " = nca_cvt<" << RC << ", " << D << ">::from(r.v[" << j << "]);\n";
+    if (fz.axis0)
+      for (int j = 0; j < vx; ++j) {
+        if (comp) os << "      nca_neumaier_add(cs[" << j << "], cc[" << j << "], (float)rv" << j << ");\n";
+        else os << "      cs[" << j << "] += (float)rv" << j << ";\n";
+      }
+    // pairwise trees over the vx values
+    auto tree = [&](const std::string& f) {
+      std::vector<std::string> v;
+      for (int j = 0; j < vx; ++j) v.push_back("rv" + std::to_string(j));
+      while (v.size() > 1) {
+        std::vector<std::string> w;
+        for (size_t k = 0; k + 1 < v.size(); k += 2) w.push_back(f + "(" + v[k] + ", " + v[k + 1] + ")");
+        if (v.size() % 2) w.push_back(v.back());
+        v = w;
+      }
+      return v[0];
+    };
+    if (fz.fuse & NCA_FUSE_SUM) {
+      os << "      const " << RC << " qs = " << tree("nca_add2") << ";\n";
+      if (comp) os << "      nca_neumaier_add(acc_s, acc_c, qs);\n";
+      else os << "      acc_s += qs;\n";
+    }
+    if (fz.fuse & NCA_FUSE_MAX) os << "      acc_mx = nca_maxop(acc_mx, " << tree("nca_maxop") << ");\n";
+    if (fz.fuse & NCA_FUSE_MIN) os << "      acc_mn = nca_minop(acc_mn, " << tree("nca_minop") << ");\n";
+    os << "    }\n";
   }
-  if (fz.fuse & NCA_FUSE_SUM) {
-    if (comp) os << "        nca_neumaier_add(acc_s, acc_c, rv);\n";
-    else os << "        acc_s += rv;\n";
-  }
-  if (fz.fuse & NCA_FUSE_MAX) os << "        acc_mx = nca_maxop(acc_mx, rv);\n";
-  if (fz.fuse & NCA_FUSE_MIN) os << "        acc_mn = nca_minop(acc_mn, rv);\n";
-  os << "      }\n    }\n";
-  if (fz.with_dst) os << "    if (valid) *reinterpret_cast<nca_vec<" << D << ", 4>*>(dst + (z * NY + gy) * NX + gx) = r;\n";
-  os << "  }\n";
-  // epilogue: column sums.  With several z runs per tile, each run writes float64
-  // partials; the last run of a tile to finish (per-tile ticket) adds them in z order.
+  if (fz.with_dst)
+    os << "    if (valid) nca_store_nt(reinterpret_cast<nca_vec<" << D << ", VX>*>(dst + (size_t)z * plane + ioff), r);\n";
+  os << "  };\n";
+  // one barrier per plane; the register set just stored is refilled two planes ahead
+  // sz: slot of plane z (= (z - z0 + RZ) % NS); the plane stored at iteration z is z + RZ
+  os << "  int sz = RZ;\n"
+     << "  for (int z = zb; z < ze; z += 2) {\n"
+     << "    lstore(wrap(sz + RZ), riA, rhA, rcA);\n"
+     << "    __syncthreads();\n"
+     << "    if (z + 2 < ze) gload(z + 2 + RZ, riA, rhA, rcA);\n"
+     << "    compute(z, sz);\n"
+     << "    sz = wrap(sz + 1);\n"
+     << "    if (z + 1 < ze) {\n"
+     << "      lstore(wrap(sz + RZ), riB, rhB, rcB);\n"
+     << "      __syncthreads();\n"
+     << "      if (z + 3 < ze) gload(z + 3 + RZ, riB, rhB, rcB);\n"
+     << "      compute(z + 1, sz);\n"
+     << "      sz = wrap(sz + 1);\n"
+     << "    }\n"
+     << "  }\n";
+  // epilogue: column sums (one run per tile: final values; several runs: float64
+  // partials per run, added in z order by the combine kernel) and per-block partials of
+  // the global reductions (combined in block order by the same kernel).
   if (fz.axis0) {
     const std::string CT = ctype_of(fz.axis0_dtype);
-    os << "  if (nzc == 1) {\n"
-       << "    if (valid) {\n#pragma unroll\n      for (int j = 0; j < 4; ++j) nca_st(reinterpret_cast<" << CT
-       << "*>(colout) + gy * NX + gx + j, (double)cs[j]" << (comp ? " + (double)cc[j]" : "") << ");\n    }\n"
-       << "  } else {\n"
-       << "    if (valid) {\n#pragma unroll\n      for (int j = 0; j < 4; ++j) colpart[((nca_i64)tzi * NY + gy) * NX + gx + j] = (double)cs[j]"
-       << (comp ? " + (double)cc[j]" : "") << ";\n    }\n"
-       << "    if (nca_last_block(counters + (txi + ntx * tyi), (unsigned)nzc)) {\n"
-       << "      if (valid) {\n#pragma unroll\n        for (int j = 0; j < 4; ++j) {\n"
-       << "          double t = 0.0;\n"
-       << "          for (int q = 0; q < nzc; ++q) t += __builtin_nontemporal_load(&colpart[((nca_i64)q * NY + gy) * NX + gx + j]);\n"
-       << "          nca_st(reinterpret_cast<" << CT << "*>(colout) + gy * NX + gx + j, t);\n"
-       << "        }\n      }\n    }\n"
-       << "  }\n";
+    os << "  if (valid) {\n"
+       << "    if (colpart) {\n#pragma unroll\n      for (int j = 0; j < VX; ++j) colpart[((nca_i64)tzi * NY + gy) * NX + gx + j] = (double)cs[j]"
+       << (comp ? " + (double)cc[j]" : "") << ";\n"
+       << "    } else {\n#pragma unroll\n      for (int j = 0; j < VX; ++j) nca_st(reinterpret_cast<" << CT
+       << "*>(colout) + gy * NX + gx + j, (double)cs[j]" << (comp ? " + (double)cc[j]" : "") << ");\n"
+       << "    }\n  }\n";
   }
   if (fz.fuse) {
-    const int32_t sum_dt = reduce_result_dtype(NCA_REDUCE_SUM, dst_dtype);
     os << "  __shared__ double red_sd[4];\n  __shared__ " << RC << " red_mx[4];\n  __shared__ " << RC << " red_mn[4];\n"
        << "  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;\n";
     if (fz.fuse & NCA_FUSE_SUM) {
@@ -279,33 +327,12 @@ static std::string gen_march(const StencilProgram& p, const StencilFuse& fz, int
     os << "  __syncthreads();\n  if (threadIdx.x == 0) {\n";
     if (fz.fuse & NCA_FUSE_SUM) os << "    ((double*)psum)[b] = red_sd[0] + red_sd[1] + red_sd[2] + red_sd[3];\n";
     if (fz.fuse & NCA_FUSE_MAX)
-      os << "    " << RC << " m = red_mx[0];\n    for (int w = 1; w < 4; ++w) m = nca_maxop(m, red_mx[w]);\n    ((" << RC
-         << "*)pmax)[b] = m;\n";
+      os << "    " << RC << " m = red_mx[0];\n    for (int w = 1; w < 4; ++w) m = nca_maxop(m, red_mx[w]);\n    *(" << RC
+         << "*)((char*)pmax + 8 * (size_t)b) = m;\n";
     if (fz.fuse & NCA_FUSE_MIN)
-      os << "    " << RC << " m2 = red_mn[0];\n    for (int w = 1; w < 4; ++w) m2 = nca_minop(m2, red_mn[w]);\n    ((" << RC
-         << "*)pmin)[b] = m2;\n";
+      os << "    " << RC << " m2 = red_mn[0];\n    for (int w = 1; w < 4; ++w) m2 = nca_minop(m2, red_mn[w]);\n    *(" << RC
+         << "*)((char*)pmin + 8 * (size_t)b) = m2;\n";
     os << "  }\n";
-    // the last block of the grid combines the per-block partials in block order
-    os << "  if (nca_last_block(counters + ntx * nty, nb)) {\n";
-    if (fz.fuse & NCA_FUSE_SUM)
-      os << "    double ts = 0.0;\n    for (unsigned k = threadIdx.x; k < nb; k += blockDim.x) ts += __builtin_nontemporal_load(&((const double*)psum)[k]);\n"
-         << "    ts = nca_wave_sum(ts);\n    if (lane == 0) red_sd[wid] = ts;\n";
-    if (fz.fuse & NCA_FUSE_MAX)
-      os << "    " << RC << " tm = nca_lim<" << RC << ">::lowest();\n    for (unsigned k = threadIdx.x; k < nb; k += blockDim.x) tm = nca_maxop(tm, __builtin_nontemporal_load(&((const "
-         << RC << "*)pmax)[k]));\n    tm = nca_wave_max(tm);\n    if (lane == 0) red_mx[wid] = tm;\n";
-    if (fz.fuse & NCA_FUSE_MIN)
-      os << "    " << RC << " tn = nca_lim<" << RC << ">::max();\n    for (unsigned k = threadIdx.x; k < nb; k += blockDim.x) tn = nca_minop(tn, __builtin_nontemporal_load(&((const "
-         << RC << "*)pmin)[k]));\n    tn = nca_wave_min(tn);\n    if (lane == 0) red_mn[wid] = tn;\n";
-    os << "    __syncthreads();\n    if (threadIdx.x == 0) {\n";
-    if (fz.fuse & NCA_FUSE_SUM)
-      os << "      nca_st(reinterpret_cast<" << ctype_of(sum_dt) << "*>(sum_out), red_sd[0] + red_sd[1] + red_sd[2] + red_sd[3]);\n";
-    if (fz.fuse & NCA_FUSE_MAX)
-      os << "      " << RC << " m = red_mx[0];\n      for (int w = 1; w < 4; ++w) m = nca_maxop(m, red_mx[w]);\n      nca_st(reinterpret_cast<"
-         << D << "*>(max_out), m);\n";
-    if (fz.fuse & NCA_FUSE_MIN)
-      os << "      " << RC << " m2 = red_mn[0];\n      for (int w = 1; w < 4; ++w) m2 = nca_minop(m2, red_mn[w]);\n      nca_st(reinterpret_cast<"
-         << D << "*>(min_out), m2);\n";
-    os << "    }\n  }\n";
   }
   os << "}\n";
   return os.str();
@@ -465,14 +492,17 @@ static void stencil_apply_impl(uint64_t handle, const nca_view* src, const nca_v
   for (int d = 0; d < p.ndim; ++d) unit = unit && step[d] == 1;
   MarchGeom g = p.ndim == 3 ? geom_of(p) : MarchGeom{};
   bool march = p.ndim == 3 && unit && !p.has_ext && g.rz <= 3 && g.ry <= 3 && g.rx <= 3 && is_c_contiguous(*src) &&
-               same_shape(*src, *dst) && (!with_dst || is_c_contiguous(*dst)) && dst->shape[2] % 4 == 0 &&
-               reinterpret_cast<uintptr_t>(src->data) % (4 * itemsize(src->dtype)) == 0 &&
-               (!with_dst || reinterpret_cast<uintptr_t>(dst->data) % (4 * itemsize(dst->dtype)) == 0) &&
+               dst->shape[1] * dst->shape[2] < (int64_t(1) << 31) &&
+               same_shape(*src, *dst) && (!with_dst || is_c_contiguous(*dst)) &&
                (itemsize(src->dtype) == 4 || itemsize(src->dtype) == 2);
+  const int vx = march_vx(src->dtype);
+  march = march && dst->shape[2] % vx == 0 &&
+          reinterpret_cast<uintptr_t>(src->data) % (vx * itemsize(src->dtype)) == 0 &&
+          (!with_dst || reinterpret_cast<uintptr_t>(dst->data) % (vx * itemsize(dst->dtype)) == 0);
   for (int j = 0; j < n_extra && march; ++j)
-    march = is_c_contiguous(extras[j]) && reinterpret_cast<uintptr_t>(extras[j].data) % (4 * itemsize(extras[j].dtype)) == 0;
+    march = is_c_contiguous(extras[j]) && reinterpret_cast<uintptr_t>(extras[j].data) % (vx * itemsize(extras[j].dtype)) == 0;
   if (axis0) {
-    NCA_REQUIRE(march, "the fused axis-0 sum needs the march kernel: 3-D, step 1, C-contiguous, last extent % 4 == 0");
+    NCA_REQUIRE(march, "the fused axis-0 sum needs the march kernel: 3-D, step 1, C-contiguous, last extent a multiple of 16 bytes");
     validate_view(*axis0, "axis0_sum");
     NCA_REQUIRE(axis0->ndim == 2 && axis0->shape[0] == dst->shape[1] && axis0->shape[1] == dst->shape[2] &&
                     is_c_contiguous(*axis0),
@@ -482,15 +512,8 @@ static void stencil_apply_impl(uint64_t handle, const nca_view* src, const nca_v
   if (n == 0) return;
   if (march) {
     const int64_t NZ = dst->shape[0], NY = dst->shape[1], NX = dst->shape[2];
-    const int ntx = static_cast<int>((NX + 63) / 64), nty = static_cast<int>((NY + 15) / 16);
+    const int ntx = static_cast<int>((NX + 16 * vx - 1) / (16 * vx)), nty = static_cast<int>((NY + 15) / 16);
     const int64_t tiles = static_cast<int64_t>(ntx) * nty;
-    // z-run length: enough blocks for ~8 per CU, runs of at least 16 planes
-    const int64_t target = static_cast<int64_t>(dev.n_cus) * 8;
-    int64_t nzc = std::max<int64_t>(1, std::min<int64_t>((target + tiles - 1) / tiles, NZ / 16));
-    int zlen = static_cast<int>((NZ + nzc - 1) / nzc);
-    nzc = (NZ + zlen - 1) / zlen;
-    const int64_t blocks = tiles * nzc;
-    NCA_REQUIRE(blocks < (int64_t(1) << 31), "grid too large");
     StencilFuse fz;
     fz.axis0 = axis0 != nullptr;
     fz.fuse = fuse;
@@ -500,8 +523,19 @@ static void stencil_apply_impl(uint64_t handle, const nca_view* src, const nca_v
     std::string key = "m|" + std::to_string(fz.axis0) + std::to_string(fz.axis0_dtype) + "|" + std::to_string(fuse) + "|" +
                       std::to_string(mode) + "|" + std::to_string(with_dst);
     hipFunction_t fn = stencil_kernel(p, key, [&]() { return gen_march(p, fz, dst->dtype); }, "nca_stencil_march");
+    // One resident wave of blocks: (tiles x z runs) = CUs x blocks per CU for this kernel's
+    // registers and LDS, so no block waits for another to finish (no tail), and runs are
+    // as long as possible (fewer halo planes).
+    int per_cu = 0;
+    HIP_CHECK(hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, 256, 0));
+    if (per_cu < 1) per_cu = 1;
+    const int64_t resident = static_cast<int64_t>(dev.n_cus) * per_cu;
+    int64_t nzc = std::max<int64_t>(1, std::min<int64_t>(resident / std::max<int64_t>(tiles, 1), NZ));
+    int zlen = static_cast<int>((NZ + nzc - 1) / nzc);
+    nzc = (NZ + zlen - 1) / zlen;
+    const int64_t blocks = tiles * nzc;
+    NCA_REQUIRE(blocks < (int64_t(1) << 31), "grid too large");
     size_t colbytes = (axis0 && nzc > 1) ? static_cast<size_t>(nzc * NY * NX) * sizeof(double) : 0;
-    NCA_REQUIRE(tiles + 1 <= kWorkspaceCounters, "too many stencil tiles for the ticket counters");
     Workspace& w = workspace(s, static_cast<size_t>(blocks) * 24, colbytes);
     std::deque<const void*> ptrs;
     std::vector<void*> params;
@@ -523,20 +557,34 @@ static void stencil_apply_impl(uint64_t handle, const nca_view* src, const nca_v
     void* psum = w.partials;
     void* pmax = static_cast<char*>(w.partials) + static_cast<size_t>(blocks) * 8;
     void* pmin = static_cast<char*>(w.partials) + static_cast<size_t>(blocks) * 16;
-    void* counters = w.counters;
-    int nzc_i = static_cast<int>(nzc);
     params.push_back(&colpart);
     params.push_back(&colout);
     params.push_back(&psum);
     params.push_back(&pmax);
     params.push_back(&pmin);
-    params.push_back(&counters);
-    params.push_back(&nzc_i);
-    params.push_back(&sum_out);
-    params.push_back(&max_out);
-    params.push_back(&min_out);
     for (auto& x : sc.slots) params.push_back(&x);
-    jit_launch(fn, dim3(static_cast<unsigned>(blocks)), dim3(256), 0, s, params);
+    {
+      ProfScope ps("nca_stencil_march", s);
+      jit_launch(fn, dim3(static_cast<unsigned>(blocks)), dim3(256), 0, s, params);
+    }
+    if (colpart || fuse) {
+      FinalizeArgs fa;
+      fa.colpart = static_cast<const double*>(colpart);
+      fa.nzc = static_cast<int>(nzc);
+      fa.K = colpart ? NY * NX : 0;
+      fa.colout = colout;
+      fa.col_dtype = axis0 ? axis0->dtype : NCA_FLOAT64;
+      fa.value_dtype = dst->dtype;
+      fa.fuse = fuse;
+      fa.psum = psum;
+      fa.pmax = pmax;
+      fa.pmin = pmin;
+      fa.nblocks = static_cast<int>(blocks);
+      fa.sum_out = sum_out;
+      fa.max_out = max_out;
+      fa.min_out = min_out;
+      launch_finalize_fused(fa, s);
+    }
     return;
   }
   // direct
@@ -587,10 +635,23 @@ static void stencil_apply_impl(uint64_t handle, const nca_view* src, const nca_v
   }
   for (int k = 0; k < n_ext_args; ++k) params.push_back(ext_args[k]);
   for (auto& x : sc.slots) params.push_back(&x);
-  jit_launch(fn, dim3(blocks), dim3(256), 0, s, params);
-  if ((fuse & NCA_FUSE_SUM) && sum_out) launch_finalize_scalar(NCA_FUSE_SUM, dst->dtype, psum, blocks, sum_out, s);
-  if ((fuse & NCA_FUSE_MAX) && max_out) launch_finalize_scalar(NCA_FUSE_MAX, dst->dtype, pmax, blocks, max_out, s);
-  if ((fuse & NCA_FUSE_MIN) && min_out) launch_finalize_scalar(NCA_FUSE_MIN, dst->dtype, pmin, blocks, min_out, s);
+  {
+    ProfScope ps("nca_stencil_direct", s);
+    jit_launch(fn, dim3(blocks), dim3(256), 0, s, params);
+  }
+  if (fuse) {
+    FinalizeArgs fa;
+    fa.value_dtype = dst->dtype;
+    fa.fuse = fuse;
+    fa.psum = psum;
+    fa.pmax = pmax;
+    fa.pmin = pmin;
+    fa.nblocks = blocks;
+    fa.sum_out = sum_out;
+    fa.max_out = max_out;
+    fa.min_out = min_out;
+    launch_finalize_fused(fa, s);
+  }
 }
 
 }  // namespace nca

@@ -19,7 +19,9 @@ def view(t, dtype_code=None):
     """capi.View over tensor t (any strides; element strides converted to bytes)."""
     code = TORCH_TO_CODE[t.dtype] if dtype_code is None else dtype_code
     item = t.element_size()
-    return capi.make_view(t.data_ptr(), code, list(t.shape), [s * item for s in t.stride()])
+    v = capi.make_view(t.data_ptr(), code, list(t.shape), [s * item for s in t.stride()])
+    v._owner = t  # the view keeps its tensor (and so its device memory) alive
+    return v
 
 
 def shape_view(shape, dtype_code):

@@ -1,0 +1,143 @@
+"""The C ABI without a GPU: the library loads, exports every declared entry point, applies
+the reference's type rules, validates tapes, and every generated kernel compiles for gfx950."""
+import os
+import re
+
+import pytest
+
+from ncarray_amd import capi, tapes
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+P = capi.pack
+
+
+def test_exports_every_declared_symbol():
+    header = open(os.path.join(ROOT, "include", "nca_hip.h")).read()
+    declared = set(re.findall(r"\b(nca_hip_\w+)\s*\(", header))
+    assert len(declared) >= 35
+    for name in declared:
+        assert hasattr(capi.lib, name), name
+    assert set(capi.EXPORTED) == declared
+    assert capi.lib.nca_hip_version().startswith(b"ncarray-hip")
+
+
+def test_calls_without_init_fail_cleanly():
+    if capi._initialised:
+        pytest.skip("a device was initialised in this process")
+    v = capi.make_view(None, capi.FLOAT32, [4])
+    out = capi.ctypes.c_void_p()
+    rc = capi.lib.nca_hip_reduce_full(capi.ctypes.byref(v), capi.REDUCE_SUM, 0, None, None)
+    assert rc != capi.OK and capi.last_error()
+
+
+def test_result_dtypes_follow_the_reference():
+    # SumTraits::OutT = op_traits<T>::sum_type (op_traits.hh:72-77, 380-410)
+    assert capi.reduce_result_dtype(capi.REDUCE_SUM, capi.FLOAT32) == capi.FLOAT32
+    assert capi.reduce_result_dtype(capi.REDUCE_SUM, capi.INT8) == capi.INT64
+    assert capi.reduce_result_dtype(capi.REDUCE_SUM, capi.UINT16) == capi.UINT64
+    assert capi.reduce_result_dtype(capi.REDUCE_SUM, capi.BOOL) == capi.UINT64
+    assert capi.reduce_result_dtype(capi.REDUCE_SUM, capi.INT32) == capi.INT32
+    assert capi.reduce_result_dtype(capi.REDUCE_SUM, capi.BFLOAT16) == capi.FLOAT32
+    assert capi.reduce_result_dtype(capi.REDUCE_ARGMAX, capi.FLOAT32) == capi.INT64
+    assert capi.reduce_result_dtype(capi.REDUCE_MAX, capi.BFLOAT16) == capi.BFLOAT16
+    assert capi.reduce_result_dtype(capi.REDUCE_MEAN, capi.INT32) == capi.FLOAT64
+    # determine_dtype_for_op (op_code.hh:101-161)
+    assert capi.op_result_dtype(capi.OP_DIV, capi.FLOAT32) == capi.FLOAT64
+    assert capi.op_result_dtype(capi.OP_ADD, capi.UINT8) == capi.UINT64
+    assert capi.op_result_dtype(capi.OP_SUB, capi.UINT8) == capi.INT64
+    assert capi.op_result_dtype(capi.OP_GT, capi.FLOAT32) == capi.BOOL
+    assert capi.op_result_dtype(capi.OP_MUL, capi.INT16) == capi.INT16
+    assert capi.op_result_dtype(capi.OP_EXP, capi.INT32) == capi.FLOAT64
+    assert capi.op_result_dtype(capi.OP_EXP, capi.FLOAT32) == capi.FLOAT32
+
+
+@pytest.mark.parametrize("instrs,err", [
+    ([P(capi.OP_ADD)], capi.ERR_INVALID),                                     # stack underflow
+    ([P(capi.OP_LOAD_NCARR, 0), P(capi.OP_LOAD_NCARR, 0)], capi.ERR_INVALID),  # two values left
+    ([P(capi.OP_LOAD_NCARR, 5)], capi.ERR_INVALID),                           # operand out of range
+    ([P(capi.OP_LOAD_NCARR, 0), P(capi.OP_SZOF)], capi.ERR_UNSUPPORTED),       # not implemented by any engine
+    ([P(capi.OP_LOAD_NCARR, 0), P(capi.OP_LOAD_NCARR, 0), P(capi.OP_BAND)], capi.ERR_UNSUPPORTED),  # bitwise on float
+])
+def test_tape_validation(instrs, err):
+    with pytest.raises(capi.NcaError) as e:
+        capi.Tape(instrs, [capi.FLOAT32], [], capi.FLOAT32, capi.FLOAT32)
+    assert e.value.code == err
+
+
+def test_unsupported_dtypes_are_reported():
+    with pytest.raises(capi.NcaError) as e:
+        capi.Tape([P(capi.OP_LOAD_NCARR, 0)], [capi.COMPLEX64], [], capi.COMPLEX64, capi.COMPLEX64)
+    assert e.value.code == capi.ERR_UNSUPPORTED
+
+
+def _compiles(src):
+    ok, log = capi.rtc_compile(src)
+    assert ok, log[:4000]
+
+
+@pytest.mark.parametrize("dt", [capi.FLOAT32, capi.FLOAT64, capi.BFLOAT16, capi.FLOAT16])
+def test_expression_kernels_compile(dt):
+    t = capi.Tape(tapes.fused_c0(), [dt] * 5, [capi.FLOAT32], dt, dt)
+    for variant in (0, 1, 2, 3, 4):
+        _compiles(t.source(variant, 3))
+
+
+def test_integer_and_bool_tapes_compile():
+    ins = [P(capi.OP_LOAD_NCARR, 0), P(capi.OP_LOAD_NCARR, 1), P(capi.OP_DIV), P(capi.OP_LOAD_CONST, 0),
+           P(capi.OP_MOD), P(capi.OP_LOAD_NCARR, 2), P(capi.OP_BAND), P(capi.OP_LOAD_CONST, 1), P(capi.OP_LSHFT),
+           P(capi.OP_ABS), P(capi.OP_LOAD_NCARR, 3), P(capi.OP_FDIV)]
+    for dt in (capi.INT32, capi.INT64, capi.UINT8, capi.UINT64):
+        t = capi.Tape(ins, [dt] * 4, [dt, dt], dt, dt)
+        _compiles(t.source(0))
+        _compiles(t.source(1, 2))
+    cmp = [P(capi.OP_LOAD_NCARR, 0), P(capi.OP_LOAD_CONST, 0), P(capi.OP_GT), P(capi.OP_LOAD_NCARR, 1), P(capi.OP_LAND),
+           P(capi.OP_LNOT)]
+    _compiles(capi.Tape(cmp, [capi.FLOAT32, capi.BOOL], [capi.FLOAT32], capi.FLOAT32, capi.BOOL).source(0))
+
+
+def test_linearizable_vs_vm_evaluation_dtype():
+    """float32 / float32 into a float64 destination: the linearizable tape runs in the work
+    dtype (float32), a deeper tape runs in the destination dtype (float64), as the two
+    reference paths do (gpuengine.hh:103-191, dynamicmvnode.hh:244-249)."""
+    lin = capi.Tape([P(capi.OP_LOAD_NCARR, 0), P(capi.OP_LOAD_NCARR, 1), P(capi.OP_DIV)], [capi.FLOAT32] * 2, [],
+                    capi.FLOAT32, capi.FLOAT64)
+    assert "const float t" in lin.source(0)
+    deep = capi.Tape([P(capi.OP_LOAD_NCARR, 0), P(capi.OP_LOAD_NCARR, 1), P(capi.OP_LOAD_NCARR, 2), P(capi.OP_MUL),
+                      P(capi.OP_DIV)], [capi.FLOAT32] * 3, [], capi.FLOAT32, capi.FLOAT64)
+    assert "const double t" in deep.source(0)
+
+
+@pytest.mark.parametrize("dt", [capi.FLOAT32, capi.BFLOAT16])
+def test_stencil_kernels_compile(dt):
+    st = capi.Stencil(tapes.LAPLACIAN_OFFSETS, tapes.laplacian(), dt, capi.FLOAT32, dt, [capi.FLOAT32])
+    for variant in (0, 1, 2, 3):
+        _compiles(st.source(variant))
+    adj = st.adjoint()
+    for variant in (0, 2):
+        _compiles(adj.source(variant))
+
+
+def test_stencil_with_extra_array_compiles():
+    st = capi.Stencil(tapes.LAPLACIAN_OFFSETS, tapes.laplacian_then_c0(), capi.FLOAT32, capi.FLOAT32, capi.FLOAT32,
+                      [capi.FLOAT32, capi.FLOAT32], extra_dtypes=[capi.FLOAT32])
+    _compiles(st.source(1))
+    _compiles(st.source(2))
+
+
+def test_nonlinear_stencil_has_no_adjoint():
+    ins = [P(capi.OP_LOAD_NCARR, 0), P(capi.OP_LOAD_NCARR, 1), P(capi.OP_MUL)]
+    st = capi.Stencil([[0, 0, 0], [1, 0, 0]], ins, capi.FLOAT32, capi.FLOAT32, capi.FLOAT32)
+    with pytest.raises(capi.NcaError):
+        st.adjoint()
+
+
+def test_adjoint_source_negates_offsets():
+    st = capi.Stencil([[0, 0, 1], [0, 0, 0]], [P(capi.OP_LOAD_NCARR, 0), P(capi.OP_LOAD_NCARR, 1), P(capi.OP_SUB)],
+                      capi.FLOAT32, capi.FLOAT32, capi.FLOAT32)
+    src = st.adjoint().source(2)
+    assert "+ (-1)" in src
+
+
+def test_stencil_rejects_bad_descriptions():
+    with pytest.raises(capi.NcaError):
+        capi.Stencil(tapes.LAPLACIAN_OFFSETS, tapes.laplacian(), capi.FLOAT32, capi.FLOAT32, capi.FLOAT32, [])

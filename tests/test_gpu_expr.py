@@ -32,13 +32,19 @@ def test_c0_expression_vs_oracle(dev, n):
     tape.launch([ops[k] for k in tapes.C0_OPERANDS], ti.view(out), scalars=tapes.C0_SCALARS)
     got = out.cpu().numpy()
     ref = O.expr(tapes.fused_c0(), [a, b, a, a, b], [0.5], 10, np.float32)
-    # exp is the only transcendental: ocml expf vs glibc expf may differ by an ulp
+    # exp is the only transcendental: the device expf and the host's glibc expf may differ
+    # by an ulp of exp(-a*a) in [e^-1, 1]; after the final additions that is an absolute
+    # difference of at most 2 ulp of the largest term (outputs near zero can therefore
+    # differ by many of their own ulps).  Everything else is evaluated in the same order.
     if n:
-        assert O.ulp_diff_f32(got, ref).max() <= 2
         a64, b64 = a.astype(np.float64), b.astype(np.float64)
+        scale = np.abs(a64 * b64) + np.exp(-a64 * a64) + 0.5 * np.abs(b64)
+        eps = float(np.finfo(np.float32).eps)
+        assert np.all(np.abs(got.astype(np.float64) - ref) <= 2 * eps * scale)
+        if n >= 1000:
+            assert np.mean(got == ref) > 0.9   # most elements bit-identical to the oracle
         f64 = a64 * b64 + np.exp(-a64 * a64) - 0.5 * b64
-        rel = np.abs(got - f64) / np.maximum(np.abs(f64), 1e-30)
-        assert np.max(np.abs(got - f64)) <= 4 * np.finfo(np.float32).eps * np.max(np.abs(f64))
+        assert np.all(np.abs(got - f64) <= 3 * eps * scale)
 
 
 def test_arithmetic_bit_exact(dev):

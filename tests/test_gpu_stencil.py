@@ -184,3 +184,28 @@ def test_laplacian_512_pipeline_properties(dev):
     err = np.abs(col.cpu().numpy() - ref_col) / np.abs(lapn).sum(axis=0, dtype=np.float64)
     assert err.max() <= 2e-7
     assert mx.item() == float(lapn.max())
+
+
+@pytest.mark.parametrize("shape", [(12, 20, 128), (40, 33, 256)])
+def test_laplacian_bf16_storage(dev, shape):
+    """bf16 in, float32 arithmetic, bf16 out: equals the float32 oracle on the bf16 inputs,
+    rounded once to bf16 (round to nearest even); compensated column sums vs FP64."""
+    capi, tapes, ti = _c()
+    x = np.random.default_rng(21).standard_normal(shape).astype(np.float32)
+    xb = torch.from_numpy(x).to(dev).to(torch.bfloat16)
+    xr = xb.float().cpu().numpy()
+    st = capi.Stencil(tapes.LAPLACIAN_OFFSETS, tapes.laplacian(), capi.BFLOAT16, capi.FLOAT32, capi.BFLOAT16,
+                      [capi.FLOAT32])
+    lap = torch.empty_like(xb)
+    col = torch.empty(shape[1:], dtype=torch.float32, device=dev)
+    mx = torch.empty((), dtype=torch.bfloat16, device=dev)
+    st.apply_reduce(ti.view(xb), ti.view(lap), capi.BOUNDARY_ZERO, axis0=ti.view(col), fuse=capi.FUSE_MAX,
+                    mode=capi.ACCUM_COMPENSATED, max_out=mx.data_ptr(), scalars=(6.0,))
+    ref32 = O.stencil_f32(xr, shape, tapes.LAPLACIAN_OFFSETS, tapes.laplacian(), [6.0], True)
+    ref = O.bf16_round(ref32)
+    got = lap.float().cpu().numpy()
+    assert np.array_equal(got, ref)
+    assert mx.float().item() == float(got.max())
+    g64 = got.astype(np.float64)
+    err = np.abs(col.cpu().numpy() - g64.sum(0)) / np.maximum(np.abs(g64).sum(0), 1e-30)
+    assert err.max() <= 2e-7

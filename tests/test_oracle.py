@@ -1,0 +1,188 @@
+"""The CPU oracle pinned against the reference's own known answers, numpy, closed forms
+and brute force (no GPU)."""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+
+P = lambda op, i=0: ((op & 0xFF) << 24) | (i & 0xFFFFFF)  # noqa: E731  (op_code.hh:174-192)
+LOAD, CONST, IDX = 2, 4, 1
+NEG, INC, DEC, ADD, SUB, MUL, DIV, MOD, GT, LT, EQ = 5, 6, 7, 14, 15, 16, 17, 18, 24, 22, 20
+EXP = 64
+
+
+def test_reduction_kats():
+    # test_gpu_reductions.cu:39-55 (float iota 256)
+    x = np.arange(256, dtype=np.float32)
+    assert O.sum_fold(x) == 32640 and O.max_fold(x) == (255.0, 255) and O.min_fold(x) == (0.0, 0)
+    # test_reductions.cc:26-38 (int32 fill 2 x 10)
+    y = np.full(10, 2, dtype=np.int32)
+    assert O.sum_fold(y) == 20 and O.max_fold(y)[0] == 2 and O.min_fold(y)[0] == 2
+    # test_reductions.cc:40-88 (iota+1 on 2x3, axis sums/max/min)
+    z = np.arange(1, 7, dtype=np.float32).reshape(2, 3)
+    assert O.reduce_axes_f32(z, [0], 0).tolist() == [5, 7, 9]
+    assert O.reduce_axes_f32(z, [1], 0).tolist() == [6, 15]
+    assert O.reduce_axes_f32(z, [0], 1).tolist() == [4, 5, 6]
+    assert O.reduce_axes_f32(z, [1], 2).tolist() == [1, 4]
+    # test_reductions.cc:90-102 (1.5 x 5 -> 7.5)
+    assert O.sum_fold(np.full(5, 1.5, dtype=np.float32)) == 7.5
+    # tests/python/test_operations.py:56-62
+    w = np.array([[1, 2, 3], [4, 5, 6]], dtype=np.int32)
+    assert O.sum_fold(w) == 21 and O.max_fold(w)[0] == 6 and O.min_fold(w)[0] == 1
+
+
+def test_sum_closed_forms_and_fold_order():
+    n = 4097
+    x = np.arange(n, dtype=np.float64)
+    assert O.sum_fold(x) == n * (n - 1) / 2
+    # float32 fold is the sequential left fold: brute force in Python float32 arithmetic
+    r = np.random.default_rng(0).standard_normal(1000).astype(np.float32)
+    acc = np.float32(0)
+    for v in r:
+        acc = np.float32(acc + v)
+    assert O.sum_fold(r) == acc
+    # the check discriminates orders: folding the same data backwards gives other bits
+    back = np.float32(0)
+    for v in r[::-1]:
+        back = np.float32(back + v)
+    assert back != acc
+
+
+def test_argmax_first_occurrence():
+    rng = np.random.default_rng(1)
+    for _ in range(20):
+        x = rng.integers(-5, 5, 200).astype(np.float32)
+        assert O.max_fold(x)[1] == int(np.argmax(x))
+        assert O.min_fold(x)[1] == int(np.argmin(x))
+    # NaN is never selected; all-NaN gives lowest() and index 0 (host/reductions.hh:130-169)
+    x = np.array([np.nan, 2.0, np.nan, 2.0, -1.0], dtype=np.float32)
+    assert O.max_fold(x) == (2.0, 1)
+    v, i = O.max_fold(np.full(3, np.nan, dtype=np.float32))
+    assert v == np.finfo(np.float32).min and i == 0
+
+
+def test_axis_reductions_vs_numpy():
+    rng = np.random.default_rng(2)
+    x = rng.integers(-9, 9, (5, 6, 7)).astype(np.float32)
+    for ax in ([0], [1], [2], [0, 2], [1, 2], [0, 1, 2]):
+        assert np.array_equal(O.reduce_axes_f32(x, ax, 0), x.sum(axis=tuple(ax)))
+        assert np.array_equal(O.reduce_axes_f32(x, ax, 1), x.max(axis=tuple(ax)))
+    am = O.reduce_axes_f32(x, [1], 3)
+    flat = np.arange(x.size).reshape(x.shape)
+    ref = np.take_along_axis(flat, np.argmax(x, axis=1)[:, None, :], 1)[:, 0, :]
+    assert np.array_equal(am, ref)
+
+
+def test_expression_semantics():
+    a = np.array([2, -3, 0, 7], dtype=np.int32)
+    b = np.array([4, 5, 6, -1], dtype=np.int32)
+    # 2 + 4 = 6 (test_binary_operations.cc:28-47), evaluated in float64 here
+    out = O.expr([P(LOAD, 0), P(LOAD, 1), P(ADD)], [a, b], [], 11, np.float64)
+    assert out.tolist() == [6, 2, 6, 6]
+    # DIV: finite / 0 -> NaN, inf / 0 -> inf (interface.hh:150-156)
+    x = np.array([1.0, 0.0, np.inf, 6.0], dtype=np.float32)
+    y = np.array([0.0, 0.0, 0.0, 3.0], dtype=np.float32)
+    d = O.expr([P(LOAD, 0), P(LOAD, 1), P(DIV)], [x, y], [], 10, np.float32)
+    assert np.isnan(d[0]) and np.isnan(d[1]) and d[2] == np.inf and d[3] == 2.0
+    # comparisons give 0/1 in the work type
+    c = O.expr([P(LOAD, 0), P(CONST, 0), P(GT)], [x], [1.5], 10, np.float32)
+    assert c.tolist() == [0, 0, 1, 1]
+    # unary ops (test_unary_operations.cc:15-37)
+    for op, f in ((NEG, lambda v: -v), (INC, lambda v: v + 1), (DEC, lambda v: v - 1)):
+        assert np.array_equal(O.expr([P(LOAD, 0), P(op)], [y], [], 10, np.float32), f(y))
+    # IDX*2+1 (test_unary_operations.cc:39-74)
+    z = np.zeros(10, dtype=np.float32)
+    i = O.expr([P(IDX), P(CONST, 0), P(MUL), P(CONST, 1), P(ADD)], [z], [2.0, 1.0], 11, np.float64)
+    assert i.tolist() == [2 * k + 1 for k in range(10)]
+
+
+def test_c0_expression_vs_fp64():
+    rng = np.random.default_rng(3)
+    a = rng.uniform(-1, 1, 10000).astype(np.float32)
+    b = rng.uniform(-1, 1, 10000).astype(np.float32)
+    tape = [P(LOAD, 0), P(LOAD, 1), P(MUL), P(LOAD, 2), P(LOAD, 3), P(MUL), P(NEG), P(EXP), P(ADD),
+            P(LOAD, 4), P(CONST, 0), P(MUL), P(SUB)]
+    got = O.expr(tape, [a, b, a, a, b], [0.5], 10, np.float32)
+    a64, b64 = a.astype(np.float64), b.astype(np.float64)
+    f64 = a64 * b64 + np.exp(-a64 * a64) - 0.5 * b64
+    scale = np.abs(a64 * b64) + np.exp(-a64 * a64) + 0.5 * np.abs(b64)
+    assert np.all(np.abs(got - f64) <= 3 * np.finfo(np.float32).eps * scale)
+    # float32 evaluation in the VM's order, element by element with numpy float32
+    t = (a * b) + np.exp(-(a * a)).astype(np.float32)
+    ref = (t - b * np.float32(0.5)).astype(np.float32)
+    assert np.max(np.abs(got - ref)) <= 2 * np.finfo(np.float32).eps
+
+
+def _lap_tape():
+    t = [P(LOAD, 0), P(LOAD, 1), P(ADD)]
+    for k in range(2, 6):
+        t += [P(LOAD, k), P(ADD)]
+    return t + [P(CONST, 0), P(LOAD, 6), P(MUL), P(SUB)]
+
+
+LAP_OFFS = [[-1, 0, 0], [1, 0, 0], [0, -1, 0], [0, 1, 0], [0, 0, -1], [0, 0, 1], [0, 0, 0]]
+
+
+def test_stencil_laplacian():
+    rng = np.random.default_rng(4)
+    x = rng.standard_normal((9, 10, 11)).astype(np.float32)
+    got = O.stencil_f32(x, x.shape, LAP_OFFS, _lap_tape(), [6.0], True)
+    # float32 numpy in the same order on the zero-padded array
+    p = np.pad(x, 1)
+    ref = ((((p[:-2, 1:-1, 1:-1] + p[2:, 1:-1, 1:-1]) + p[1:-1, :-2, 1:-1]) + p[1:-1, 2:, 1:-1])
+           + p[1:-1, 1:-1, :-2]) + p[1:-1, 1:-1, 2:]
+    ref = ref - np.float32(6.0) * p[1:-1, 1:-1, 1:-1]
+    assert np.array_equal(got, ref)
+    # closed form: the Laplacian of x^2 + 2y^2 + 3z^2 is 12 in the interior
+    z, y, xx = np.mgrid[0:8, 0:8, 0:8].astype(np.float64)
+    q = (xx ** 2 + 2 * y ** 2 + 3 * z ** 2).astype(np.float32)
+    lq = O.stencil_f32(q, q.shape, LAP_OFFS, _lap_tape(), [6.0], True)
+    assert np.all(lq[1:-1, 1:-1, 1:-1] == 12.0)
+    # pipeline == stencil + fold
+    lap, col, mx = O.laplacian_pipeline(x)
+    assert np.array_equal(lap, got)
+    assert np.array_equal(col, O.reduce_axes_f32(lap, [0], 0))
+    assert mx == lap.max()
+
+
+def test_binning_vs_numpy():
+    rng = np.random.default_rng(5)
+    F, ny, nx, nb = 3, 20, 30, 17
+    bm = O.radial_binmap(ny, nx, 10, 15, nb)
+    bm[0, 0] = -1
+    bm[1, 1] = nb + 4
+    fr = rng.exponential(50.0, (F, ny, nx)).astype(np.float32)
+    fr[:, 2, :5] = 0.0
+    c, s = O.bin_f32(fr, bm, nb, nonzero=True)
+    ok = (bm.ravel() >= 0) & (bm.ravel() < nb)
+    for f in range(F):
+        w = fr[f].ravel()
+        sel = ok & (w != 0)
+        assert np.array_equal(c[f], np.bincount(bm.ravel()[sel], minlength=nb))
+        assert np.allclose(s[f], np.bincount(bm.ravel()[ok], weights=w[ok].astype(np.float64), minlength=nb),
+                           rtol=1e-13)
+    g = rng.standard_normal((F, nb)).astype(np.float32)
+    gb = O.bin_backward_f32(g, bm)
+    for f in range(F):
+        exp = np.where(ok, g[f][np.clip(bm.ravel(), 0, nb - 1)], 0.0)
+        assert np.array_equal(gb[f], exp)
+    d0 = np.zeros(nb, dtype=np.float32)
+    idx = bm.ravel()
+    src = fr[0].ravel()
+    ref = d0.copy()
+    np.add.at(ref, idx[ok], src[ok])
+    assert np.allclose(O.scatter_add_f32(d0, idx, src), ref, rtol=1e-5)
+
+
+def test_radial_binmap_spec():
+    bm = O.radial_binmap()
+    assert bm.shape == (1024, 1024) and bm.dtype == np.int32
+    assert bm[512, 512] == 0 and bm[0, 0] == 999 and bm.max() == 999 and bm.min() == 0
+
+
+def test_bf16_rounding_matches_torch():
+    torch = pytest.importorskip("torch")
+    x = np.random.default_rng(6).standard_normal(10000).astype(np.float32)
+    x[:4] = [np.inf, -np.inf, 0.0, -0.0]
+    ref = torch.from_numpy(x).to(torch.bfloat16).float().numpy()
+    assert np.array_equal(O.bf16_round(x), ref)
