@@ -20,7 +20,17 @@ void set_last_error(const std::string& msg) { t_last_error = msg; }
 bool device_available() { return t_device >= 0; }
 
 DeviceState& current_device() {
-  if (t_device < 0) NCA_THROW(NCA_ERR_NOT_INIT, "nca_hip_init() was not called on this thread");
+  if (t_device < 0) {
+    // A thread that did not call nca_hip_init (a worker thread of the caller, e.g. an
+    // autograd engine thread) adopts its current HIP device once the library has been
+    // initialised for that device by any thread.
+    int d = -1;
+    if (hipGetDevice(&d) == hipSuccess) {
+      std::lock_guard<std::mutex> lk(g_devices_mu);
+      if (g_devices.count(d)) t_device = d;
+    }
+    if (t_device < 0) NCA_THROW(NCA_ERR_NOT_INIT, "nca_hip_init() was not called for this thread's device");
+  }
   std::lock_guard<std::mutex> lk(g_devices_mu);
   auto it = g_devices.find(t_device);
   if (it == g_devices.end()) NCA_THROW(NCA_ERR_NOT_INIT, "device state missing");

@@ -1,0 +1,144 @@
+"""Hand-written adjoints behind torch.autograd.Function: checked with gradcheck (float64),
+central finite differences of the FP64 loss (h = 1e-3) and an FP64 reference gradient."""
+import numpy as np
+import pytest
+import torch
+
+pytestmark = pytest.mark.gpu
+
+
+def _mods():
+    from ncarray_amd import autograd as A, capi, tapes
+    return A, capi, tapes
+
+
+def lap64(a):
+    p = np.pad(np.asarray(a, dtype=np.float64), 1)
+    return (p[:-2, 1:-1, 1:-1] + p[2:, 1:-1, 1:-1] + p[1:-1, :-2, 1:-1] + p[1:-1, 2:, 1:-1]
+            + p[1:-1, 1:-1, :-2] + p[1:-1, 1:-1, 2:] - 6.0 * p[1:-1, 1:-1, 1:-1])
+
+
+def loss64(x, y):
+    a = lap64(x)
+    b = np.asarray(y, dtype=np.float64)
+    return float(np.sum(a * b + np.exp(-a * a) - 0.5 * b))
+
+
+def grad64(x, y):
+    a = lap64(x)
+    b = np.asarray(y, dtype=np.float64)
+    return lap64(b - 2 * a * np.exp(-a * a)), a - 0.5   # the Laplacian is self-adjoint (zero Dirichlet)
+
+
+def test_expr_gradcheck(dev):
+    A, capi, tapes = _mods()
+    tape = A.c0_tape(capi.FLOAT64, capi.FLOAT64, capi.FLOAT64)
+    a = torch.randn(40, dtype=torch.float64, device=dev, requires_grad=True)
+    b = torch.randn(40, dtype=torch.float64, device=dev, requires_grad=True)
+    f = lambda a_, b_: A.expr(tape, (0.5,), a_, b_, a_, a_, b_)  # noqa: E731
+    assert torch.autograd.gradcheck(f, (a, b), eps=1e-6, atol=1e-7)
+
+
+def test_stencil_and_sum_gradcheck(dev):
+    A, capi, tapes = _mods()
+    x = torch.randn((4, 5, 8), dtype=torch.float64, device=dev, requires_grad=True)
+    P = capi.pack
+    sq = capi.Tape([P(capi.OP_LOAD_NCARR, 0), P(capi.OP_LOAD_NCARR, 1), P(capi.OP_MUL)], [capi.FLOAT64] * 2, [],
+                   capi.FLOAT64, capi.FLOAT64)
+
+    def f(t):
+        a = A.laplacian(t)
+        return A.sum(A.expr(sq, (), a, a))
+
+    assert torch.autograd.gradcheck(f, (x,), eps=1e-6, atol=1e-6)
+    assert torch.autograd.gradcheck(lambda t: A.mean(A.laplacian(t)), (x,), eps=1e-6, atol=1e-7)
+
+
+def test_bin_weighted_sum_backward(dev):
+    A, capi, tapes = _mods()
+    from oracle import oracle as O
+    bm = torch.from_numpy(O.radial_binmap(16, 24, 8, 12, 20)).to(dev)
+    fr = torch.rand((3, 16, 24), device=dev, requires_grad=True)
+    s = A.bin_weighted_sum(fr, bm, 20)
+    g = torch.randn_like(s)
+    (s * g).sum().backward()
+    ref = g[:, bm.long().reshape(-1)].reshape(3, 16, 24)
+    assert torch.equal(fr.grad, ref)
+
+
+@pytest.mark.parametrize("n", [24, 64])
+def test_fused_loss_vs_fp64_and_fd(dev, n):
+    A, capi, tapes = _mods()
+    rng = np.random.default_rng(n)
+    x = rng.standard_normal((n, n, n)).astype(np.float32)
+    y = rng.standard_normal((n, n, n)).astype(np.float32)
+    xt = torch.from_numpy(x).to(dev).requires_grad_()
+    yt = torch.from_numpy(y).to(dev).requires_grad_()
+    loss = A.laplacian_c0_loss(xt, yt)
+    L64 = loss64(x, y)
+    a = lap64(x)
+    mag = float(np.sum(np.abs(a * y) + np.exp(-a * a) + 0.5 * np.abs(y)))
+    assert abs(loss.item() - L64) <= 1e-6 * mag
+    loss.backward()
+    gx64, gy64 = grad64(x, y)
+    gx, gy = xt.grad.cpu().numpy(), yt.grad.cpu().numpy()
+    assert np.max(np.abs(gx - gx64)) <= 1e-4 * max(1.0, np.max(np.abs(gx64)))
+    assert np.max(np.abs(gy - gy64)) <= 1e-5 * max(1.0, np.max(np.abs(gy64)))
+    # central finite differences of the FP64 loss along random directions, h = 1e-3
+    h = 1e-3
+    for k in range(2):
+        v = rng.standard_normal(x.shape)
+        w = rng.standard_normal(y.shape)
+        fd = (loss64(x + h * v, y + h * w) - loss64(x - h * v, y - h * w)) / (2 * h)
+        an = float(np.sum(gx.astype(np.float64) * v) + np.sum(gy.astype(np.float64) * w))
+        assert abs(fd - an) <= 1e-4 * (abs(fd) + 1.0) + 1e-3 * np.sqrt(x.size), (fd, an)
+
+
+def test_fused_loss_256(dev):
+    """The differentiable pipeline at 256^3 (fp32): gradients vs the FP64 reference."""
+    A, capi, tapes = _mods()
+    g = torch.Generator(device=dev).manual_seed(5)
+    xt = torch.randn((256, 256, 256), generator=g, device=dev).requires_grad_()
+    yt = torch.randn((256, 256, 256), generator=g, device=dev).requires_grad_()
+    loss = A.laplacian_c0_loss(xt, yt)
+    loss.backward()
+    x, y = xt.detach().cpu().numpy(), yt.detach().cpu().numpy()
+    gx64, gy64 = grad64(x, y)
+    assert np.max(np.abs(xt.grad.cpu().numpy() - gx64)) <= 1e-4 * np.max(np.abs(gx64))
+    assert np.max(np.abs(yt.grad.cpu().numpy() - gy64)) <= 1e-5 * np.max(np.abs(gy64))
+    L64 = loss64(x, y)
+    assert abs(loss.item() - L64) <= 1e-6 * abs(L64) + 1.0
+
+
+def test_fused_loss_bf16(dev):
+    """BF16 storage: gradients vs FP64 computed from the bf16-rounded inputs."""
+    A, capi, tapes = _mods()
+    n = 64
+    rng = np.random.default_rng(9)
+    xt = torch.from_numpy(rng.standard_normal((n, n, n)).astype(np.float32)).to(dev).to(torch.bfloat16).requires_grad_()
+    yt = torch.from_numpy(rng.standard_normal((n, n, n)).astype(np.float32)).to(dev).to(torch.bfloat16).requires_grad_()
+    loss = A.laplacian_c0_loss(xt, yt)
+    loss.backward()
+    x, y = xt.detach().float().cpu().numpy(), yt.detach().float().cpu().numpy()
+    gx64, gy64 = grad64(x, y)
+    assert xt.grad.dtype == torch.bfloat16
+    # bf16 gradients: one rounding to bf16 (rel. 2^-8) of float32 results
+    assert np.max(np.abs(xt.grad.float().cpu().numpy() - gx64)) <= 8e-3 * np.max(np.abs(gx64))
+    assert np.max(np.abs(yt.grad.float().cpu().numpy() - gy64)) <= 8e-3 * np.max(np.abs(gy64))
+    L64 = loss64(x, y)
+    assert abs(loss.item() - L64) <= 1e-5 * abs(L64) + 1.0
+
+
+def test_unfused_composition_matches_fused(dev):
+    A, capi, tapes = _mods()
+    rng = np.random.default_rng(2)
+    x = torch.from_numpy(rng.standard_normal((32, 32, 32)).astype(np.float32)).to(dev)
+    y = torch.from_numpy(rng.standard_normal((32, 32, 32)).astype(np.float32)).to(dev)
+    x1, y1 = x.clone().requires_grad_(), y.clone().requires_grad_()
+    x2, y2 = x.clone().requires_grad_(), y.clone().requires_grad_()
+    A.laplacian_c0_loss(x1, y1).backward()
+    a = A.laplacian(x2)
+    tape = A.c0_tape(capi.FLOAT32, capi.FLOAT32, capi.FLOAT32)
+    A.sum(A.expr(tape, (0.5,), a, y2, a, a, y2)).backward()
+    assert torch.allclose(x1.grad, x2.grad, rtol=1e-5, atol=1e-5)
+    assert torch.allclose(y1.grad, y2.grad, rtol=1e-6, atol=1e-6)
