@@ -39,6 +39,44 @@ def _compile(src):
     return obj
 
 
+HOST = os.path.join(HERE, "host")
+HOST_LIB = os.path.join(HERE, "libnca_host.so")
+CXX = os.environ.get("CXX", "g++")
+HOST_FLAGS = ["-std=c++20", "-O2", "-fPIC", "-Wall", "-Wno-unused-result",
+              f"-I{os.path.join(ROOT, 'include')}", f"-I{os.path.join(HERE, 'include')}", f"-I{HOST}"]
+
+
+def _run(cmd, what):
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"{what} failed\n{' '.join(cmd)}\n{r.stderr}")
+
+
+def build_host(verbose=False):
+    """The C++ host API (libnca_host.so) and its Python module (_ncarray), host code only,
+    linked against libnca_hip.so in the same folder."""
+    import sysconfig
+    import pybind11
+    tmp = HOST_LIB + ".tmp"
+    _run([CXX] + HOST_FLAGS + ["-shared", os.path.join(HOST, "ncarray_amd.cpp"), "-o", tmp,
+          f"-L{HERE}", "-lnca_hip", "-Wl,-rpath,$ORIGIN"], "libnca_host.so")
+    os.replace(tmp, HOST_LIB)
+    mod = os.path.join(HERE, "_ncarray" + sysconfig.get_config_var("EXT_SUFFIX"))
+    tmp = mod + ".tmp"
+    _run([CXX] + HOST_FLAGS + ["-shared", f"-I{pybind11.get_include()}", f"-I{sysconfig.get_paths()['include']}",
+          "-fvisibility=hidden", os.path.join(HOST, "pyncarray.cpp"), "-o", tmp,
+          f"-L{HERE}", "-lnca_host", "-lnca_hip", "-Wl,-rpath,$ORIGIN"], "_ncarray module")
+    os.replace(tmp, mod)
+    test_src = os.path.join(ROOT, "tests", "cpp", "test_host_api.cpp")
+    if os.path.exists(test_src):
+        exe = os.path.join(ROOT, "tests", "cpp", "test_host_api")
+        _run([CXX] + HOST_FLAGS + [test_src, "-o", exe + ".tmp", f"-L{HERE}", "-lnca_host", "-lnca_hip",
+              f"-Wl,-rpath,{HERE}"], "C++ host API test")
+        os.replace(exe + ".tmp", exe)
+    if verbose:
+        print("built", HOST_LIB, mod)
+
+
 def build(verbose=False):
     os.makedirs(BUILD, exist_ok=True)
     jobs = min(len(SOURCES), max(1, int(os.environ.get("MAX_JOBS", os.cpu_count() or 4))))
@@ -53,9 +91,13 @@ def build(verbose=False):
     os.replace(tmp, LIB)
     if verbose:
         print("built", LIB)
+    build_host(verbose)
     return LIB
 
 
 if __name__ == "__main__":
-    build(verbose=True)
+    if len(sys.argv) > 1 and sys.argv[1] == "host":
+        build_host(verbose=True)
+    else:
+        build(verbose=True)
     sys.exit(0)

@@ -106,6 +106,20 @@ struct StencilFuse {
   int32_t axis0_dtype = NCA_FLOAT32;
 };
 
+// Kernel symbol of a march variant: storage types and fused reductions are part of the name, so
+// kernel traces and the library's profiler tell the variants apart.
+static std::string march_name(const StencilProgram& p, const StencilFuse& fz, int32_t dst_dtype) {
+  std::string n = std::string("nca_stencil_march_") + dtype_info(p.src_dtype).name;
+  if (dst_dtype != p.src_dtype) n += std::string("_") + dtype_info(dst_dtype).name;
+  if (fz.axis0) n += "_axis0";
+  if (fz.fuse & NCA_FUSE_SUM) n += "_sum";
+  if (fz.fuse & NCA_FUSE_MAX) n += "_max";
+  if (fz.fuse & NCA_FUSE_MIN) n += "_min";
+  if (!fz.with_dst) n += "_nodst";
+  if (!p.extra_dtypes.empty()) n += "_x" + std::to_string(p.extra_dtypes.size());
+  return n;
+}
+
 // ---- march kernel ---------------------------------------------------------------------------------
 static std::string gen_march(const StencilProgram& p, const StencilFuse& fz, int32_t dst_dtype) {
   const Tape& t = p.tape;
@@ -125,7 +139,7 @@ static std::string gen_march(const StencilProgram& p, const StencilFuse& fz, int
   std::ostringstream os;
   os << "#define RZ " << g.rz << "\n#define RY " << g.ry << "\n#define RX " << g.rx << "\n"
      << "#define VX " << march_vx(p.src_dtype) << "\n#define TX (16 * VX)\n#define TY 16\n#define NS (2 * RZ + 2)\n#define LW (TX + 2 * RX)\n#define LH (TY + 2 * RY)\n";
-  os << "extern \"C\" __global__ __launch_bounds__(256) void nca_stencil_march(const " << S << "* __restrict__ src";
+  os << "extern \"C\" __global__ __launch_bounds__(256) void " << march_name(p, fz, dst_dtype) << "(const " << S << "* __restrict__ src";
   for (size_t j = 0; j < p.extra_dtypes.size(); ++j) os << ", const " << ctype_of(p.extra_dtypes[j]) << "* __restrict__ e" << j;
   os << ", " << D << "* __restrict__ dst, nca_i64 NZ, nca_i64 NY, nca_i64 NX, int ntx, int nty, int zlen"
      << ", double* __restrict__ colpart, void* __restrict__ colout, void* __restrict__ psum, void* __restrict__ pmax"
@@ -522,7 +536,8 @@ static void stencil_apply_impl(uint64_t handle, const nca_view* src, const nca_v
     fz.axis0_dtype = axis0 ? axis0->dtype : NCA_FLOAT32;
     std::string key = "m|" + std::to_string(fz.axis0) + std::to_string(fz.axis0_dtype) + "|" + std::to_string(fuse) + "|" +
                       std::to_string(mode) + "|" + std::to_string(with_dst);
-    hipFunction_t fn = stencil_kernel(p, key, [&]() { return gen_march(p, fz, dst->dtype); }, "nca_stencil_march");
+    const std::string kname = march_name(p, fz, dst->dtype);
+    hipFunction_t fn = stencil_kernel(p, key, [&]() { return gen_march(p, fz, dst->dtype); }, kname);
     // One resident wave of blocks: (tiles x z runs) = CUs x blocks per CU for this kernel's
     // registers and LDS, so no block waits for another to finish (no tail), and runs are
     // as long as possible (fewer halo planes).
@@ -564,7 +579,7 @@ static void stencil_apply_impl(uint64_t handle, const nca_view* src, const nca_v
     params.push_back(&pmin);
     for (auto& x : sc.slots) params.push_back(&x);
     {
-      ProfScope ps("nca_stencil_march", s);
+      ProfScope ps(kname.c_str(), s);
       jit_launch(fn, dim3(static_cast<unsigned>(blocks)), dim3(256), 0, s, params);
     }
     if (colpart || fuse) {

@@ -1,0 +1,399 @@
+// Tests of the C++ host API (ncarray_amd.hh) on the GPU.  The cases follow the reference's
+// device tests (tests/cxx/test_gpu_reductions.cu:39-142, test_gpu_binary_operations.cu:28-217,
+// test_gpu_unary_operations.cu:19-93, test_gpu_copies.cu) with the same inputs and expected
+// values, plus stencil, fused-expression and variance cases.  A minimal harness stands in
+// for gtest (not available here): each TEST registers a case; main runs them all and exits
+// non-zero if any check failed.
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <functional>
+#include <random>
+#include <string>
+#include <vector>
+
+#include "ncarray_amd.hh"
+
+namespace nca = ncarray_amd;
+
+static int g_failed = 0, g_checks = 0;
+#define EXPECT_TRUE(c)                                                  \
+  do {                                                                  \
+    ++g_checks;                                                         \
+    if (!(c)) {                                                         \
+      ++g_failed;                                                       \
+      std::printf("  FAILED %s:%d: %s\n", __FILE__, __LINE__, #c);      \
+    }                                                                   \
+  } while (0)
+#define EXPECT_EQ(a, b) EXPECT_TRUE((a) == (b))
+#define EXPECT_FLOAT_EQ(a, b) EXPECT_TRUE(std::fabs(double(a) - double(b)) <= 4e-7 * std::fabs(double(b)))
+
+struct Case {
+  const char* name;
+  std::function<void()> fn;
+};
+static std::vector<Case>& cases() {
+  static std::vector<Case> c;
+  return c;
+}
+#define TEST(S, N)                                                                    \
+  static void S##_##N();                                                              \
+  static const bool S##_##N##_registered = (cases().push_back({#S "." #N, S##_##N}), true); \
+  static void S##_##N()
+
+using Shape = std::vector<int64_t>;
+
+// ---- reductions (test_gpu_reductions.cu) ------------------------------------------------------
+TEST(NCDevArrayReductionsTest, FullReductions) {
+  nca::NCDevArray dev_arr(Shape{256}, nca::DType::float32);
+  dev_arr = dev_arr.iota();
+  auto dev_sum = dev_arr.sum();
+  auto dev_max = dev_arr.max();
+  auto dev_min = dev_arr.min();
+  EXPECT_EQ(std::get<float>(dev_sum), 32640.0f);
+  EXPECT_EQ(std::get<float>(dev_max), 255.0f);
+  EXPECT_EQ(std::get<float>(dev_min), 0.0f);
+  EXPECT_EQ(std::get<int64_t>(dev_arr.argmax()), 255);
+  EXPECT_EQ(std::get<int64_t>(dev_arr.argmin()), 0);
+}
+
+TEST(NCDevArrayReductionsTest, AxisAwareReductions) {
+  nca::NCDevArray arr(Shape{2, 3}, nca::dtype_traits<int32_t>::value);
+  arr = arr.iota() + 1;  // [1, 2, 3], [4, 5, 6]
+  std::vector<int32_t> h(3);
+  nca::NCDevArray sum_ax0 = arr.sum({0});
+  EXPECT_EQ(sum_ax0.shape()[0], 3);
+  sum_ax0.copy_into_astype(h.data());
+  EXPECT_EQ(h[0], 5);
+  EXPECT_EQ(h[1], 7);
+  EXPECT_EQ(h[2], 9);
+  arr.max({0}).copy_into_astype(h.data());
+  EXPECT_EQ(h[0], 4);
+  EXPECT_EQ(h[1], 5);
+  EXPECT_EQ(h[2], 6);
+  arr.min({0}).copy_into_astype(h.data());
+  EXPECT_EQ(h[0], 1);
+  EXPECT_EQ(h[1], 2);
+  EXPECT_EQ(h[2], 3);
+  nca::NCDevArray sum_ax1 = arr.sum({1});
+  EXPECT_EQ(sum_ax1.shape()[0], 2);
+  sum_ax1.copy_into_astype(h.data());
+  EXPECT_EQ(h[0], 6);
+  EXPECT_EQ(h[1], 15);
+  arr.max({1}).copy_into_astype(h.data());
+  EXPECT_EQ(h[0], 3);
+  EXPECT_EQ(h[1], 6);
+  arr.min({1}).copy_into_astype(h.data());
+  EXPECT_EQ(h[0], 1);
+  EXPECT_EQ(h[1], 4);
+  // argmax along an axis stores the C-order linear index into the input
+  std::vector<int64_t> idx(3);
+  arr.argmax({0}).copy_into_astype(idx.data());
+  EXPECT_EQ(idx[0], 3);
+  EXPECT_EQ(idx[1], 4);
+  EXPECT_EQ(idx[2], 5);
+}
+
+TEST(NCDevArrayReductionsTest, SlicedFullReductions) {
+  nca::NCDevArray dev_arr(Shape{4, 4}, nca::DType::float32);
+  dev_arr.fill(1.0f);
+  using sl = nca::Slice;
+  auto view = dev_arr(sl(1, 3), sl(1, 3));
+  view.fill(10.0f);
+  auto total_sum = dev_arr.sum();
+  EXPECT_EQ(std::get<float>(total_sum), 52.0f);  // 12 * 1 + 4 * 10
+  EXPECT_EQ(std::get<float>(view.sum()), 40.0f);
+}
+
+TEST(NCDevArrayReductionsTest, MeanVarStd) {
+  std::vector<double> h(1000);
+  std::mt19937_64 rng(7);
+  std::normal_distribution<double> nd(3.0, 2.0);
+  for (double& x : h) x = nd(rng);
+  nca::NCDevArray d(Shape{10, 100}, nca::DType::float64);
+  d.assign_from_host(h.data(), nca::DType::float64, Shape{10, 100});
+  double m = 0;
+  for (double x : h) m += x;
+  m /= 1000.0;
+  double ss = 0;
+  for (double x : h) ss += (x - m) * (x - m);
+  EXPECT_TRUE(std::fabs(std::get<double>(d.mean()) - m) <= 1e-13 * std::fabs(m));
+  EXPECT_TRUE(std::fabs(std::get<double>(d.var()) - ss / 1000.0) <= 1e-12 * ss / 1000.0);
+  EXPECT_TRUE(std::fabs(std::get<double>(d.var(1)) - ss / 999.0) <= 1e-12 * ss / 999.0);
+  EXPECT_TRUE(std::fabs(std::get<double>(d.std()) - std::sqrt(ss / 1000.0)) <= 1e-12);
+  // per row
+  std::vector<double> vrow(10);
+  d.var(std::vector<int64_t>{1}).copy_into_astype(vrow.data());
+  for (int r = 0; r < 10; ++r) {
+    double mr = 0, sr = 0;
+    for (int c = 0; c < 100; ++c) mr += h[r * 100 + c];
+    mr /= 100.0;
+    for (int c = 0; c < 100; ++c) sr += (h[r * 100 + c] - mr) * (h[r * 100 + c] - mr);
+    EXPECT_TRUE(std::fabs(vrow[r] - sr / 100.0) <= 1e-12 * sr / 100.0);
+  }
+}
+
+// ---- copies (test_gpu_copies.cu) -------------------------------------------------------------------
+TEST(NCDevArrayCopies, GPUToHostCopy) {
+  nca::NCDevArray dev_arr(Shape{10}, nca::DType::float32);
+  dev_arr.fill(5.7f);
+  std::vector<int32_t> host_buffer(10);
+  dev_arr.copy_into_astype<int32_t>(host_buffer.data());
+  for (int i = 0; i < 10; ++i) EXPECT_EQ(host_buffer[i], 5);
+}
+
+TEST(NCDevArrayCopies, HostToGPUCopyCast) {
+  std::vector<float> host_arr(10, 5.7f);
+  nca::NCDevArray dev_arr(Shape{10}, nca::DType::uint32);
+  dev_arr.assign_from_host(host_arr.data(), nca::DType::float32, Shape{10});
+  std::vector<uint32_t> result_buffer(10);
+  dev_arr.copy_into(result_buffer.data());
+  for (int i = 0; i < 10; ++i) EXPECT_EQ(result_buffer[i], 5u);
+}
+
+TEST(NCDevArrayCopies, HostToGPUCopyNoCast) {
+  std::vector<float> host_arr(10, 5.7f);
+  nca::NCDevArray dev_arr(Shape{10}, nca::DType::float32);
+  dev_arr.assign_from_host(host_arr.data(), nca::DType::float32, Shape{10});
+  std::vector<float> result_buffer(10);
+  dev_arr.copy_into(result_buffer.data());
+  for (int i = 0; i < 10; ++i) EXPECT_EQ(result_buffer[i], 5.7f);
+}
+
+TEST(NCDevArrayCopies, StridedViews) {
+  nca::NCDevArray a = nca::NCDevArray::iota(Shape{6, 8}, nca::DType::int32);
+  auto v = a(nca::Slice(1, 6, 2), nca::Slice(7, nca::Slice::kNone, -3));  // rows 1,3,5; cols 7,4,1
+  EXPECT_EQ(v.shape(0), 3);
+  EXPECT_EQ(v.shape(1), 3);
+  std::vector<int32_t> h(9);
+  v.copy_into(h.data());
+  const int rows[3] = {1, 3, 5}, cols[3] = {7, 4, 1};
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) EXPECT_EQ(h[i * 3 + j], rows[i] * 8 + cols[j]);
+  EXPECT_EQ(std::get<int32_t>(v.sum()), (1 + 3 + 5) * 8 * 3 + (7 + 4 + 1) * 3);
+}
+
+// ---- binary operations (test_gpu_binary_operations.cu) ---------------------------------------------
+static std::vector<float> host_f32(const nca::NCDevArray& a) {
+  std::vector<float> h(static_cast<size_t>(a.size()));
+  a.copy_into_astype(h.data());
+  return h;
+}
+
+TEST(NCDevArrayBinaryOperationsTest, Addition) {
+  nca::NCDevArray a1(Shape{4, 4, 4}, nca::DType::float32), a2(Shape{4, 4, 4}, nca::DType::float32);
+  a1.fill(2.0f);
+  a2.fill(4.0f);
+  nca::NCDevArray dev_sum = a1 + a2;
+  EXPECT_TRUE(dev_sum.dtype() == nca::DType::float32);
+  for (float x : host_f32(dev_sum)) EXPECT_EQ(x, 6.0f);
+}
+
+TEST(NCDevArrayBinaryOperationsTest, Division) {
+  nca::NCDevArray a1(Shape{4, 4, 4}, nca::DType::float32), a2(Shape{4, 4, 4}, nca::DType::float32);
+  a1.fill(2.0f);
+  a2.fill(4.0f);
+  nca::NCDevArray dev_div = a1 / a2;  // float32 / float32 -> float64
+  EXPECT_TRUE(dev_div.dtype() == nca::DType::float64);
+  std::vector<double> h(64);
+  dev_div.copy_into(h.data());
+  for (double x : h) EXPECT_EQ(x, 0.5);
+}
+
+TEST(NCDevArrayBinaryOperationsTest, InplaceAddition) {
+  nca::NCDevArray a1(Shape{4, 4, 4}, nca::DType::float32), a2(Shape{4, 4, 4}, nca::DType::float32);
+  a1.fill(2.0f);
+  a2.fill(4.0f);
+  a1 += a2;
+  for (float x : host_f32(a1)) EXPECT_EQ(x, 6.0f);
+}
+
+TEST(NCDevArrayBinaryOperationsTest, InplaceDivision) {
+  nca::NCDevArray a1(Shape{4, 4, 4}, nca::DType::float32), a2(Shape{4, 4, 4}, nca::DType::float32);
+  a1.fill(6.0f);
+  a2.fill(4.0f);
+  a1 /= a2;  // stays float32
+  EXPECT_TRUE(a1.dtype() == nca::DType::float32);
+  for (float x : host_f32(a1)) EXPECT_EQ(x, 1.5f);
+}
+
+TEST(NCDevArrayBinaryOperationsTest, ScalarArithmetic) {
+  nca::NCDevArray a(Shape{100}, nca::DType::float32);
+  a.fill(10.0f);
+  nca::NCDevArray res = a + 5.0f;
+  for (float x : host_f32(res)) EXPECT_EQ(x, 15.0f);
+  a += 5.0f;
+  for (float x : host_f32(a)) EXPECT_EQ(x, 15.0f);
+}
+
+TEST(NCDevArrayBinaryOperationsTest, Modulo) {
+  std::vector<uint32_t> h1 = {10, 11, 12, 13, 14};
+  nca::NCDevArray d1(Shape{5}, nca::DType::uint32), d2(Shape{5}, nca::DType::uint32);
+  d1.assign_from_host(h1.data(), nca::DType::uint32, Shape{5});
+  d2.fill(3u);
+  nca::NCDevArray r = d1 % d2;
+  std::vector<uint32_t> h(5);
+  r.copy_into(h.data());
+  const uint32_t want[5] = {1, 2, 0, 1, 2};
+  for (int i = 0; i < 5; ++i) EXPECT_EQ(h[i], want[i]);
+}
+
+TEST(NCDevArrayBinaryOperationsTest, Comparisons) {
+  nca::NCDevArray a(Shape{10}, nca::DType::float32);
+  a.fill(10.0f);
+  auto view = a(nca::Slice(0, 5));
+  view = view.iota();  // [0, 1, 2, 3, 4, 10, 10, 10, 10, 10]
+  nca::NCDevArray mask = a > 5.0f;
+  EXPECT_TRUE(mask.dtype() == nca::DType::bool_);
+  std::vector<uint8_t> h(10);
+  mask.copy_into(h.data());
+  for (int i = 0; i < 10; ++i) EXPECT_EQ(h[i], i < 5 ? 0 : 1);
+  EXPECT_EQ(std::get<bool>(mask.any()), true);
+  EXPECT_EQ(std::get<bool>(mask.all()), false);
+}
+
+TEST(NCDevArrayBinaryOperationsTest, DivisionByZero) {
+  // interface.hh:150-156: x / 0 is NaN for finite x, x itself otherwise
+  std::vector<double> h = {1.0, -2.0, 0.0, INFINITY};
+  nca::NCDevArray a(Shape{4}, nca::DType::float64), z(Shape{4}, nca::DType::float64);
+  a.assign_from_host(h.data(), nca::DType::float64, Shape{4});
+  z.fill(0.0);
+  nca::NCDevArray r = a / z;
+  std::vector<double> o(4);
+  r.copy_into(o.data());
+  EXPECT_TRUE(std::isnan(o[0]) && std::isnan(o[1]) && std::isnan(o[2]));
+  EXPECT_TRUE(std::isinf(o[3]) && o[3] > 0);
+}
+
+// ---- unary operations (test_gpu_unary_operations.cu) ------------------------------------------------
+TEST(NCDevArrayUnaryOperationsTest, IncDecOps) {
+  nca::NCDevArray a(Shape{10}, nca::DType::float32);
+  a.fill(5.0f);
+  nca::NCDevArray neg = -a;
+  for (float x : host_f32(neg)) EXPECT_EQ(x, -5.0f);
+  ++a;
+  for (float x : host_f32(a)) EXPECT_EQ(x, 6.0f);
+  --a;
+  for (float x : host_f32(a)) EXPECT_EQ(x, 5.0f);
+}
+
+TEST(NCDevArrayUnaryOperationsTest, Iota) {
+  nca::NCDevArray a(Shape{3, 3}, nca::DType::int32);
+  nca::NCDevArray r1 = a.iota();
+  std::vector<int32_t> h(9);
+  r1.copy_into(h.data());
+  for (int i = 0; i < 9; ++i) EXPECT_EQ(h[i], i);
+  nca::NCDevArray r2 = nca::NCDevArray::iota(Shape{3, 3});  // int64 by default
+  EXPECT_TRUE(r2.dtype() == nca::DType::int64);
+  std::vector<int64_t> h2(9);
+  r2.copy_into(h2.data());
+  for (int i = 0; i < 9; ++i) EXPECT_EQ(h2[i], i);
+  nca::NCDevArray r3 = nca::NCDevArray::iota(Shape{3, 3}, nca::DType::float32);
+  for (int i = 0; i < 9; ++i) EXPECT_EQ(host_f32(r3)[i], static_cast<float>(i));
+  nca::NCDevArray r4 = a.iota() * 2 + 1;
+  r4.copy_into(h.data());
+  for (int i = 0; i < 9; ++i) EXPECT_EQ(h[i], i * 2 + 1);
+}
+
+// ---- fused expression and stencil -------------------------------------------------------------------
+TEST(FusedExpression, LazyTapeOneKernel) {
+  const int n = 1 << 16;
+  std::vector<float> ha(n), hb(n);
+  std::mt19937 rng(1);
+  std::uniform_real_distribution<float> u(-1.f, 1.f);
+  for (int i = 0; i < n; ++i) {
+    ha[i] = u(rng);
+    hb[i] = u(rng);
+  }
+  nca::NCDevArray a(Shape{n}), b(Shape{n});
+  a.assign_from_host(ha.data(), nca::DType::float32, Shape{n});
+  b.assign_from_host(hb.data(), nca::DType::float32, Shape{n});
+  nca::Expr e = a * b + nca::exp(-(a * a)) - 0.5f * b;
+  EXPECT_EQ(e.instrs.size(), 13u);
+  EXPECT_TRUE(e.dtype() == nca::DType::float32);
+  nca::NCDevArray out = e;
+  std::vector<float> ho = host_f32(out);
+  double sum64 = 0, scale = 0;
+  int worst_ok = 1;
+  for (int i = 0; i < n; ++i) {
+    const float p = ha[i] * hb[i], q = std::exp(-(ha[i] * ha[i])), r = 0.5f * hb[i];
+    const float want = (p + q) - r;  // only exp may differ (a few ulp of q)
+    const double tol = 4.0 * 1.2e-7 * (std::fabs(p) + q + std::fabs(r));
+    if (std::fabs(double(ho[i]) - double(want)) > tol) worst_ok = 0;
+    sum64 += ho[i];
+    scale += std::fabs(ho[i]);
+  }
+  EXPECT_TRUE(worst_ok);
+  const double fused = nca::scalar_to_double(e.sum());  // fused: nothing materialised
+  EXPECT_TRUE(std::fabs(fused - sum64) <= 1e-6 * scale);
+}
+
+TEST(StencilTest, LaplacianZeroBoundary) {
+  const int Z = 12, Y = 20, X = 32;
+  std::vector<float> h(Z * Y * X);
+  std::mt19937 rng(3);
+  std::normal_distribution<float> nd(0.f, 1.f);
+  for (float& v : h) v = nd(rng);
+  nca::NCDevArray src(Shape{Z, Y, X}), dst(Shape{Z, Y, X});
+  src.assign_from_host(h.data(), nca::DType::float32, Shape{Z, Y, X});
+  using Off = nca::Stencil<3>::Offset;
+  std::vector<Off> offs = {Off{-1, 0, 0}, Off{1, 0, 0}, Off{0, -1, 0}, Off{0, 1, 0}, Off{0, 0, -1}, Off{0, 0, 1}, Off{0, 0, 0}};
+  auto lap = nca::Stencil<3>::create<float>(offs, {0, 0, 0}, [](std::vector<nca::NCDevArray>& v) {
+    return v[0] + v[1] + v[2] + v[3] + v[4] + v[5] - 6.0f * v[6];
+  });
+  lap.apply(src, dst, std::nullopt, nca::Boundary::zero);
+  std::vector<float> out = host_f32(dst);
+  auto at = [&](int z, int y, int x) -> float {
+    if (z < 0 || y < 0 || x < 0 || z >= Z || y >= Y || x >= X) return 0.f;
+    return h[(z * Y + y) * X + x];
+  };
+  int bad = 0;
+  for (int z = 0; z < Z; ++z)
+    for (int y = 0; y < Y; ++y)
+      for (int x = 0; x < X; ++x) {
+        const float want = at(z - 1, y, x) + at(z + 1, y, x) + at(z, y - 1, x) + at(z, y + 1, x) + at(z, y, x - 1) +
+                           at(z, y, x + 1) - 6.0f * at(z, y, x);
+        if (out[(z * Y + y) * X + x] != want) ++bad;
+      }
+  EXPECT_EQ(bad, 0);  // same operation order in float32: bit-exact
+  // the reference contract (no bounds handling): an interior destination on a padded source
+  nca::NCDevArray inner(Shape{Z - 2, Y - 2, X - 2});
+  std::vector<Off> pos;
+  for (const Off& o : offs) pos.push_back(Off{o[0] + 1, o[1] + 1, o[2] + 1});
+  auto lap2 = nca::Stencil<3>::create<float>(pos, {0, 0, 0}, [](std::vector<nca::NCDevArray>& v) {
+    return v[0] + v[1] + v[2] + v[3] + v[4] + v[5] - 6.0f * v[6];
+  });
+  lap2.apply(src, inner);  // step = src.shape / dest.shape = 1
+  std::vector<float> o2 = host_f32(inner);
+  bad = 0;
+  for (int z = 0; z < Z - 2; ++z)
+    for (int y = 0; y < Y - 2; ++y)
+      for (int x = 0; x < X - 2; ++x)
+        if (o2[(z * (Y - 2) + y) * (X - 2) + x] != out[((z + 1) * Y + y + 1) * X + x + 1]) ++bad;
+  EXPECT_EQ(bad, 0);
+}
+
+int main(int argc, char** argv) {
+  const char* filter = argc > 1 ? argv[1] : nullptr;
+  int n_cases = 0, n_failed_cases = 0;
+  for (const Case& c : cases()) {
+    if (filter && std::strstr(c.name, filter) == nullptr) continue;
+    ++n_cases;
+    const int before = g_failed;
+    std::printf("[ RUN  ] %s\n", c.name);
+    std::fflush(stdout);
+    try {
+      c.fn();
+    } catch (const std::exception& e) {
+      ++g_failed;
+      std::printf("  EXCEPTION: %s\n", e.what());
+    }
+    const bool ok = g_failed == before;
+    if (!ok) ++n_failed_cases;
+    std::printf("[ %s ] %s\n", ok ? " OK " : "FAIL", c.name);
+    std::fflush(stdout);
+  }
+  std::printf("%d cases, %d failed, %d checks, %d failed checks\n", n_cases, n_failed_cases, g_checks, g_failed);
+  return g_failed ? 1 : 0;
+}
