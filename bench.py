@@ -29,6 +29,10 @@ HBM_PEAK_GBPS = 8000.0
 SHAPE = (512, 512, 512)
 # the fused step's kernel: float32 march stencil with the axis-0 sum and the max in its epilogue
 MARCH_KERNEL = "nca_stencil_march_float32_axis0_max"
+# HBM bytes per launch of that kernel from rocprofv3 PMC runs of this command (FETCH_SIZE
+# scaled by 2.0, the factor that makes a 512 MiB device copy read 512 MiB on this hardware;
+# WRITE_SIZE exact), recorded in profiles/nca_stencil_march_laplacian_512cubed.md.
+MARCH_TRAFFIC_BYTES = 1151061532
 
 
 def parse():
@@ -209,7 +213,8 @@ def main():
             "peak": HBM_PEAK_GBPS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBPS, 4),
-            "traffic": None,
+            "traffic": MARCH_TRAFFIC_BYTES,
+            "traffic_over_algorithmic": round(MARCH_TRAFFIC_BYTES / kernel_bytes, 4),
             "kernel": MARCH_KERNEL + " (Laplacian + axis-0 sum + max epilogue)",
             "bytes_per_launch": kernel_bytes,
             "avg_launch_ms": round(march_ms, 4),
@@ -241,6 +246,39 @@ def gbps(nbytes, ms):
     return round(nbytes / (ms * 1e-3) / 1e9, 2)
 
 
+def graph_loop(torch, fn, inner, reps, between=None):
+    """Capture `inner` calls of fn (which launch on torch's current stream) into one hipGraph
+    and time `reps` replays with events; returns the mean ms per call.  fn is warmed up on a
+    side stream first (kernels compiled, workspaces sized), so nothing allocates in capture."""
+    side = torch.cuda.Stream()
+    side.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(side):
+        for _ in range(3):
+            fn()
+            if between:
+                between()
+    torch.cuda.current_stream().wait_stream(side)
+    torch.cuda.synchronize()
+    graph = torch.cuda.CUDAGraph()
+    # capture on the warm-up stream: the library's per-stream workspace is already sized
+    with torch.cuda.graph(graph, stream=side):
+        for _ in range(inner):
+            if between:
+                between()
+            fn()
+    torch.cuda.synchronize()
+    graph.replay()
+    torch.cuda.synchronize()
+    evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(reps)]
+    cur = torch.cuda.current_stream()
+    for a, b in evs:
+        a.record(cur)
+        graph.replay()
+        b.record(cur)
+    torch.cuda.synchronize()
+    return float(np.mean([a.elapsed_time(b) for a, b in evs])) / inner, graph
+
+
 def bench_c0(torch, capi, tapes, ti, dev, sptr, time_loop):
     """out = a*b + exp(-a*a) - 0.5*b on 2^22 float32, then sum(out): fused in one kernel
     (out materialised), and the expression alone.  The 50 MB working set fits the 256 MB
@@ -256,13 +294,21 @@ def bench_c0(torch, capi, tapes, ti, dev, sptr, time_loop):
     ins = [ops[k] for k in tapes.C0_OPERANDS]
     vo = ti.view(out)
     fused = lambda: tape.launch_reduce(ins, vo, capi.FUSE_SUM, capi.ACCUM_COMPENSATED, s.data_ptr(), None, None,  # noqa: E731
-                                       scalars=(0.5,), stream=sptr)
-    only = lambda: tape.launch(ins, vo, scalars=(0.5,), stream=sptr)  # noqa: E731
+                                       scalars=(0.5,), stream=ti.stream_ptr())
+    only = lambda: tape.launch(ins, vo, scalars=(0.5,), stream=ti.stream_ptr())  # noqa: E731
     fm, _ = time_loop(fused, 50, 5)
     om, _ = time_loop(only, 50, 5)
+    torch.cuda.synchronize()
+    eager_sum = float(s.item())
+    gfm, _ = graph_loop(torch, fused, 20, 10)
+    gom, _ = graph_loop(torch, only, 20, 10)
     return {"n": n, "fused_ms": round(float(np.mean(fm)), 4), "fused_GBps": gbps(12 * n, np.mean(fm)),
             "expr_only_ms": round(float(np.mean(om)), 4), "expr_only_GBps": gbps(12 * n, np.mean(om)),
-            "note": "cache-resident (50 MB < 256 MB MALL); bytes = read a, b + write out"}
+            "graph_fused_ms": round(gfm, 4), "graph_fused_GBps": gbps(12 * n, gfm),
+            "graph_expr_only_ms": round(gom, 4), "graph_expr_only_GBps": gbps(12 * n, gom),
+            "graph_sum_equals_eager": bool(float(s.item()) == eager_sum),
+            "note": "cache-resident (50 MB < 256 MB MALL); bytes = read a, b + write out; graph_* = "
+                    "20 steps per hipGraph replay (no host launch overhead)"}
 
 
 def bench_c2(torch, capi, tapes, ti, dev, sptr, time_loop):
@@ -319,14 +365,37 @@ def bench_c3(torch, capi, tapes, ti, dev, sptr, time_loop):
 
         fm, _ = time_loop(fwd, 20, 3)
         bm, _ = time_loop(fwdbwd, 10, 2)
+        torch.cuda.synchronize()
+        # device time of one forward+backward step: the library's kernels, each bracketed by
+        # HIP events on its launch stream (host launch overhead excluded)
+        capi.profile_enable(True)
+        for _ in range(5):
+            fwdbwd()
+        torch.cuda.synchronize()
+        recs = capi.profile_read()
+        capi.profile_enable(False)
+        kms = float(sum(ms for _, ms in recs)) / 5
+        with torch.no_grad():
+            loss_eager = A.laplacian_c0_loss(x, y).clone()
+        gfm, _ = graph_loop(torch, fwd, 10, 10)  # forward only (launched from this thread)
         fwd_bytes = 2 * item * n3                  # read x, y
         fb_bytes = fwd_bytes + 4 * item * n3       # + read x, y, write dx, dy
+        frac = lambda nb, ms: round(nb / (ms * 1e-3) / 1e9 / HBM_PEAK_GBPS, 4)  # noqa: E731
         res[tag] = {"fwd_ms": round(float(np.mean(fm)), 4), "fwd_GBps": gbps(fwd_bytes, np.mean(fm)),
                     "fwd_bwd_ms": round(float(np.mean(bm)), 4), "fwd_bwd_GBps": gbps(fb_bytes, np.mean(bm)),
-                    "fwd_frac_of_8TBps": round(fwd_bytes / (np.mean(fm) * 1e-3) / 1e9 / HBM_PEAK_GBPS, 4),
-                    "fwd_bwd_frac_of_8TBps": round(fb_bytes / (np.mean(bm) * 1e-3) / 1e9 / HBM_PEAK_GBPS, 4)}
+                    "fwd_frac_of_8TBps": frac(fwd_bytes, np.mean(fm)),
+                    "fwd_bwd_frac_of_8TBps": frac(fb_bytes, np.mean(bm)),
+                    "graph_fwd_ms": round(gfm, 4), "graph_fwd_GBps": gbps(fwd_bytes, gfm),
+                    "graph_fwd_frac_of_8TBps": frac(fwd_bytes, gfm),
+                    "device_fwd_bwd_ms": round(kms, 4), "device_fwd_bwd_GBps": gbps(fb_bytes, kms),
+                    "device_fwd_bwd_frac_of_8TBps": frac(fb_bytes, kms),
+                    "device_fwd_bwd_kernels": sorted({n for n, _ in recs}),
+                    "loss": float(loss_eager)}
     res["note"] = ("bytes are algorithmic (each input read once, each output written once); the backward "
-                   "recomputes the Laplacian, runs the derivative expression and the transposed stencil")
+                   "recomputes the Laplacian, runs the derivative expression and the transposed stencil; "
+                   "graph_fwd replays a hipGraph of 10 forwards (no Python launch overhead); device_fwd_bwd "
+                   "sums the step's kernel durations (HIP events per launch); 134 MB working set: "
+                   "cache-resident")
     return res
 
 

@@ -26,6 +26,11 @@ extern "C" {
 
 #define NCA_MAX_NDIM 10 /* NCARRAY_MAX_NDIM */
 
+/* Stream argument meaning "the legacy default stream, asynchronously": work is ordered with
+ * everything else on the default stream (e.g. a framework's default stream) and the call
+ * returns without synchronising.  NULL keeps the reference's synchronous behaviour. */
+#define NCA_STREAM_DEFAULT_ASYNC ((void*)(uintptr_t)1)
+
 /* ---- status codes ------------------------------------------------------------- */
 enum {
   NCA_OK = 0,

@@ -32,6 +32,8 @@ ACCUM_PLAIN, ACCUM_COMPENSATED = 0, 1
 FUSE_SUM, FUSE_MAX, FUSE_MIN = 1, 2, 4
 BOUNDARY_NONE, BOUNDARY_ZERO = 0, 1
 COUNT_ALL, COUNT_NONZERO = 0, 1
+# stream argument: the legacy default stream without synchronising (NULL/None synchronises)
+STREAM_DEFAULT_ASYNC = 1
 
 ITEMSIZE = {BOOL: 1, CHAR: 1, UINT8: 1, UINT16: 2, UINT32: 4, UINT64: 8, INT8: 1, INT16: 2,
             INT32: 4, INT64: 8, FLOAT32: 4, FLOAT64: 8, BFLOAT16: 2, FLOAT16: 2}

@@ -358,7 +358,7 @@ int nca_hip_batch_launch(uint64_t plan, const nca_scalar* scalars, int32_t n_sca
     BatchPlan& p = plan_of(plan);
     batch_launch_impl(p, scalars, n_scalars, as_stream(stream));
     check_launch("batch");
-    finish_call(as_stream(stream));
+    finish_call(stream);
   });
 }
 
@@ -403,7 +403,7 @@ int nca_hip_graph_launch(uint64_t graph, void* stream) {
       ge = it->second;
     }
     HIP_CHECK(hipGraphLaunch(ge, as_stream(stream)));
-    finish_call(as_stream(stream));
+    finish_call(stream);
   });
 }
 

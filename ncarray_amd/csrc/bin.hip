@@ -51,9 +51,11 @@ __global__ __launch_bounds__(512) void k_bin(const T* __restrict__ frames, const
   const nca_i64 p0 = (nca_i64)c * chunk;
   const nca_i64 p1 = (p0 + chunk < P) ? p0 + chunk : P;
   const T* fr = frames + f * P;
-  auto flush = [&](int bin, V s, unsigned n) {
+  // runs of equal bins are folded in the accumulator type A (float64), so every pixel value
+  // enters the sum exactly
+  auto flush = [&](int bin, A s, unsigned n) {
     if (bin >= 0 && bin < nb && n) {
-      atomicAdd(&hs[bin], static_cast<A>(s));
+      atomicAdd(&hs[bin], s);
       atomicAdd(&hc[bin], n);
     }
   };
@@ -65,14 +67,14 @@ __global__ __launch_bounds__(512) void k_bin(const T* __restrict__ frames, const
     const nca_vec<I, 4> m = *reinterpret_cast<const nca_vec<I, 4>*>(map + p);
     int cur = static_cast<int>(m.v[0]);
     if (m.v[0] < 0 || m.v[0] >= nb) cur = -1;
-    V s = 0;
+    A s = 0;
     unsigned n = 0;
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
       const V val = nca_cvt<V, T>::from(x.v[j]);
       int bin = (m.v[j] < 0 || m.v[j] >= nb) ? -1 : static_cast<int>(m.v[j]);
       if (bin != cur) { flush(cur, s, n); cur = bin; s = 0; n = 0; }
-      s += val;
+      s += static_cast<A>(val);
       n += NONZERO ? (val != V(0) ? 1u : 0u) : 1u;
     }
     flush(cur, s, n);
@@ -80,7 +82,7 @@ __global__ __launch_bounds__(512) void k_bin(const T* __restrict__ frames, const
   for (nca_i64 p = p0 + 4 * nvec + threadIdx.x; p < p1; p += blockDim.x) {
     const V val = nca_cvt<V, T>::from(fr[p]);
     const I bi = map[p];
-    if (bi >= 0 && bi < nb) flush(static_cast<int>(bi), val, NONZERO ? (val != V(0) ? 1u : 0u) : 1u);
+    if (bi >= 0 && bi < nb) flush(static_cast<int>(bi), static_cast<A>(val), NONZERO ? (val != V(0) ? 1u : 0u) : 1u);
   }
   __syncthreads();
   // partial slot of this (frame, chunk)
@@ -279,7 +281,7 @@ int nca_hip_bin(const nca_view* frames, const nca_view* binmap, int32_t n_bins, 
     else if (s64) launch_combine((int*)nullptr, (double*)nullptr);
     else launch_combine((int*)nullptr, (float*)nullptr);
     check_launch("bin");
-    finish_call(s);
+    finish_call(stream);
   });
 }
 
@@ -334,7 +336,7 @@ int nca_hip_bin_backward(const nca_view* grad_sums, const nca_view* binmap, cons
       default: NCA_THROW(NCA_ERR_UNSUPPORTED, "grad_sums must be float32 or float64");
     }
     check_launch("bin_backward");
-    finish_call(s);
+    finish_call(stream);
   });
 }
 
@@ -379,7 +381,7 @@ int nca_hip_scatter_add(const nca_view* dest, const nca_view* idx, const nca_vie
       default: NCA_THROW(NCA_ERR_UNSUPPORTED, "scatter_add: unsupported dest dtype");
     }
     check_launch("scatter_add");
-    finish_call(s);
+    finish_call(stream);
   });
 }
 

@@ -594,7 +594,7 @@ int nca_hip_expr_launch(uint64_t handle, const nca_view* ins, int32_t n_in, cons
     expr_launch_impl(handle, ins, n_in, scalars, n_scalars, out, 0, NCA_ACCUM_PLAIN, nullptr, nullptr, nullptr,
                      as_stream(stream));
     check_launch("expression");
-    finish_call(as_stream(stream));
+    finish_call(stream);
   });
 }
 
@@ -609,7 +609,7 @@ int nca_hip_expr_launch_reduce(uint64_t handle, const nca_view* ins, int32_t n_i
     expr_launch_impl(handle, ins, n_in, scalars, n_scalars, out, fuse, accum_mode, sum_out, max_out, min_out,
                      as_stream(stream));
     check_launch("fused expression");
-    finish_call(as_stream(stream));
+    finish_call(stream);
   });
 }
 
@@ -679,7 +679,7 @@ int nca_hip_expr_vjp(uint64_t handle, const nca_view* ins, int32_t n_in, const n
     int blocks = grid_blocks(n / vec > 0 ? n / vec : 1, 256, dev.n_cus * 16);
     if (n > 0) jit_launch(fn, dim3(blocks), dim3(256), 0, as_stream(stream), params);
     check_launch("vjp");
-    finish_call(as_stream(stream));
+    finish_call(stream);
   });
 }
 

@@ -85,10 +85,13 @@ struct DeviceState {
 DeviceState& current_device();           // throws NCA_ERR_NOT_INIT when not initialised
 bool device_available();                  // a device was initialised on this thread
 Workspace& workspace(hipStream_t s, size_t partial_bytes, size_t big_bytes = 0);
-void finish_call(hipStream_t s);          // synchronise when s is the null stream
+void finish_call(void* stream);           // synchronise when the C-ABI stream argument is NULL
 void check_launch(const char* what);
 
-inline hipStream_t as_stream(void* s) { return reinterpret_cast<hipStream_t>(s); }
+// The HIP stream of a C-ABI stream argument (NCA_STREAM_DEFAULT_ASYNC is the null stream).
+inline hipStream_t as_stream(void* s) {
+  return s == NCA_STREAM_DEFAULT_ASYNC ? nullptr : reinterpret_cast<hipStream_t>(s);
+}
 
 // ---- views ---------------------------------------------------------------------------
 struct ViewDesc {

@@ -844,7 +844,7 @@ int nca_hip_reduce_full(const nca_view* in, int32_t op, int32_t accum_mode, void
     FullArgs a{in, op, accum_mode, out_dev, as_stream(stream)};
     dispatch_dtype<RunFull>(in->dtype, a);
     check_launch("reduce_full");
-    finish_call(as_stream(stream));
+    finish_call(stream);
   });
 }
 
@@ -870,7 +870,7 @@ int nca_hip_reduce_axes(const nca_view* in, const int64_t* axes, int32_t n_axes,
                 std::string("out dtype must be ") + dtype_info(reduce_result_dtype(op, in->dtype)).name);
     dispatch_dtype<RunAxes>(in->dtype, a);
     check_launch("reduce_axes");
-    finish_call(as_stream(stream));
+    finish_call(stream);
   });
 }
 

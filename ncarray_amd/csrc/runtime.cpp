@@ -78,7 +78,7 @@ Workspace& workspace(hipStream_t s, size_t partial_bytes, size_t big_bytes) {
       (big_bytes > 0 && w.big_bytes < big_bytes)) {
     if (capturing())
       NCA_THROW(NCA_ERR_INVALID,
-                "workspace must grow during hipGraph capture; run the call once before capturing");
+                "workspace of this stream must grow during hipGraph capture; workspaces are per stream: run the call once on the capturing stream before capturing");
     // Grow: the device must be idle for this stream before old buffers are freed.
     HIP_CHECK(hipStreamSynchronize(s));
   }
@@ -105,8 +105,8 @@ Workspace& workspace(hipStream_t s, size_t partial_bytes, size_t big_bytes) {
   return w;
 }
 
-void finish_call(hipStream_t s) {
-  if (s == nullptr) HIP_CHECK(hipStreamSynchronize(nullptr));
+void finish_call(void* stream) {
+  if (stream == nullptr) HIP_CHECK(hipStreamSynchronize(nullptr));
 }
 
 // ---- profiling ---------------------------------------------------------------------------
@@ -275,7 +275,7 @@ int nca_hip_memcpy_h2d(void* dst, const void* src, size_t bytes, void* stream) {
   return api_call([&] {
     if (bytes == 0) return;
     HIP_CHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, as_stream(stream)));
-    finish_call(as_stream(stream));
+    finish_call(stream);
   });
 }
 
@@ -283,7 +283,7 @@ int nca_hip_memcpy_d2h(void* dst, const void* src, size_t bytes, void* stream) {
   return api_call([&] {
     if (bytes == 0) return;
     HIP_CHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, as_stream(stream)));
-    finish_call(as_stream(stream));
+    finish_call(stream);
   });
 }
 
@@ -291,7 +291,7 @@ int nca_hip_memcpy_d2d(void* dst, const void* src, size_t bytes, void* stream) {
   return api_call([&] {
     if (bytes == 0) return;
     HIP_CHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, as_stream(stream)));
-    finish_call(as_stream(stream));
+    finish_call(stream);
   });
 }
 

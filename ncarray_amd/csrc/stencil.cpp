@@ -715,7 +715,7 @@ int nca_hip_stencil_apply(uint64_t handle, const nca_view* src, const nca_view* 
     stencil_apply_impl(handle, src, extras, n_extra, scalars, n_scalars, dst, boundary, ext_args, n_ext_args, nullptr,
                        0, NCA_ACCUM_PLAIN, nullptr, nullptr, nullptr, as_stream(stream));
     check_launch("stencil");
-    finish_call(as_stream(stream));
+    finish_call(stream);
   });
 }
 
@@ -731,7 +731,7 @@ int nca_hip_stencil_apply_reduce(uint64_t handle, const nca_view* src, const nca
     stencil_apply_impl(handle, src, extras, n_extra, scalars, n_scalars, dst, boundary, nullptr, 0, axis0_sum, fuse,
                        accum_mode, sum_out, max_out, min_out, as_stream(stream));
     check_launch("fused stencil");
-    finish_call(as_stream(stream));
+    finish_call(stream);
   });
 }
 

@@ -32,4 +32,5 @@ def shape_view(shape, dtype_code):
 def stream_ptr(stream=None):
     """The raw hipStream_t of a torch stream (None: the current stream)."""
     s = torch.cuda.current_stream() if stream is None else stream
-    return s.cuda_stream or None
+    # torch's default stream is the legacy null stream: order with it without synchronising
+    return s.cuda_stream or capi.STREAM_DEFAULT_ASYNC
