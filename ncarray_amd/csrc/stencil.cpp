@@ -15,6 +15,8 @@
 //  * direct (any rank, any step, pro/epilogue hooks): one output per thread, one
 //    predicated load per offset, the reference's launch shape (ceil(n/256) x 256).
 #include <cstdio>
+#include <map>
+#include <set>
 #include <sstream>
 #include <unordered_map>
 
@@ -352,6 +354,311 @@ static std::string gen_march(const StencilProgram& p, const StencilFuse& fz, int
   return os.str();
 }
 
+// ---- march kernel with a register z-queue -----------------------------------------------------------
+// For stencils whose z-offsets are pure (dz != 0 only with dy = dx = 0), e.g. the 7-point
+// Laplacian: each thread keeps its own columns of planes z-RZ .. z+RZ+1 in registers (a queue
+// unrolled so that every slot index is a constant), and LDS holds only the current plane
+// (double-buffered, one barrier per plane) for the x/y neighbours.  The LDS interior starts at a
+// 16-byte aligned column, so a row of VX values is one vector read; only the elements an
+// x-offset shifts across the thread's VX columns are scalar reads, and x-offsets within a row
+// are taken from registers.  Each thread computes RPT rows (ty + 16 r).
+struct MarchPlan {
+  bool queue = false;
+  int vx = 4;
+  int rpt = 1;
+};
+
+static MarchPlan march_plan(const StencilProgram& p) {
+  MarchPlan m;
+  m.vx = march_vx(p.src_dtype);
+  if (p.ndim != 3) return m;
+  const MarchGeom g = geom_of(p);
+  bool pure_z = true;
+  for (auto& o : p.offsets)
+    if (o[0] != 0 && (o[1] != 0 || o[2] != 0)) pure_z = false;
+  auto halo_threads = [&](int rpt) { return 2 * g.ry * 16 + 2 * g.rx * (16 * rpt + 2 * g.ry); };
+  m.queue = pure_z && g.rx <= 4;
+  m.rpt = (m.queue && itemsize(p.src_dtype) >= 4) ? 2 : 1;
+  if (halo_threads(m.rpt) > 256) m.rpt = 1;
+  if (halo_threads(m.rpt) > 256) m.queue = false;
+  if (!m.queue) m.rpt = 1;
+  return m;
+}
+
+static std::string gen_march_q(const StencilProgram& p, const StencilFuse& fz, int32_t dst_dtype, const MarchPlan& mp) {
+  const Tape& t = p.tape;
+  const MarchGeom g = geom_of(p);
+  const std::string C = ctype_of(t.compute_dtype);
+  const std::string S = ctype_of(p.src_dtype);
+  const std::string D = ctype_of(dst_dtype);
+  const int n_arr = static_cast<int>(t.arr_dtypes.size());
+  std::vector<int> alias(n_arr);
+  for (int k = 0; k < n_arr; ++k) alias[k] = k;
+  std::vector<std::string> roots;
+  std::string body = emit_nodes(t, {t.root}, alias, names_u(n_arr), names_s(static_cast<int>(t.scalar_dtypes.size())),
+                                "lin", "t", roots);
+  const int32_t rcd = compute_dtype(dst_dtype);
+  const std::string RC = ctype_of(rcd);
+  const bool comp = fz.mode == NCA_ACCUM_COMPENSATED;
+  const int VX = mp.vx, RPT = mp.rpt, RZ = g.rz, RY = g.ry, RX = g.rx;
+  const int Q = 2 * RZ + 2;
+  const int TX = 16 * VX, TY = 16 * RPT, LH = TY + 2 * RY, PADL = 4, LW = PADL + TX + 4;
+  auto I = [](long long v) { return std::to_string(v); };
+  std::ostringstream os;
+  os << "#define VX " << VX << "\n#define TX " << TX << "\n#define TY " << TY << "\n#define LH " << LH << "\n#define LW " << LW
+     << "\n#define PADL " << PADL << "\n#define RY " << RY << "\n#define RX " << RX << "\n";
+  os << "extern \"C\" __global__ __launch_bounds__(256) void " << march_name(p, fz, dst_dtype) << "(const " << S
+     << "* __restrict__ src";
+  for (size_t j = 0; j < p.extra_dtypes.size(); ++j) os << ", const " << ctype_of(p.extra_dtypes[j]) << "* __restrict__ e" << j;
+  os << ", " << D << "* __restrict__ dst, nca_i64 NZ, nca_i64 NY, nca_i64 NX, int ntx, int nty, int zlen"
+     << ", double* __restrict__ colpart, void* __restrict__ colout, void* __restrict__ psum, void* __restrict__ pmax"
+     << ", void* __restrict__ pmin";
+  for (size_t i = 0; i < t.scalar_dtypes.size(); ++i) os << ", " << C << " s" << i;
+  os << ") {\n";
+  // LDS rows are 16-byte aligned (not aligned to a whole VX vector when VX * sizeof(C) > 16)
+  os << "  struct __attribute__((aligned(16))) lvec { " << C << " v[VX]; };\n";
+  os << "  __shared__ __attribute__((aligned(16))) " << C << " lds[2][LH][LW];\n";
+  os << "  const unsigned nb = gridDim.x, b = blockIdx.x;\n"
+     << "  const unsigned tix = (nb % 8u == 0u) ? (b % 8u) * (nb / 8u) + b / 8u : b;\n"
+     << "  const int txi = tix % ntx, tyi = (tix / ntx) % nty, tzi = tix / (ntx * nty);\n"
+     << "  const nca_i64 x0 = (nca_i64)txi * TX, y0 = (nca_i64)tyi * TY, z0 = (nca_i64)tzi * zlen;\n"
+     << "  const nca_i64 z1 = (z0 + zlen < NZ) ? z0 + zlen : NZ;\n"
+     << "  const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;\n"
+     << "  const nca_i64 gx = x0 + VX * tx;\n"
+     << "  const size_t plane = (size_t)NY * (size_t)NX;\n  const int NZi = (int)NZ;\n"
+     << "  const int hid = threadIdx.x;\n";
+  for (int r = 0; r < RPT; ++r)
+    os << "  const nca_i64 gy" << r << " = y0 + ty + " << 16 * r << ";\n"
+       << "  const bool valid" << r << " = (gx < NX) && (gy" << r << " < NY);\n"
+       << "  const unsigned ioff" << r << " = valid" << r << " ? (unsigned)(gy" << r << " * NX + gx) : 0u;\n";
+  const int nhr = 2 * RY * 16, nhc = 2 * RX * LH;
+  // halo roles: halo-row vectors (threads [0, nhr)) and halo-column scalars ([nhr, nhr + nhc))
+  os << "  bool hrok = false;\n  unsigned hroff = 0u;\n  int hrrow = 0, hrcol = 0;\n";
+  if (nhr > 0)
+    os << "  if (hid < " << nhr << ") {\n"
+       << "    const int hr = hid >> 4, hx = hid & 15;\n"
+       << "    const nca_i64 yy = y0 + ((hr < RY) ? (hr - RY) : (TY + hr - RY));\n"
+       << "    const nca_i64 xx = x0 + VX * hx;\n"
+       << "    hrok = (yy >= 0) && (yy < NY) && (xx < NX);\n"
+       << "    hroff = hrok ? (unsigned)(yy * NX + xx) : 0u;\n"
+       << "    hrrow = (hr < RY) ? hr : (TY + hr);\n    hrcol = PADL + VX * hx;\n"
+       << "  }\n";
+  os << "  bool hcok = false;\n  unsigned hcoff = 0u;\n  int hcrow = 0, hccol = 0;\n";
+  if (nhc > 0)
+    os << "  if (hid >= " << nhr << " && hid < " << nhr + nhc << ") {\n"
+       << "    const int q = hid - " << nhr << ", hc = q / LH, hrow = q - hc * LH;\n"
+       << "    const nca_i64 xx = x0 + ((hc < RX) ? (hc - RX) : (TX + hc - RX));\n"
+       << "    const nca_i64 yy = y0 + hrow - RY;\n"
+       << "    hcok = (yy >= 0) && (yy < NY) && (xx >= 0) && (xx < NX);\n"
+       << "    hcoff = hcok ? (unsigned)(yy * NX + xx) : 0u;\n"
+       << "    hcrow = hrow;\n    hccol = (hc < RX) ? (PADL - RX + hc) : (PADL + TX + hc - RX);\n"
+       << "  }\n";
+  os << "  " << C << " cq[" << Q << "][" << RPT << "][VX];\n";
+  if (nhr > 0) os << "  " << C << " hq[" << Q << "][VX];\n";
+  if (nhc > 0) os << "  " << C << " hcq[" << Q << "];\n";
+  // accumulators
+  if (fz.axis0) {
+    os << "  float cs[" << RPT << "][VX] = {};\n";
+    if (comp) os << "  float cc[" << RPT << "][VX] = {};\n";
+  }
+  if (fz.fuse & NCA_FUSE_SUM) {
+    os << "  " << RC << " acc_s = 0;\n";
+    if (comp) os << "  " << RC << " acc_c = 0;\n";
+  }
+  if (fz.fuse & NCA_FUSE_MAX) os << "  " << RC << " acc_mx = nca_lim<" << RC << ">::lowest();\n";
+  if (fz.fuse & NCA_FUSE_MIN) os << "  " << RC << " acc_mn = nca_lim<" << RC << ">::max();\n";
+  const std::string zero = "(" + C + ")0";
+  // plane pz (an int expression) into register slot q; the plane starts at element offset off
+  auto gload = [&](int q, const std::string& pz, const std::string& off) {
+    os << "    {\n      const bool zin = (unsigned)(" << pz << ") < (unsigned)NZi;\n"
+       << "      const " << S << "* __restrict__ pl = src + (zin ? (" << off << ") : (size_t)0);\n";
+    for (int r = 0; r < RPT; ++r) {
+      os << "      { const bool ok = zin && valid" << r << ";\n"
+         << "        const nca_vec<" << S << ", VX> v = *reinterpret_cast<const nca_vec<" << S << ", VX>*>(ok ? pl + ioff" << r
+         << " : src);\n";
+      for (int j = 0; j < VX; ++j)
+        os << "        cq[" << q << "][" << r << "][" << j << "] = ok ? nca_cvt<" << C << ", " << S << ">::from(v.v[" << j
+           << "]) : " << zero << ";\n";
+      os << "      }\n";
+    }
+    if (nhr > 0) {
+      os << "      if (hid < " << nhr << ") { const bool ok = zin && hrok;\n"
+         << "        const nca_vec<" << S << ", VX> v = *reinterpret_cast<const nca_vec<" << S << ", VX>*>(ok ? pl + hroff : src);\n";
+      for (int j = 0; j < VX; ++j)
+        os << "        hq[" << q << "][" << j << "] = ok ? nca_cvt<" << C << ", " << S << ">::from(v.v[" << j << "]) : " << zero
+           << ";\n";
+      os << "      }\n";
+    }
+    if (nhc > 0)
+      os << "      if (hid >= " << nhr << " && hid < " << nhr + nhc << ") { const bool ok = zin && hcok;\n"
+         << "        const " << C << " v = nca_ld<" << C << ">(ok ? pl + hcoff : src);\n"
+         << "        hcq[" << q << "] = ok ? v : " << zero << "; }\n";
+    os << "    }\n";
+  };
+  auto lstore = [&](int q, int buf) {
+    for (int r = 0; r < RPT; ++r) {
+      os << "    { lvec w;\n";
+      for (int j = 0; j < VX; ++j) os << "      w.v[" << j << "] = cq[" << q << "][" << r << "][" << j << "];\n";
+      os << "      *reinterpret_cast<lvec*>(&lds[" << buf << "][RY + ty + " << 16 * r << "][PADL + VX * tx]) = w; }\n";
+    }
+    if (nhr > 0) {
+      os << "    if (hid < " << nhr << ") { lvec w;\n";
+      for (int j = 0; j < VX; ++j) os << "      w.v[" << j << "] = hq[" << q << "][" << j << "];\n";
+      os << "      *reinterpret_cast<lvec*>(&lds[" << buf << "][hrrow][hrcol]) = w; }\n";
+    }
+    if (nhc > 0)
+      os << "    if (hid >= " << nhr << " && hid < " << nhr + nhc << ") lds[" << buf << "][hcrow][hccol] = hcq[" << q << "];\n";
+  };
+  // rows (dy) read from LDS, and how far x-offsets reach left/right of the VX columns
+  std::map<int64_t, std::pair<int, int>> reach;  // dy -> (left, right) element counts
+  std::set<int64_t> vec_rows;                     // dy != 0 rows read as vectors
+  for (auto& o : p.offsets) {
+    if (o[0] != 0) continue;
+    auto& rr = reach[o[1]];
+    if (o[2] < 0) rr.first = std::max<int>(rr.first, static_cast<int>(-o[2]));
+    if (o[2] > 0) rr.second = std::max<int>(rr.second, static_cast<int>(o[2]));
+    if (o[1] != 0) vec_rows.insert(o[1]);
+  }
+  auto nm = [](int64_t v) { return v < 0 ? "m" + std::to_string(-v) : std::to_string(v); };
+  auto compute = [&](int k, int buf) {
+    // slot of plane z + k + dz
+    auto slot = [&](int64_t dz) { return static_cast<int>(((k + RZ + dz) % Q + Q) % Q); };
+    for (int r = 0; r < RPT; ++r) {
+      os << "    {\n";
+      for (int64_t dy : vec_rows)
+        os << "      const lvec V" << nm(dy) << " = *reinterpret_cast<const lvec*>(&lds[" << buf << "][RY + ty + " << 16 * r
+           << " + (" << dy << ")][PADL + VX * tx]);\n";
+      for (auto& kv : reach) {
+        for (int m = 1; m <= kv.second.first; ++m)
+          os << "      const " << C << " L" << nm(kv.first) << "_" << m << " = lds[" << buf << "][RY + ty + " << 16 * r << " + ("
+             << kv.first << ")][PADL + VX * tx - " << m << "];\n";
+        for (int m = 1; m <= kv.second.second; ++m)
+          os << "      const " << C << " R" << nm(kv.first) << "_" << m << " = lds[" << buf << "][RY + ty + " << 16 * r << " + ("
+             << kv.first << ")][PADL + VX * tx + VX - 1 + " << m << "];\n";
+      }
+      for (size_t jx = 0; jx < p.extra_dtypes.size(); ++jx) {
+        const std::string E = ctype_of(p.extra_dtypes[jx]);
+        os << "      nca_vec<" << E << ", VX> X" << jx << " = {};\n"
+           << "      if (valid" << r << ") X" << jx << " = *reinterpret_cast<const nca_vec<" << E << ", VX>*>(e" << jx
+           << " + soff + ioff" << r << ");\n";
+      }
+      os << "      nca_vec<" << D << ", VX> res;\n";
+      for (int j = 0; j < VX; ++j) {
+        os << "      {\n";
+        for (int kk = 0; kk < p.n_off; ++kk) {
+          const auto& o = p.offsets[kk];
+          std::string v;
+          if (o[0] != 0) {
+            v = "cq[" + I(slot(o[0])) + "][" + I(r) + "][" + I(j) + "]";
+          } else {
+            const int64_t idx = j + o[2];
+            if (idx >= 0 && idx < VX) v = o[1] == 0 ? "cq[" + I(slot(0)) + "][" + I(r) + "][" + I(idx) + "]"
+                                                    : "V" + nm(o[1]) + ".v[" + I(idx) + "]";
+            else if (idx < 0) v = "L" + nm(o[1]) + "_" + I(-idx);
+            else v = "R" + nm(o[1]) + "_" + I(idx - VX + 1);
+          }
+          os << "        const " << C << " u" << kk << " = " << v << ";\n";
+        }
+        for (size_t jx = 0; jx < p.extra_dtypes.size(); ++jx)
+          os << "        const " << C << " u" << (p.n_off + jx) << " = nca_cvt<" << C << ", " << ctype_of(p.extra_dtypes[jx])
+             << ">::from(X" << jx << ".v[" << j << "]);\n";
+        if (t.uses_idx) os << "        const nca_i64 lin = (nca_i64)soff + ioff" << r << " + " << j << ";\n";
+        os << body;
+        os << "        res.v[" << j << "] = nca_store_cvt<" << D << ", " << C << ">::to(" << roots[0] << ");\n"
+           << "      }\n";
+      }
+      if (fz.axis0 || fz.fuse) {
+        os << "      if (valid" << r << ") {\n";
+        for (int j = 0; j < VX; ++j)
+          os << "        const " << RC << " rv" << j << " = nca_cvt<" << RC << ", " << D << ">::from(res.v[" << j << "]);\n";
+        if (fz.axis0)
+          for (int j = 0; j < VX; ++j) {
+            if (comp) os << "        nca_neumaier_add(cs[" << r << "][" << j << "], cc[" << r << "][" << j << "], (float)rv" << j << ");\n";
+            else os << "        cs[" << r << "][" << j << "] += (float)rv" << j << ";\n";
+          }
+        auto tree = [&](const std::string& f) {
+          std::vector<std::string> v;
+          for (int j = 0; j < VX; ++j) v.push_back("rv" + std::to_string(j));
+          while (v.size() > 1) {
+            std::vector<std::string> w;
+            for (size_t q = 0; q + 1 < v.size(); q += 2) w.push_back(f + "(" + v[q] + ", " + v[q + 1] + ")");
+            if (v.size() % 2) w.push_back(v.back());
+            v = w;
+          }
+          return v[0];
+        };
+        if (fz.fuse & NCA_FUSE_SUM) {
+          os << "        const " << RC << " qs = " << tree("nca_add2") << ";\n";
+          if (comp) os << "        nca_neumaier_add(acc_s, acc_c, qs);\n";
+          else os << "        acc_s += qs;\n";
+        }
+        if (fz.fuse & NCA_FUSE_MAX) os << "        acc_mx = nca_maxop(acc_mx, " << tree("nca_maxop") << ");\n";
+        if (fz.fuse & NCA_FUSE_MIN) os << "        acc_mn = nca_minop(acc_mn, " << tree("nca_minop") << ");\n";
+        os << "      }\n";
+      }
+      if (fz.with_dst)
+        os << "      if (valid" << r << ") nca_store_nt(reinterpret_cast<nca_vec<" << D << ", VX>*>(dst + soff + ioff" << r
+           << "), res);\n";
+      os << "    }\n";
+    }
+  };
+  // prologue: planes zb-RZ .. zb+RZ into slots 0 .. 2RZ
+  os << "  const int zb = (int)z0, ze = (int)z1;\n";
+  for (int m = 0; m <= 2 * RZ; ++m) {
+    const std::string pz = "zb - " + I(RZ) + " + " + I(m);
+    gload(m, pz, "(size_t)(" + pz + ") * plane");
+  }
+  // steady state: step k of an unrolled group handles plane z = z' + k; it stores plane z
+  // (slot (k + RZ) % Q) into LDS buffer k & 1, loads plane z + RZ + 1 into the slot just
+  // freed ((k - 1) mod Q) and computes plane z.
+  os << "  size_t loff = (size_t)(zb + " << RZ + 1 << ") * plane;\n"
+     << "  size_t soff = (size_t)zb * plane;\n"
+     << "  for (int z = zb; z < ze; z += " << Q << ") {\n";
+  for (int k = 0; k < Q; ++k) {
+    if (k > 0) os << "    if (z + " << k << " >= ze) break;\n";
+    lstore((k + RZ) % Q, k & 1);
+    os << "    __syncthreads();\n";
+    gload((k + Q - 1) % Q, "z + " + I(k + RZ + 1), "loff");
+    os << "    loff += plane;\n";
+    compute(k, k & 1);
+    os << "    soff += plane;\n";
+  }
+  os << "  }\n";
+  // epilogue (as in gen_march)
+  if (fz.axis0) {
+    const std::string CT = ctype_of(fz.axis0_dtype);
+    for (int r = 0; r < RPT; ++r)
+      os << "  if (valid" << r << ") {\n"
+         << "    if (colpart) {\n#pragma unroll\n      for (int j = 0; j < VX; ++j) colpart[((nca_i64)tzi * NY + gy" << r
+         << ") * NX + gx + j] = (double)cs[" << r << "][j]" << (comp ? " + (double)cc[" + I(r) + "][j]" : "") << ";\n"
+         << "    } else {\n#pragma unroll\n      for (int j = 0; j < VX; ++j) nca_st(reinterpret_cast<" << CT
+         << "*>(colout) + gy" << r << " * NX + gx + j, (double)cs[" << r << "][j]"
+         << (comp ? " + (double)cc[" + I(r) + "][j]" : "") << ");\n"
+         << "    }\n  }\n";
+  }
+  if (fz.fuse) {
+    os << "  __shared__ double red_sd[4];\n  __shared__ " << RC << " red_mx[4];\n  __shared__ " << RC << " red_mn[4];\n"
+       << "  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;\n";
+    if (fz.fuse & NCA_FUSE_SUM) {
+      if (comp) os << "  nca_wave_comp(acc_s, acc_c);\n  if (lane == 0) red_sd[wid] = (double)acc_s + (double)acc_c;\n";
+      else os << "  acc_s = nca_wave_sum(acc_s);\n  if (lane == 0) red_sd[wid] = (double)acc_s;\n";
+    }
+    if (fz.fuse & NCA_FUSE_MAX) os << "  acc_mx = nca_wave_max(acc_mx);\n  if (lane == 0) red_mx[wid] = acc_mx;\n";
+    if (fz.fuse & NCA_FUSE_MIN) os << "  acc_mn = nca_wave_min(acc_mn);\n  if (lane == 0) red_mn[wid] = acc_mn;\n";
+    os << "  __syncthreads();\n  if (threadIdx.x == 0) {\n";
+    if (fz.fuse & NCA_FUSE_SUM) os << "    ((double*)psum)[b] = red_sd[0] + red_sd[1] + red_sd[2] + red_sd[3];\n";
+    if (fz.fuse & NCA_FUSE_MAX)
+      os << "    " << RC << " m = red_mx[0];\n    for (int w = 1; w < 4; ++w) m = nca_maxop(m, red_mx[w]);\n    *(" << RC
+         << "*)((char*)pmax + 8 * (size_t)b) = m;\n";
+    if (fz.fuse & NCA_FUSE_MIN)
+      os << "    " << RC << " m2 = red_mn[0];\n    for (int w = 1; w < 4; ++w) m2 = nca_minop(m2, red_mn[w]);\n    *(" << RC
+         << "*)((char*)pmin + 8 * (size_t)b) = m2;\n";
+    os << "  }\n";
+  }
+  os << "}\n";
+  return os.str();
+}
+
 // ---- direct kernel ----------------------------------------------------------------------------------------
 static std::string gen_direct(const StencilProgram& p, int32_t dst_dtype, bool zero_boundary, int fuse, int mode) {
   const Tape& t = p.tape;
@@ -525,8 +832,10 @@ static void stencil_apply_impl(uint64_t handle, const nca_view* src, const nca_v
   }
   if (n == 0) return;
   if (march) {
+    const MarchPlan mp = march_plan(p);
     const int64_t NZ = dst->shape[0], NY = dst->shape[1], NX = dst->shape[2];
-    const int ntx = static_cast<int>((NX + 16 * vx - 1) / (16 * vx)), nty = static_cast<int>((NY + 15) / 16);
+    const int tyw = 16 * mp.rpt;
+    const int ntx = static_cast<int>((NX + 16 * vx - 1) / (16 * vx)), nty = static_cast<int>((NY + tyw - 1) / tyw);
     const int64_t tiles = static_cast<int64_t>(ntx) * nty;
     StencilFuse fz;
     fz.axis0 = axis0 != nullptr;
@@ -537,7 +846,8 @@ static void stencil_apply_impl(uint64_t handle, const nca_view* src, const nca_v
     std::string key = "m|" + std::to_string(fz.axis0) + std::to_string(fz.axis0_dtype) + "|" + std::to_string(fuse) + "|" +
                       std::to_string(mode) + "|" + std::to_string(with_dst);
     const std::string kname = march_name(p, fz, dst->dtype);
-    hipFunction_t fn = stencil_kernel(p, key, [&]() { return gen_march(p, fz, dst->dtype); }, kname);
+    hipFunction_t fn = stencil_kernel(
+        p, key, [&]() { return mp.queue ? gen_march_q(p, fz, dst->dtype, mp) : gen_march(p, fz, dst->dtype); }, kname);
     // One resident wave of blocks: (tiles x z runs) = CUs x blocks per CU for this kernel's
     // registers and LDS, so no block waits for another to finish (no tail), and runs are
     // as long as possible (fewer halo planes).
@@ -787,16 +1097,15 @@ int nca_hip_stencil_source(uint64_t handle, int32_t variant, char* buf, size_t n
   std::string src;
   int rc = api_call([&] {
     StencilProgram& p = stencil_program(handle);
-    if (variant == 0) {
+    if (variant == 0 || variant == 1) {
       NCA_REQUIRE(p.ndim == 3, "the march kernel is 3-D");
       StencilFuse fz;
-      src = gen_march(p, fz, p.tape.dest_dtype);
-    } else if (variant == 1) {
-      StencilFuse fz;
-      fz.axis0 = true;
-      fz.fuse = NCA_FUSE_MAX;
-      NCA_REQUIRE(p.ndim == 3, "the march kernel is 3-D");
-      src = gen_march(p, fz, p.tape.dest_dtype);
+      if (variant == 1) {
+        fz.axis0 = true;
+        fz.fuse = NCA_FUSE_MAX;
+      }
+      const MarchPlan mp = march_plan(p);
+      src = mp.queue ? gen_march_q(p, fz, p.tape.dest_dtype, mp) : gen_march(p, fz, p.tape.dest_dtype);
     } else {
       src = gen_direct(p, p.tape.dest_dtype, variant == 3, 0, 0);
     }

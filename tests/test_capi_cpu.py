@@ -117,11 +117,35 @@ def test_stencil_kernels_compile(dt):
         _compiles(adj.source(variant))
 
 
-def test_stencil_with_extra_array_compiles():
-    st = capi.Stencil(tapes.LAPLACIAN_OFFSETS, tapes.laplacian_then_c0(), capi.FLOAT32, capi.FLOAT32, capi.FLOAT32,
-                      [capi.FLOAT32, capi.FLOAT32], extra_dtypes=[capi.FLOAT32])
-    _compiles(st.source(1))
-    _compiles(st.source(2))
+@pytest.mark.parametrize("dt", [capi.FLOAT32, capi.BFLOAT16])
+def test_stencil_with_extra_array_compiles(dt):
+    st = capi.Stencil(tapes.LAPLACIAN_OFFSETS, tapes.laplacian_then_c0(), dt, capi.FLOAT32, capi.FLOAT32,
+                      [capi.FLOAT32, capi.FLOAT32], extra_dtypes=[dt])
+    for variant in (0, 1, 2):
+        _compiles(st.source(variant))
+
+
+def _sum_tape(n):
+    t = [P(capi.OP_LOAD_NCARR, 0)]
+    for k in range(1, n):
+        t += [P(capi.OP_LOAD_NCARR, k), P(capi.OP_ADD)]
+    return t
+
+
+def test_march_generator_choice_and_radius():
+    # pure z offsets (dz only with dy = dx = 0): register z-queue march; radius 2 in z and y,
+    # 3 in x, plus an xy diagonal within the plane
+    offs = [[-2, 0, 0], [2, 0, 0], [-1, 0, 0], [0, -2, 0], [0, 2, 1], [0, 0, -3], [0, 0, 3], [0, -1, -1], [0, 0, 0]]
+    st = capi.Stencil(offs, _sum_tape(len(offs)), capi.FLOAT32, capi.FLOAT32, capi.FLOAT32)
+    for variant in (0, 1):
+        src = st.source(variant)
+        assert "cq[" in src and "lds[2][LH][LW]" in src
+        _compiles(src)
+    # a z offset combined with an x offset needs other planes' neighbours: LDS ring march
+    diag = capi.Stencil([[1, 0, 1], [0, 0, 0]], _sum_tape(2), capi.FLOAT32, capi.FLOAT32, capi.FLOAT32)
+    src = diag.source(1)
+    assert "lds[NS][LH][LW]" in src
+    _compiles(src)
 
 
 def test_nonlinear_stencil_has_no_adjoint():

@@ -209,3 +209,48 @@ def test_laplacian_bf16_storage(dev, shape):
     g64 = got.astype(np.float64)
     err = np.abs(col.cpu().numpy() - g64.sum(0)) / np.maximum(np.abs(g64).sum(0), 1e-30)
     assert err.max() <= 2e-7
+
+
+def _sum_tape(capi, n):
+    P = capi.pack
+    t = [P(capi.OP_LOAD_NCARR, 0)]
+    for k in range(1, n):
+        t += [P(capi.OP_LOAD_NCARR, k), P(capi.OP_ADD)]
+    return t
+
+
+# pure-z offsets of radius up to 3 (register z-queue march) and a z+x diagonal (LDS ring march)
+PURE_Z = [[-2, 0, 0], [2, 0, 0], [-1, 0, 0], [0, -2, 0], [0, 2, 1], [0, 0, -3], [0, 0, 3], [0, -1, -1], [0, 0, 0]]
+DIAG = [[1, 0, 1], [-1, 1, 0], [0, 0, -1], [0, 0, 0]]
+
+
+@pytest.mark.parametrize("offs", [PURE_Z, DIAG], ids=["pure_z", "diagonal"])
+@pytest.mark.parametrize("src_code", ["float32", "bfloat16"])
+@pytest.mark.parametrize("shape", [(3, 37, 96), (11, 50, 128), (40, 17, 64)])
+def test_march_general_offsets(dev, offs, src_code, shape):
+    """Both march generators against the oracle on ragged tiles (NY not a multiple of the tile
+    height, z runs shorter than the unrolled queue), bit-exact: the float32 sums run in the same
+    order; bf16 sources are compared with the oracle on the bf16-rounded values."""
+    capi, tapes, ti = _c()
+    code = getattr(capi, src_code.upper())
+    rng = np.random.default_rng(len(offs) + shape[0])
+    x = rng.standard_normal(shape).astype(np.float32)
+    xd = torch.from_numpy(x).to(dev)
+    if src_code == "bfloat16":
+        xd = xd.to(torch.bfloat16)
+        x = xd.float().cpu().numpy()
+    out = torch.empty(shape, dtype=torch.float32, device=dev)
+    st = capi.Stencil(offs, _sum_tape(capi, len(offs)), code, capi.FLOAT32, capi.FLOAT32)
+    st.apply(ti.view(xd), ti.view(out), capi.BOUNDARY_ZERO)
+    ref = O.stencil_f32(x, shape, offs, _sum_tape(capi, len(offs)), [], True)
+    assert np.array_equal(out.cpu().numpy().view(np.uint32), ref.view(np.uint32))
+    # fused reductions on the same kernel: axis-0 column sums and the max
+    col = torch.empty(shape[1:], dtype=torch.float64, device=dev)
+    mx = torch.empty((), dtype=torch.float32, device=dev)
+    out2 = torch.empty_like(out)
+    st.apply_reduce(ti.view(xd), ti.view(out2), capi.BOUNDARY_ZERO, axis0=ti.view(col), fuse=capi.FUSE_MAX,
+                    mode=capi.ACCUM_COMPENSATED, max_out=mx.data_ptr())
+    torch.cuda.synchronize()
+    assert torch.equal(out2, out)
+    assert float(mx) == float(ref.max())
+    assert np.allclose(col.cpu().numpy(), ref.astype(np.float64).sum(axis=0), rtol=1e-12, atol=1e-9)
