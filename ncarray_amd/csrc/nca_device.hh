@@ -17,12 +17,10 @@ struct nca_f16 { unsigned short bits; };
 __device__ __forceinline__ float nca_bf16_to_f32(unsigned short b) {
   return __uint_as_float(static_cast<unsigned>(b) << 16);
 }
-// Round to nearest, ties to even; NaN stays a (quiet) NaN.
+// Round to nearest, ties to even; NaN stays a (quiet) NaN.  gfx950 converts in hardware
+// (v_cvt_pk_bf16_f32, two values per instruction).
 __device__ __forceinline__ unsigned short nca_f32_to_bf16(float f) {
-  unsigned u = __float_as_uint(f);
-  if ((u & 0x7fffffffu) > 0x7f800000u) return static_cast<unsigned short>((u >> 16) | 0x40u);
-  u += 0x7fffu + ((u >> 16) & 1u);
-  return static_cast<unsigned short>(u >> 16);
+  return __builtin_bit_cast(unsigned short, static_cast<__bf16>(f));
 }
 __device__ __forceinline__ float nca_f16_to_f32(unsigned short b) {
   _Float16 h;
@@ -188,6 +186,12 @@ __device__ __forceinline__ void nca_store_nt(nca_vec<T, N>* p, const nca_vec<T, 
     nca_u32x4 w;
     __builtin_memcpy(&w, &v, 16);
     __builtin_nontemporal_store(w, reinterpret_cast<nca_u32x4*>(p));
+  } else if constexpr (B == 32) {
+    nca_u32x4 w0, w1;
+    __builtin_memcpy(&w0, &v, 16);
+    __builtin_memcpy(&w1, reinterpret_cast<const char*>(&v) + 16, 16);
+    __builtin_nontemporal_store(w0, reinterpret_cast<nca_u32x4*>(p));
+    __builtin_nontemporal_store(w1, reinterpret_cast<nca_u32x4*>(p) + 1);
   } else if constexpr (B == 8) {
     nca_u32x2 w;
     __builtin_memcpy(&w, &v, 8);

@@ -15,6 +15,7 @@
 //  * direct (any rank, any step, pro/epilogue hooks): one output per thread, one
 //    predicated load per offset, the reference's launch shape (ceil(n/256) x 256).
 #include <cstdio>
+#include <cstdlib>
 #include <map>
 #include <set>
 #include <sstream>
@@ -368,6 +369,18 @@ struct MarchPlan {
   int rpt = 1;
 };
 
+// NCA_HIP_MARCH_TILE="rows,vx" overrides the tile shape of the register-queue march (rows per
+// thread 1 or 2; x elements per thread 4 or 8), for tuning on a given device.
+static bool march_override(int& rpt, int& vx) {
+  const char* e = std::getenv("NCA_HIP_MARCH_TILE");
+  if (!e) return false;
+  int a = 0, b = 0;
+  if (std::sscanf(e, "%d,%d", &a, &b) != 2 || (a != 1 && a != 2) || (b != 4 && b != 8)) return false;
+  rpt = a;
+  vx = b;
+  return true;
+}
+
 static MarchPlan march_plan(const StencilProgram& p) {
   MarchPlan m;
   m.vx = march_vx(p.src_dtype);
@@ -379,9 +392,19 @@ static MarchPlan march_plan(const StencilProgram& p) {
   auto halo_threads = [&](int rpt) { return 2 * g.ry * 16 + 2 * g.rx * (16 * rpt + 2 * g.ry); };
   m.queue = pure_z && g.rx <= 4;
   m.rpt = (m.queue && itemsize(p.src_dtype) >= 4) ? 2 : 1;
+  if (m.queue) {
+    int rpt = m.rpt, vx = m.vx;
+    if (march_override(rpt, vx) && (vx * itemsize(p.src_dtype) == 16 || vx * itemsize(p.src_dtype) == 32)) {
+      m.rpt = rpt;
+      m.vx = vx;
+    }
+  }
   if (halo_threads(m.rpt) > 256) m.rpt = 1;
   if (halo_threads(m.rpt) > 256) m.queue = false;
-  if (!m.queue) m.rpt = 1;
+  if (!m.queue) {
+    m.rpt = 1;
+    m.vx = march_vx(p.src_dtype);
+  }
   return m;
 }
 
@@ -469,24 +492,42 @@ static std::string gen_march_q(const StencilProgram& p, const StencilFuse& fz, i
   if (fz.fuse & NCA_FUSE_MIN) os << "  " << RC << " acc_mn = nca_lim<" << RC << ">::max();\n";
   const std::string zero = "(" + C + ")0";
   // plane pz (an int expression) into register slot q; the plane starts at element offset off
+  // one row vector of VX source values into dst[0..VX) (compute type), zero when !ok.  2-byte
+  // storage types are read as four 32-bit words (zeroed per word) and unpacked by shifts.
+  const bool two_byte = itemsize(p.src_dtype) == 2;
+  auto load_row = [&](const std::string& ok, const std::string& addr, const std::string& dst) {
+    if (two_byte) {
+      const bool bf = p.src_dtype == NCA_BFLOAT16;
+      os << "        nca_u32x4 w = *reinterpret_cast<const nca_u32x4*>(" << ok << " ? " << addr << " : src);\n"
+         << "        w = " << ok << " ? w : (nca_u32x4)(0u);\n";
+      for (int i = 0; i < VX / 2; ++i) {
+        if (bf) {
+          os << "        " << dst << "[" << 2 * i << "] = (" << C << ")__uint_as_float(w[" << i << "] << 16);\n"
+             << "        " << dst << "[" << 2 * i + 1 << "] = (" << C << ")__uint_as_float(w[" << i << "] & 0xffff0000u);\n";
+        } else {
+          os << "        " << dst << "[" << 2 * i << "] = (" << C << ")nca_f16_to_f32((unsigned short)(w[" << i << "] & 0xffffu));\n"
+             << "        " << dst << "[" << 2 * i + 1 << "] = (" << C << ")nca_f16_to_f32((unsigned short)(w[" << i << "] >> 16));\n";
+        }
+      }
+    } else {
+      os << "        const nca_vec<" << S << ", VX> v = *reinterpret_cast<const nca_vec<" << S << ", VX>*>(" << ok << " ? " << addr
+         << " : src);\n";
+      for (int j = 0; j < VX; ++j)
+        os << "        " << dst << "[" << j << "] = " << ok << " ? nca_cvt<" << C << ", " << S << ">::from(v.v[" << j << "]) : "
+           << zero << ";\n";
+    }
+  };
   auto gload = [&](int q, const std::string& pz, const std::string& off) {
     os << "    {\n      const bool zin = (unsigned)(" << pz << ") < (unsigned)NZi;\n"
        << "      const " << S << "* __restrict__ pl = src + (zin ? (" << off << ") : (size_t)0);\n";
     for (int r = 0; r < RPT; ++r) {
-      os << "      { const bool ok = zin && valid" << r << ";\n"
-         << "        const nca_vec<" << S << ", VX> v = *reinterpret_cast<const nca_vec<" << S << ", VX>*>(ok ? pl + ioff" << r
-         << " : src);\n";
-      for (int j = 0; j < VX; ++j)
-        os << "        cq[" << q << "][" << r << "][" << j << "] = ok ? nca_cvt<" << C << ", " << S << ">::from(v.v[" << j
-           << "]) : " << zero << ";\n";
+      os << "      { const bool ok = zin && valid" << r << ";\n";
+      load_row("ok", "pl + ioff" + I(r), "cq[" + I(q) + "][" + I(r) + "]");
       os << "      }\n";
     }
     if (nhr > 0) {
-      os << "      if (hid < " << nhr << ") { const bool ok = zin && hrok;\n"
-         << "        const nca_vec<" << S << ", VX> v = *reinterpret_cast<const nca_vec<" << S << ", VX>*>(ok ? pl + hroff : src);\n";
-      for (int j = 0; j < VX; ++j)
-        os << "        hq[" << q << "][" << j << "] = ok ? nca_cvt<" << C << ", " << S << ">::from(v.v[" << j << "]) : " << zero
-           << ";\n";
+      os << "      if (hid < " << nhr << ") { const bool ok = zin && hrok;\n";
+      load_row("ok", "pl + hroff", "hq[" + I(q) + "]");
       os << "      }\n";
     }
     if (nhc > 0)
@@ -816,7 +857,8 @@ static void stencil_apply_impl(uint64_t handle, const nca_view* src, const nca_v
                dst->shape[1] * dst->shape[2] < (int64_t(1) << 31) &&
                same_shape(*src, *dst) && (!with_dst || is_c_contiguous(*dst)) &&
                (itemsize(src->dtype) == 4 || itemsize(src->dtype) == 2);
-  const int vx = march_vx(src->dtype);
+  const MarchPlan mplan = march_plan(p);
+  const int vx = mplan.vx;
   march = march && dst->shape[2] % vx == 0 &&
           reinterpret_cast<uintptr_t>(src->data) % (vx * itemsize(src->dtype)) == 0 &&
           (!with_dst || reinterpret_cast<uintptr_t>(dst->data) % (vx * itemsize(dst->dtype)) == 0);
@@ -832,7 +874,7 @@ static void stencil_apply_impl(uint64_t handle, const nca_view* src, const nca_v
   }
   if (n == 0) return;
   if (march) {
-    const MarchPlan mp = march_plan(p);
+    const MarchPlan& mp = mplan;
     const int64_t NZ = dst->shape[0], NY = dst->shape[1], NX = dst->shape[2];
     const int tyw = 16 * mp.rpt;
     const int ntx = static_cast<int>((NX + 16 * vx - 1) / (16 * vx)), nty = static_cast<int>((NY + tyw - 1) / tyw);

@@ -232,3 +232,28 @@ def test_vjp_c0(dev):
     one = torch.ones(1, dtype=torch.float32, device=dev)
     tape.vjp(ins, grads, [1, 1, 1, 1, 1], grad_scalar=one.data_ptr(), scalars=(0.5,))
     assert np.max(np.abs(gb.cpu().numpy() - (a64 - 0.5))) < 1e-6
+
+
+def test_bf16_store_rounding_matches_torch(dev):
+    """float32 -> bfloat16 stores round to nearest even: ties both ways, carries into the
+    exponent, subnormals, signed zeros, infinities and overflow, compared bit for bit with
+    torch's conversion (NaN only has to stay NaN)."""
+    import torch
+    from ncarray_amd import capi
+    from ncarray_amd import torch_interop as ti
+    bits = [0x3F808000, 0x3F818000, 0x3F80FFFF, 0x3F807FFF, 0x3FFF8000, 0x7F7FFFFF, 0x7F7F8000, 0x00008000,
+            0x00018000, 0x00000001, 0x807FFFFF, 0x80000000, 0x00000000, 0x7F800000, 0xFF800000, 0x477FE000]
+    rng = np.random.default_rng(12)
+    rnd = rng.integers(0, 2 ** 32, 4096, dtype=np.uint64).astype(np.uint32)
+    rnd = rnd[(rnd & 0x7F800000) != 0x7F800000]  # finite values
+    x = np.concatenate([np.array(bits, dtype=np.uint32), rnd]).view(np.float32)
+    xd = torch.from_numpy(x.copy()).cuda()
+    out = torch.empty(x.shape, dtype=torch.bfloat16, device="cuda")
+    tape = capi.Tape([capi.pack(capi.OP_LOAD_NCARR, 0)], [capi.FLOAT32], [], capi.FLOAT32, capi.BFLOAT16)
+    tape.launch([ti.view(xd)], ti.view(out))
+    want = torch.from_numpy(x.copy()).to(torch.bfloat16)
+    assert torch.equal(out.cpu().view(torch.int16), want.view(torch.int16))
+    nan = torch.tensor([float("nan")], dtype=torch.float32, device="cuda")
+    on = torch.empty(1, dtype=torch.bfloat16, device="cuda")
+    tape.launch([ti.view(nan)], ti.view(on))
+    assert torch.isnan(on.float()).all()
