@@ -62,29 +62,44 @@ __global__ __launch_bounds__(512) void k_bin(const T* __restrict__ frames, const
       atomicAdd(&hc[fb * nb + bin], n);
     }
   };
-  // vector body: 4 pixels per thread per step (aligned: host guarantees p0 % 4 == 0)
+  // vector body: 4 pixels per thread per vector (aligned: host guarantees p0 % 4 == 0); the
+  // loads of U vectors are issued before any of them is folded into the histogram, so each
+  // thread keeps U * FB frame loads in flight across the LDS atomics
+  constexpr int U = (FB == 1) ? 4 : 2;
   const nca_i64 nvec = (p1 - p0) / 4;
-  for (nca_i64 v = threadIdx.x; v < nvec; v += blockDim.x) {
-    const nca_i64 p = p0 + 4 * v;
-    const nca_vec<I, 4> m = *reinterpret_cast<const nca_vec<I, 4>*>(map + p);
-    int bins[4];
+  for (nca_i64 v0 = threadIdx.x; v0 < nvec; v0 += (nca_i64)U * blockDim.x) {
+    nca_vec<I, 4> m[U];
+    nca_vec<T, 4> x[U][FB];
 #pragma unroll
-    for (int j = 0; j < 4; ++j) bins[j] = (m.v[j] < 0 || m.v[j] >= nb) ? -1 : static_cast<int>(m.v[j]);
+    for (int u = 0; u < U; ++u) {
+      const nca_i64 v = v0 + (nca_i64)u * blockDim.x;
+      const nca_i64 p = p0 + 4 * (v < nvec ? v : 0);
+      m[u] = *reinterpret_cast<const nca_vec<I, 4>*>(map + p);
 #pragma unroll
-    for (int fb = 0; fb < FB; ++fb) {
-      if (fb >= nf) break;
-      const nca_vec<T, 4> x = *reinterpret_cast<const nca_vec<T, 4>*>(frames + (f0 + fb) * P + p);
-      int cur = bins[0];
-      A s = 0;
-      unsigned n = 0;
+      for (int fb = 0; fb < FB; ++fb)
+        x[u][fb] = *reinterpret_cast<const nca_vec<T, 4>*>(frames + (f0 + (fb < nf ? fb : 0)) * P + p);
+    }
 #pragma unroll
-      for (int j = 0; j < 4; ++j) {
-        const V val = nca_cvt<V, T>::from(x.v[j]);
-        if (bins[j] != cur) { flush(fb, cur, s, n); cur = bins[j]; s = 0; n = 0; }
-        s += static_cast<A>(val);
-        n += NONZERO ? (val != V(0) ? 1u : 0u) : 1u;
+    for (int u = 0; u < U; ++u) {
+      if (v0 + (nca_i64)u * blockDim.x >= nvec) break;
+      int bins[4];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) bins[j] = (m[u].v[j] < 0 || m[u].v[j] >= nb) ? -1 : static_cast<int>(m[u].v[j]);
+#pragma unroll
+      for (int fb = 0; fb < FB; ++fb) {
+        if (fb >= nf) break;
+        int cur = bins[0];
+        A s = 0;
+        unsigned n = 0;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          const V val = nca_cvt<V, T>::from(x[u][fb].v[j]);
+          if (bins[j] != cur) { flush(fb, cur, s, n); cur = bins[j]; s = 0; n = 0; }
+          s += static_cast<A>(val);
+          n += NONZERO ? (val != V(0) ? 1u : 0u) : 1u;
+        }
+        flush(fb, cur, s, n);
       }
-      flush(fb, cur, s, n);
     }
   }
   for (nca_i64 p = p0 + 4 * nvec + threadIdx.x; p < p1; p += blockDim.x) {
