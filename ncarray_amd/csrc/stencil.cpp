@@ -367,21 +367,29 @@ struct MarchPlan {
   bool queue = false;
   int vx = 4;
   int rpt = 1;
+  int txt = 16;  // threads across x (256 / txt thread rows)
 };
 
-// NCA_HIP_MARCH_TILE="rows,vx" overrides the tile shape of the register-queue march (rows per
-// thread 1 or 2; x elements per thread 4 or 8), for tuning on a given device.
-static bool march_override(int& rpt, int& vx) {
+// NCA_HIP_MARCH_TILE="rows,vx[,tx]" overrides the tile shape of the register-queue march (rows
+// per thread 1 or 2; x elements per thread 4 or 8; threads across x 16, 32, 64 or 128), for tuning.
+static bool march_override(int& rpt, int& vx, int& txt) {
   const char* e = std::getenv("NCA_HIP_MARCH_TILE");
   if (!e) return false;
-  int a = 0, b = 0;
-  if (std::sscanf(e, "%d,%d", &a, &b) != 2 || (a != 1 && a != 2) || (b != 4 && b != 8)) return false;
+  int a = 0, b = 0, c = txt;
+  const int n = std::sscanf(e, "%d,%d,%d", &a, &b, &c);
+  if (n < 2 || (a != 1 && a != 2) || (b != 4 && b != 8) || (c != 16 && c != 32 && c != 64 && c != 128)) return false;
   rpt = a;
   vx = b;
+  txt = c;
   return true;
 }
 
-static MarchPlan march_plan(const StencilProgram& p) {
+// Tile shape of the register-queue march: 256 elements across and 8 rows (64 threads x 4
+// elements x 2 rows for 4-byte sources, 32 x 8 x 1 for 2-byte ones).  Measured on MI355X for the
+// 512^3 Laplacian: wide tiles keep the halo-column reads (a 128-byte line per row for one
+// element) small against the interior (HBM reads 1.04x the array instead of 1.24x with 64x32
+// tiles); narrower arrays get narrower tiles so that threads are not idle.
+static MarchPlan march_plan(const StencilProgram& p, int64_t nx = int64_t(1) << 40) {
   MarchPlan m;
   m.vx = march_vx(p.src_dtype);
   if (p.ndim != 3) return m;
@@ -389,14 +397,23 @@ static MarchPlan march_plan(const StencilProgram& p) {
   bool pure_z = true;
   for (auto& o : p.offsets)
     if (o[0] != 0 && (o[1] != 0 || o[2] != 0)) pure_z = false;
-  auto halo_threads = [&](int rpt) { return 2 * g.ry * 16 + 2 * g.rx * (16 * rpt + 2 * g.ry); };
+  auto halo_threads = [&](int rpt) {
+    const int tyt = 256 / m.txt;
+    return 2 * g.ry * m.txt + 2 * g.rx * (tyt * rpt + 2 * g.ry);
+  };
   m.queue = pure_z && g.rx <= 4;
   m.rpt = (m.queue && itemsize(p.src_dtype) >= 4) ? 2 : 1;
   if (m.queue) {
-    int rpt = m.rpt, vx = m.vx;
-    if (march_override(rpt, vx) && (vx * itemsize(p.src_dtype) == 16 || vx * itemsize(p.src_dtype) == 32)) {
+    m.txt = 256 / m.vx;
+    while (m.txt > 16 && static_cast<int64_t>(m.txt / 2) * m.vx >= nx) m.txt /= 2;
+    while (m.txt > 16 && halo_threads(m.rpt) > 256) m.txt /= 2;
+  }
+  if (m.queue) {
+    int rpt = m.rpt, vx = m.vx, txt = m.txt;
+    if (march_override(rpt, vx, txt) && (vx * itemsize(p.src_dtype) == 16 || vx * itemsize(p.src_dtype) == 32)) {
       m.rpt = rpt;
       m.vx = vx;
+      m.txt = txt;
     }
   }
   if (halo_threads(m.rpt) > 256) m.rpt = 1;
@@ -404,6 +421,7 @@ static MarchPlan march_plan(const StencilProgram& p) {
   if (!m.queue) {
     m.rpt = 1;
     m.vx = march_vx(p.src_dtype);
+    m.txt = 16;
   }
   return m;
 }
@@ -425,7 +443,8 @@ static std::string gen_march_q(const StencilProgram& p, const StencilFuse& fz, i
   const bool comp = fz.mode == NCA_ACCUM_COMPENSATED;
   const int VX = mp.vx, RPT = mp.rpt, RZ = g.rz, RY = g.ry, RX = g.rx;
   const int Q = 2 * RZ + 2;
-  const int TX = 16 * VX, TY = 16 * RPT, LH = TY + 2 * RY, PADL = 4, LW = PADL + TX + 4;
+  const int TXT = mp.txt, TYT = 256 / mp.txt;
+  const int TX = TXT * VX, TY = TYT * RPT, LH = TY + 2 * RY, PADL = 4, LW = PADL + TX + 4;
   auto I = [](long long v) { return std::to_string(v); };
   std::ostringstream os;
   os << "#define VX " << VX << "\n#define TX " << TX << "\n#define TY " << TY << "\n#define LH " << LH << "\n#define LW " << LW
@@ -446,20 +465,20 @@ static std::string gen_march_q(const StencilProgram& p, const StencilFuse& fz, i
      << "  const int txi = tix % ntx, tyi = (tix / ntx) % nty, tzi = tix / (ntx * nty);\n"
      << "  const nca_i64 x0 = (nca_i64)txi * TX, y0 = (nca_i64)tyi * TY, z0 = (nca_i64)tzi * zlen;\n"
      << "  const nca_i64 z1 = (z0 + zlen < NZ) ? z0 + zlen : NZ;\n"
-     << "  const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;\n"
+     << "  const int tx = threadIdx.x % " << TXT << ", ty = threadIdx.x / " << TXT << ";\n"
      << "  const nca_i64 gx = x0 + VX * tx;\n"
      << "  const size_t plane = (size_t)NY * (size_t)NX;\n  const int NZi = (int)NZ;\n"
      << "  const int hid = threadIdx.x;\n";
   for (int r = 0; r < RPT; ++r)
-    os << "  const nca_i64 gy" << r << " = y0 + ty + " << 16 * r << ";\n"
+    os << "  const nca_i64 gy" << r << " = y0 + ty + " << TYT * r << ";\n"
        << "  const bool valid" << r << " = (gx < NX) && (gy" << r << " < NY);\n"
        << "  const unsigned ioff" << r << " = valid" << r << " ? (unsigned)(gy" << r << " * NX + gx) : 0u;\n";
-  const int nhr = 2 * RY * 16, nhc = 2 * RX * LH;
+  const int nhr = 2 * RY * TXT, nhc = 2 * RX * LH;
   // halo roles: halo-row vectors (threads [0, nhr)) and halo-column scalars ([nhr, nhr + nhc))
   os << "  bool hrok = false;\n  unsigned hroff = 0u;\n  int hrrow = 0, hrcol = 0;\n";
   if (nhr > 0)
     os << "  if (hid < " << nhr << ") {\n"
-       << "    const int hr = hid >> 4, hx = hid & 15;\n"
+       << "    const int hr = hid / " << TXT << ", hx = hid % " << TXT << ";\n"
        << "    const nca_i64 yy = y0 + ((hr < RY) ? (hr - RY) : (TY + hr - RY));\n"
        << "    const nca_i64 xx = x0 + VX * hx;\n"
        << "    hrok = (yy >= 0) && (yy < NY) && (xx < NX);\n"
@@ -540,7 +559,7 @@ static std::string gen_march_q(const StencilProgram& p, const StencilFuse& fz, i
     for (int r = 0; r < RPT; ++r) {
       os << "    { lvec w;\n";
       for (int j = 0; j < VX; ++j) os << "      w.v[" << j << "] = cq[" << q << "][" << r << "][" << j << "];\n";
-      os << "      *reinterpret_cast<lvec*>(&lds[" << buf << "][RY + ty + " << 16 * r << "][PADL + VX * tx]) = w; }\n";
+      os << "      *reinterpret_cast<lvec*>(&lds[" << buf << "][RY + ty + " << TYT * r << "][PADL + VX * tx]) = w; }\n";
     }
     if (nhr > 0) {
       os << "    if (hid < " << nhr << ") { lvec w;\n";
@@ -567,14 +586,14 @@ static std::string gen_march_q(const StencilProgram& p, const StencilFuse& fz, i
     for (int r = 0; r < RPT; ++r) {
       os << "    {\n";
       for (int64_t dy : vec_rows)
-        os << "      const lvec V" << nm(dy) << " = *reinterpret_cast<const lvec*>(&lds[" << buf << "][RY + ty + " << 16 * r
+        os << "      const lvec V" << nm(dy) << " = *reinterpret_cast<const lvec*>(&lds[" << buf << "][RY + ty + " << TYT * r
            << " + (" << dy << ")][PADL + VX * tx]);\n";
       for (auto& kv : reach) {
         for (int m = 1; m <= kv.second.first; ++m)
-          os << "      const " << C << " L" << nm(kv.first) << "_" << m << " = lds[" << buf << "][RY + ty + " << 16 * r << " + ("
+          os << "      const " << C << " L" << nm(kv.first) << "_" << m << " = lds[" << buf << "][RY + ty + " << TYT * r << " + ("
              << kv.first << ")][PADL + VX * tx - " << m << "];\n";
         for (int m = 1; m <= kv.second.second; ++m)
-          os << "      const " << C << " R" << nm(kv.first) << "_" << m << " = lds[" << buf << "][RY + ty + " << 16 * r << " + ("
+          os << "      const " << C << " R" << nm(kv.first) << "_" << m << " = lds[" << buf << "][RY + ty + " << TYT * r << " + ("
              << kv.first << ")][PADL + VX * tx + VX - 1 + " << m << "];\n";
       }
       for (size_t jx = 0; jx < p.extra_dtypes.size(); ++jx) {
@@ -857,7 +876,7 @@ static void stencil_apply_impl(uint64_t handle, const nca_view* src, const nca_v
                dst->shape[1] * dst->shape[2] < (int64_t(1) << 31) &&
                same_shape(*src, *dst) && (!with_dst || is_c_contiguous(*dst)) &&
                (itemsize(src->dtype) == 4 || itemsize(src->dtype) == 2);
-  const MarchPlan mplan = march_plan(p);
+  const MarchPlan mplan = march_plan(p, p.ndim == 3 ? dst->shape[2] : 0);
   const int vx = mplan.vx;
   march = march && dst->shape[2] % vx == 0 &&
           reinterpret_cast<uintptr_t>(src->data) % (vx * itemsize(src->dtype)) == 0 &&
@@ -876,8 +895,8 @@ static void stencil_apply_impl(uint64_t handle, const nca_view* src, const nca_v
   if (march) {
     const MarchPlan& mp = mplan;
     const int64_t NZ = dst->shape[0], NY = dst->shape[1], NX = dst->shape[2];
-    const int tyw = 16 * mp.rpt;
-    const int ntx = static_cast<int>((NX + 16 * vx - 1) / (16 * vx)), nty = static_cast<int>((NY + tyw - 1) / tyw);
+    const int txw = mp.txt * vx, tyw = (256 / mp.txt) * mp.rpt;
+    const int ntx = static_cast<int>((NX + txw - 1) / txw), nty = static_cast<int>((NY + tyw - 1) / tyw);
     const int64_t tiles = static_cast<int64_t>(ntx) * nty;
     StencilFuse fz;
     fz.axis0 = axis0 != nullptr;
@@ -886,7 +905,8 @@ static void stencil_apply_impl(uint64_t handle, const nca_view* src, const nca_v
     fz.with_dst = with_dst;
     fz.axis0_dtype = axis0 ? axis0->dtype : NCA_FLOAT32;
     std::string key = "m|" + std::to_string(fz.axis0) + std::to_string(fz.axis0_dtype) + "|" + std::to_string(fuse) + "|" +
-                      std::to_string(mode) + "|" + std::to_string(with_dst);
+                      std::to_string(mode) + "|" + std::to_string(with_dst) + "|" + std::to_string(mp.queue) + "," +
+                      std::to_string(mp.vx) + "," + std::to_string(mp.rpt) + "," + std::to_string(mp.txt);
     const std::string kname = march_name(p, fz, dst->dtype);
     hipFunction_t fn = stencil_kernel(
         p, key, [&]() { return mp.queue ? gen_march_q(p, fz, dst->dtype, mp) : gen_march(p, fz, dst->dtype); }, kname);
