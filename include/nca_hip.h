@@ -96,9 +96,16 @@ enum { NCA_COUNT_ALL = 0, NCA_COUNT_NONZERO = 1 };
 
 /* A strided device array view.  `data` addresses element [0,...,0]; per-axis byte
  * offsets of the reference layout (layout.hh:468-476) are folded into it.  Strides are
- * in bytes (layout.hh m_strides).  pointer_axis >= 0 marks an NC pointer axis
- * (layout.hh:469-471): data is then a device table of row pointers along that axis and
- * ptr_offset is the reference's m_offsets[pointer_axis].  ndim == 0 is a scalar. */
+ * in bytes (layout.hh m_strides).  ndim == 0 is a scalar.
+ * pointer_axis >= 0 marks an NC pointer axis P (NCOffsetsPolicy::advance, layout.hh:
+ * 468-480): element (i_0, ..., i_{n-1}) is at
+ *     row + row_offset + sum_{d > P} i_d * strides[d],
+ *     row = ((char**)(data + sum_{d < P} i_d * strides[d]))[i_P * strides[P] + ptr_offset],
+ * i.e. data is a device table of row pointers, strides[P] counts table entries (1 for the
+ * reference's layouts, whose advance ignores it; 0 broadcasts one row; k > 1 is a stepped
+ * slice), ptr_offset is the reference's m_offsets[P] and row_offset the sum of its byte
+ * offsets of the axes after P.  Only the expression entry points (expr_launch,
+ * expr_launch_reduce) accept pointer-axis views. */
 typedef struct nca_view {
   void* data;
   int32_t dtype;
@@ -108,6 +115,7 @@ typedef struct nca_view {
   int32_t pointer_axis;
   int32_t reserved;
   int64_t ptr_offset;
+  int64_t row_offset;
 } nca_view;
 
 /* A scalar constant.  The expression's dtype tables say which member is meant. */

@@ -18,7 +18,7 @@ before the native modules exist; there is no fallback when they are missing.
 __version__ = "0.1.0"
 
 _ARRAY_NAMES = {"DType", "OpCode", "Boundary", "Accum", "NCDevArray", "DevExpr", "Stencil", "NcaError", "NCArray",
-                "NCArrayRef", "set_eager", "is_eager", "materialize", "lazy_mode", "exp", "log", "sqrt", "abs",
+                "NCArrayRef", "NCDevArrayRef", "set_eager", "is_eager", "materialize", "lazy_mode", "exp", "log", "sqrt", "abs",
                 "init", "from_torch", "numpy_dtype"}
 
 

@@ -64,6 +64,37 @@ class NCArrayRef(NCArray):
         return view
 
 
+def _interface_dtype(obj):
+    """The DType of a framework tensor whose __cuda_array_interface__ typestr is ambiguous
+    (torch exports bfloat16 as "|V2")."""
+    if str(getattr(obj, "dtype", "")) == "torch.bfloat16":
+        return DType.bfloat16
+    return None
+
+
+def _device_row(obj):
+    if isinstance(obj, NCDevArray):
+        return obj
+    if hasattr(obj, "__cuda_array_interface__"):
+        return NCDevArray.from_cuda_array(obj, _interface_dtype(obj))
+    raise TypeError(f"NCDevArrayRef rows must be device arrays (NCDevArray or __cuda_array_interface__), "
+                    f"not {type(obj).__name__}")
+
+
+def NCDevArrayRef(data, read_only=False):  # noqa: N802  (the reference's class name)
+    """A device array over separately allocated rows, with a pointer axis in front.
+
+    Mirrors the reference's NCDevArrayRef constructors (pyncarray/ncdevarray_ref.cu:39-52,
+    utilities.hh:226-314): a list of equally shaped device arrays becomes one array of shape
+    (len(list), *row_shape) whose axis 0 is a pointer axis; a single array gets a pointer axis
+    of length 1.  The rows are referenced, not copied, and kept alive by the result.
+    """
+    if read_only:
+        raise ValueError("read-only device references are not supported")
+    rows = list(data) if isinstance(data, (list, tuple)) else [data]
+    return NCDevArray.from_rows([_device_row(r) for r in rows])
+
+
 def _dtype_of(np_dtype):
     np_dtype = np.dtype(np_dtype)
     for k, v in _NUMPY.items():
@@ -89,11 +120,11 @@ def abs(x):  # noqa: A001  (mirrors numpy's name)
 
 def from_torch(tensor):
     """Zero-copy NCDevArray view of a device torch tensor (kept alive by the view)."""
-    return NCDevArray.from_cuda_array(tensor)
+    return NCDevArray.from_cuda_array(tensor, _interface_dtype(tensor))
 
 
 _ncarray._set_host_array_type(NCArray)
 
 __all__ = ["DType", "OpCode", "Boundary", "Accum", "NCDevArray", "DevExpr", "Stencil", "NcaError", "NCArray",
-           "NCArrayRef", "set_eager", "is_eager", "materialize", "lazy_mode", "exp", "log", "sqrt", "abs", "init",
+           "NCArrayRef", "NCDevArrayRef", "set_eager", "is_eager", "materialize", "lazy_mode", "exp", "log", "sqrt", "abs", "init",
            "from_torch", "numpy_dtype"]

@@ -49,7 +49,7 @@ class View(ctypes.Structure):
     _fields_ = [("data", ctypes.c_void_p), ("dtype", ctypes.c_int32), ("ndim", ctypes.c_int32),
                 ("shape", ctypes.c_int64 * MAX_NDIM), ("strides", ctypes.c_int64 * MAX_NDIM),
                 ("pointer_axis", ctypes.c_int32), ("reserved", ctypes.c_int32),
-                ("ptr_offset", ctypes.c_int64)]
+                ("ptr_offset", ctypes.c_int64), ("row_offset", ctypes.c_int64)]
 
 
 class Scalar(ctypes.Union):
@@ -187,6 +187,17 @@ def make_view(ptr, dtype, shape, strides_bytes=None):
         v.strides[d] = int(strides_bytes[d])
     v.pointer_axis = -1
     v.ptr_offset = 0
+    v.row_offset = 0
+    return v
+
+
+def make_nc_view(table_ptr, dtype, shape, strides, pointer_axis, ptr_offset=0, row_offset=0):
+    """A View with an NC pointer axis: `table_ptr` is a device table of row pointers;
+    strides are bytes except strides[pointer_axis], which counts table entries."""
+    v = make_view(table_ptr, dtype, shape, strides)
+    v.pointer_axis = pointer_axis
+    v.ptr_offset = ptr_offset
+    v.row_offset = row_offset
     return v
 
 

@@ -85,6 +85,8 @@ ScalarSlots make_scalar_slots(const Tape& t, const nca_scalar* s, int n) {
 // ---- operand analysis ----------------------------------------------------------------------
 static bool views_alias(const nca_view& a, const nca_view& b) {
   if (a.data != b.data || a.dtype != b.dtype || a.ndim != b.ndim) return false;
+  if (a.pointer_axis != b.pointer_axis) return false;
+  if (a.pointer_axis >= 0 && (a.ptr_offset != b.ptr_offset || a.row_offset != b.row_offset)) return false;
   for (int d = 0; d < a.ndim; ++d)
     if (a.shape[d] != b.shape[d] || (a.shape[d] != 1 && a.strides[d] != b.strides[d])) return false;
   return true;
@@ -95,19 +97,20 @@ Operands analyse_operands(const Tape& t, const nca_view* ins, int n_in, const nc
   NCA_REQUIRE(n_in == static_cast<int>(t.arr_dtypes.size()),
               "expected " + std::to_string(t.arr_dtypes.size()) + " array operands, got " + std::to_string(n_in));
   NCA_REQUIRE(n_in == 0 || ins != nullptr, "null operand views");
-  validate_view(out, "out", out_may_be_null);
+  validate_view_nc(out, "out", out_may_be_null);
   Operands ops;
   ops.n = view_size(out);
   ops.alias.resize(n_in);
   ops.bcast.assign(n_in, 0);
+  ops.pax.assign(n_in, -1);
   for (int k = 0; k < n_in; ++k) {
     const nca_view& v = ins[k];
-    validate_view(v, "operand");
+    validate_view_nc(v, "operand");
     NCA_REQUIRE(v.dtype == t.arr_dtypes[k], "operand " + std::to_string(k) + " dtype " +
                                                 dtype_info(v.dtype).name + " does not match the tape's " +
                                                 dtype_info(t.arr_dtypes[k]).name);
-    if (v.ndim == 0 || (is_broadcast_scalar(v) && view_size(v) <= 1) ||
-        (is_broadcast_scalar(v) && v.ndim == out.ndim && same_shape(v, out) && ops.n > 1)) {
+    if (v.pointer_axis < 0 && (v.ndim == 0 || (is_broadcast_scalar(v) && view_size(v) <= 1) ||
+                               (is_broadcast_scalar(v) && v.ndim == out.ndim && same_shape(v, out) && ops.n > 1))) {
       ops.bcast[k] = 1;
     } else {
       NCA_REQUIRE(same_shape(v, out), "operand " + std::to_string(k) + " shape does not match the output shape");
@@ -139,6 +142,32 @@ Operands analyse_operands(const Tape& t, const nca_view* ins, int n_in, const nc
   for (int k : ops.unique)
     if (!ops.bcast[k] && !aligned(ins[k].data, itemsize(ins[k].dtype))) vec = 1;
   ops.vec = contig ? vec : 1;
+  // NC pointer-axis views: strided kind over the output's own axes (no coalescing across the
+  // pointer axis)
+  bool nc = out.data && out.pointer_axis >= 0;
+  for (int k : ops.unique)
+    if (!ops.bcast[k] && ins[k].pointer_axis >= 0) nc = true;
+  if (nc) {
+    ops.nc = true;
+    ops.contiguous = false;
+    ops.vec = 1;
+    const int nd = std::max(out.ndim, 1);
+    ops.sndim = nd;
+    ops.sshape.assign(nd, 1);
+    ops.sout.assign(nd, 0);
+    for (int d = 0; d < out.ndim; ++d) {
+      ops.sshape[d] = out.shape[d];
+      ops.sout[d] = out.data ? out.strides[d] : 0;
+    }
+    ops.paxo = out.data ? out.pointer_axis : -1;
+    ops.sstrides.assign(n_in, std::vector<int64_t>(nd, 0));
+    for (int k : ops.unique) {
+      if (ops.bcast[k]) continue;
+      for (int d = 0; d < ins[k].ndim; ++d) ops.sstrides[k][d] = ins[k].strides[d];
+      ops.pax[k] = ins[k].pointer_axis;
+    }
+    return ops;
+  }
   // strided: jointly coalesce output + non-broadcast unique operands
   if (!contig) {
     int nd = out.ndim;
@@ -189,6 +218,11 @@ Operands analyse_operands(const Tape& t, const nca_view* ins, int n_in, const nc
 static std::string sig_operands(const Tape& t, const Operands& ops) {
   std::ostringstream os;
   for (size_t k = 0; k < ops.alias.size(); ++k) os << ops.alias[k] << (ops.bcast[k] ? "b" : "") << ",";
+  if (ops.nc) {
+    os << "nc";
+    for (int k : ops.unique) os << ops.pax[k] << ",";
+    os << "o" << ops.paxo;
+  }
   return os.str();
 }
 
@@ -214,6 +248,10 @@ static std::string param_list(const Tape& t, const Operands& ops, bool strided, 
     os << "nca_dims shp";
     for (int k : ops.unique) if (!ops.bcast[k]) os << ", nca_dims st" << k;
     if (with_out) os << ", nca_dims sto";
+    // NC views: row-table index offset and row byte offset
+    for (int k : ops.unique)
+      if (!ops.bcast[k] && ops.nc && ops.pax[k] >= 0) os << ", nca_i64 po" << k << ", nca_i64 ro" << k;
+    if (with_out && ops.nc && ops.paxo >= 0) os << ", nca_i64 poo, nca_i64 roo";
   }
   sep();
   os << "nca_i64 n";
@@ -248,18 +286,55 @@ static std::string strided_prelude(int nd) {
   return os.str();
 }
 
-// Body that computes offsets of element `idx` for the strided variant.
+// Body that computes offsets of element `idx` for the strided variant.  For an NC view with
+// pointer axis P (NCOffsetsPolicy::advance, layout.hh:468-480), o accumulates the axes before
+// P (bytes on the row-pointer table), i the table entry along P (index times strides[P], which
+// counts entries) and r the axes after P (bytes on the row).
 static std::string strided_offsets(const Tape& t, const Operands& ops, bool with_out) {
   std::ostringstream os;
+  auto ncp = [&](int k) { return ops.nc && ops.pax[k] >= 0; };
+  const bool nco = with_out && ops.nc && ops.paxo >= 0;
   os << "      nca_i64 rem = idx;\n";
-  for (int k : ops.unique) if (!ops.bcast[k]) os << "      nca_i64 o" << k << " = 0;\n";
-  if (with_out) os << "      nca_i64 oo = 0;\n";
+  for (int k : ops.unique)
+    if (!ops.bcast[k]) {
+      os << "      nca_i64 o" << k << " = 0;\n";
+      if (ncp(k)) os << "      nca_i64 i" << k << " = 0, r" << k << " = 0;\n";
+    }
+  if (with_out) {
+    os << "      nca_i64 oo = 0;\n";
+    if (nco) os << "      nca_i64 io = 0, rpo = 0;\n";
+  }
   os << "#pragma unroll\n      for (int d = NCA_ND - 1; d >= 0; --d) {\n"
      << "        const nca_i64 c = rem % shp.v[d];\n        rem /= shp.v[d];\n";
-  for (int k : ops.unique) if (!ops.bcast[k]) os << "        o" << k << " += c * st" << k << ".v[d];\n";
-  if (with_out) os << "        oo += c * sto.v[d];\n";
+  for (int k : ops.unique) {
+    if (ops.bcast[k]) continue;
+    if (ncp(k))
+      os << "        if (d < " << ops.pax[k] << ") o" << k << " += c * st" << k << ".v[d];\n"
+         << "        else if (d == " << ops.pax[k] << ") i" << k << " = c * st" << k << ".v[d];\n"
+         << "        else r" << k << " += c * st" << k << ".v[d];\n";
+    else
+      os << "        o" << k << " += c * st" << k << ".v[d];\n";
+  }
+  if (with_out) {
+    if (nco)
+      os << "        if (d < " << ops.paxo << ") oo += c * sto.v[d];\n"
+         << "        else if (d == " << ops.paxo << ") io = c * sto.v[d];\n"
+         << "        else rpo += c * sto.v[d];\n";
+    else
+      os << "        oo += c * sto.v[d];\n";
+  }
   os << "      }\n";
   return os.str();
+}
+
+// Address expression of operand k (strided kind): plain views add the offset, NC views read
+// the row pointer first.
+static std::string operand_address(const Operands& ops, int k) {
+  if (ops.nc && ops.pax[k] >= 0)
+    return "(*reinterpret_cast<const char* const*>(p" + std::to_string(k) + " + o" + std::to_string(k) + " + (i" +
+           std::to_string(k) + " + po" + std::to_string(k) + ") * (nca_i64)sizeof(void*)) + ro" + std::to_string(k) +
+           " + r" + std::to_string(k) + ")";
+  return "(p" + std::to_string(k) + " + o" + std::to_string(k) + ")";
 }
 
 struct ReduceSpec {
@@ -379,11 +454,15 @@ std::string gen_elementwise(const Tape& t, const Operands& ops, bool strided, in
     for (int k : ops.unique)
       if (!ops.bcast[k])
         os << "      const " << C << " u" << k << " = nca_ld<" << C << ">(reinterpret_cast<const "
-           << ctype_of(t.arr_dtypes[k]) << "*>(p" << k << " + o" << k << "));\n";
+           << ctype_of(t.arr_dtypes[k]) << "*>" << operand_address(ops, k) << ");\n";
     if (t.uses_idx) os << "      const nca_i64 lin = idx;\n";
     os << body;
     os << "      const " << D << " st = nca_store_cvt<" << D << ", " << C << ">::to(" << roots[0] << ");\n";
-    os << "      if (outp) *reinterpret_cast<" << D << "*>(outp + oo) = st;\n";
+    if (ops.nc && ops.paxo >= 0)
+      os << "      if (outp) *reinterpret_cast<" << D << "*>(*reinterpret_cast<char* const*>(outp + oo + (io + poo) * "
+         << "(nca_i64)sizeof(void*)) + roo + rpo) = st;\n";
+    else
+      os << "      if (outp) *reinterpret_cast<" << D << "*>(outp + oo) = st;\n";
     os << rc.update;
     os << "  }\n";
   }
@@ -403,6 +482,7 @@ int grid_blocks(int64_t work_items, int threads, int max_blocks) {
 struct LaunchArgs {
   std::deque<const void*> ptrs;             // storage for pointer params (stable addresses)
   std::deque<std::array<int64_t, NCA_MAX_NDIM>> dims;
+  std::deque<int64_t> scal;                 // NC row-table and row offsets
   int64_t n = 0;
   std::vector<void*> params;
 };
@@ -435,6 +515,21 @@ static void push_common(LaunchArgs& la, const Tape& t, const Operands& ops, cons
       for (int d = 0; d < ops.sndim; ++d) s[d] = ops.sout[d];
       la.dims.push_back(s);
       la.params.push_back(la.dims.back().data());
+    }
+    if (ops.nc) {
+      for (int k : ops.unique)
+        if (!ops.bcast[k] && ops.pax[k] >= 0) {
+          la.scal.push_back(ins[k].ptr_offset);
+          la.params.push_back(&la.scal.back());
+          la.scal.push_back(ins[k].row_offset);
+          la.params.push_back(&la.scal.back());
+        }
+      if (with_out && ops.paxo >= 0) {
+        la.scal.push_back(out.ptr_offset);
+        la.params.push_back(&la.scal.back());
+        la.scal.push_back(out.row_offset);
+        la.params.push_back(&la.scal.back());
+      }
     }
   }
   la.n = ops.n;

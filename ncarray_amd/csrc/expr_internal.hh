@@ -39,6 +39,11 @@ struct Operands {
   std::vector<int64_t> sshape;
   std::vector<int64_t> sout;
   std::vector<std::vector<int64_t>> sstrides;
+  // NC pointer-axis views (strided kind only, axes not coalesced): the pointer axis of each
+  // operand and of the output, -1 for plain strided views
+  bool nc = false;
+  std::vector<int> pax;
+  int paxo = -1;
 };
 Operands analyse_operands(const Tape& t, const nca_view* ins, int n_in, const nca_view& out,
                           bool out_may_be_null);

@@ -104,6 +104,8 @@ struct ViewDesc {
 };
 int64_t view_size(const nca_view& v);
 void validate_view(const nca_view& v, const char* what, bool allow_null_data = false);
+// As validate_view, but also accepts NC pointer-axis views (expression operands and outputs).
+void validate_view_nc(const nca_view& v, const char* what, bool allow_null_data = false);
 bool is_c_contiguous(const nca_view& v);
 bool same_shape(const nca_view& a, const nca_view& b);
 // True when every stride is zero (a broadcast scalar).

@@ -1,6 +1,7 @@
 // libnca_hip runtime: initialisation, errors, device memory, scratch workspaces and view
 // helpers.  Replaces the CUDA context / DevOwnerPolicy / CircularDevicePool plumbing of
 // the reference (jit_utils.cc:28-46, storage.hh:176-186, 242-252, mem_pool.cuh:47-123).
+#include <algorithm>
 #include <cstdio>
 #include <cstdlib>
 
@@ -150,6 +151,21 @@ int64_t view_size(const nca_view& v) {
   return n;
 }
 
+void validate_view_nc(const nca_view& v, const char* what, bool allow_null_data) {
+  if (v.pointer_axis < 0) {
+    validate_view(v, what, allow_null_data);
+    return;
+  }
+  nca_view plain = v;
+  plain.pointer_axis = -1;
+  validate_view(plain, what, allow_null_data);
+  NCA_REQUIRE(v.pointer_axis < v.ndim, std::string(what) + ": pointer axis out of range");
+  NCA_REQUIRE(v.ptr_offset >= 0, std::string(what) + ": negative pointer offset");
+  const int64_t n = v.shape[v.pointer_axis];
+  NCA_REQUIRE(n == 0 || v.ptr_offset + std::min<int64_t>(0, (n - 1) * v.strides[v.pointer_axis]) >= 0,
+              std::string(what) + ": pointer-axis table index below zero");
+}
+
 void validate_view(const nca_view& v, const char* what, bool allow_null_data) {
   NCA_REQUIRE(v.ndim >= 0 && v.ndim <= NCA_MAX_NDIM, std::string(what) + ": ndim out of range");
   if (!is_supported(v.dtype))
@@ -160,12 +176,15 @@ void validate_view(const nca_view& v, const char* what, bool allow_null_data) {
   for (int d = 0; d < v.ndim; ++d)
     NCA_REQUIRE(v.shape[d] >= 0, std::string(what) + ": negative extent");
   if (v.pointer_axis >= 0)
-    NCA_THROW(NCA_ERR_UNSUPPORTED, std::string(what) + ": NC pointer-axis layouts are not supported by the device kernels");
+    NCA_THROW(NCA_ERR_UNSUPPORTED, std::string(what) +
+                                       ": this entry point does not take NC pointer-axis views (the expression "
+                                       "entry points do)");
   if (!allow_null_data && view_size(v) > 0)
     NCA_REQUIRE(v.data != nullptr, std::string(what) + ": null data pointer");
 }
 
 bool is_c_contiguous(const nca_view& v) {
+  if (v.pointer_axis >= 0) return false;
   int64_t expect = itemsize(v.dtype);
   for (int d = v.ndim - 1; d >= 0; --d) {
     if (v.shape[d] == 1) continue;

@@ -158,6 +158,9 @@ nca_view broadcast_view(const nca_view& v, const std::vector<int64_t>& shape) {
   nca_view r = v;
   r.ndim = n;
   const int lead = n - v.ndim;
+  // a pointer axis keeps its place counted from the right; broadcasting it (extent 1, table
+  // stride 0) repeats its one row
+  if (v.pointer_axis >= 0) r.pointer_axis = v.pointer_axis + lead;
   for (int d = n - 1; d >= 0; --d) {
     if (d < lead) {
       r.shape[d] = shape[d];
@@ -489,6 +492,64 @@ NCDevArray NCDevArray::wrap(void* data, const Shape& shape, const Shape& strides
   return a;
 }
 
+NCDevArray NCDevArray::from_row_pointers(const std::vector<void*>& rows, const Shape& row_shape,
+                                         const Shape& row_strides, DType dtype, std::shared_ptr<void> keep) {
+  ensure_init();
+  if (row_shape.size() != row_strides.size() || row_shape.size() + 1 > NCA_MAX_NDIM)
+    throw Error(NCA_ERR_INVALID, "bad row shape/strides");
+  if (!nca::is_supported(static_cast<int32_t>(dtype)))
+    throw Error(NCA_ERR_UNSUPPORTED, std::string("dtype ") + dtype_name(dtype) + " has no device kernels");
+  int64_t row_n = 1;
+  for (int64_t e : row_shape) row_n *= e;
+  for (void* r : rows)
+    if (r == nullptr && row_n > 0) throw Error(NCA_ERR_INVALID, "null row pointer");
+  NCDevArray a;
+  a.dt_ = dtype;
+  a.nd_ = static_cast<int>(row_shape.size()) + 1;
+  a.sh_[0] = static_cast<int64_t>(rows.size());
+  a.st_[0] = 1;
+  for (size_t d = 0; d < row_shape.size(); ++d) {
+    if (row_shape[d] < 0) throw Error(NCA_ERR_INVALID, "negative extent");
+    a.sh_[d + 1] = row_shape[d];
+    a.st_[d + 1] = row_strides[d];
+  }
+  a.storage_ = std::make_shared<Storage>(rows.size() * sizeof(void*));
+  a.storage_->hold(std::move(keep));
+  if (!rows.empty())
+    check(nca_hip_memcpy_h2d(a.storage_->ptr(), rows.data(), rows.size() * sizeof(void*), nullptr), "row table copy");
+  a.data_ = a.storage_->ptr();
+  a.pax_ = 0;
+  a.rows_ = std::make_shared<const std::vector<void*>>(rows);
+  return a;
+}
+
+NCDevArray NCDevArray::from_rows(const std::vector<NCDevArray>& rows) {
+  if (rows.empty()) throw Error(NCA_ERR_INVALID, "from_rows needs at least one row");
+  const NCDevArray& r0 = rows[0];
+  auto keep = std::make_shared<std::vector<std::shared_ptr<Storage>>>();
+  std::vector<void*> ptrs;
+  for (const NCDevArray& r : rows) {
+    if (r.pax_ >= 0) throw Error(NCA_ERR_INVALID, "rows of a pointer-axis array cannot have a pointer axis");
+    if (r.dt_ != r0.dt_ || r.shape_vec() != r0.shape_vec()) throw Error(NCA_ERR_INVALID, "rows differ in shape or dtype");
+    if (r.strides_vec() != r0.strides_vec()) throw Error(NCA_ERR_INVALID, "rows differ in stride layout");
+    ptrs.push_back(r.data_);
+    keep->push_back(r.storage_);
+  }
+  return from_row_pointers(ptrs, r0.shape_vec(), r0.strides_vec(), r0.dt_, keep);
+}
+
+void* NCDevArray::row(int64_t i) const {
+  if (pax_ < 0 || !rows_) throw Error(NCA_ERR_INVALID, "not a pointer-axis array");
+  if (i < 0 || i >= sh_[pax_]) throw Error(NCA_ERR_INVALID, "Index out of bounds!");
+  const int64_t base = (static_cast<const char*>(data_) - static_cast<const char*>(storage_->ptr())) /
+                       static_cast<int64_t>(sizeof(void*));
+  const int64_t entry = base + i * st_[pax_] + poff_;
+  if (entry < 0 || entry >= static_cast<int64_t>(rows_->size())) throw Error(NCA_ERR_INVALID, "row table index out of range");
+  return static_cast<char*>((*rows_)[static_cast<size_t>(entry)]) + roff_;
+}
+
+NCDevArray NCDevArray::dense() const { return pax_ >= 0 ? copy() : *this; }
+
 NCDevArray& NCDevArray::operator=(const Expr& e) {
   if (empty()) {
     *this = NCDevArray(e.final_shape, e.expr_dtype);
@@ -504,6 +565,7 @@ int64_t NCDevArray::size() const {
 }
 
 bool NCDevArray::is_contiguous() const {
+  if (pax_ >= 0) return false;
   int64_t s = itemsize();
   for (int d = nd_ - 1; d >= 0; --d) {
     if (sh_[d] != 1 && st_[d] != s) return false;
@@ -521,7 +583,9 @@ nca_view NCDevArray::cview() const {
     v.shape[d] = sh_[d];
     v.strides[d] = st_[d];
   }
-  v.pointer_axis = -1;
+  v.pointer_axis = pax_;
+  v.ptr_offset = pax_ >= 0 ? poff_ : 0;
+  v.row_offset = pax_ >= 0 ? roff_ : 0;
   return v;
 }
 
@@ -543,7 +607,13 @@ NCDevArray NCDevArray::view_from_indices(const std::vector<IndexItem>& items) co
   if (consumed > nd_) throw Error(NCA_ERR_INVALID, "too many indices for array");
   NCDevArray r = *this;
   r.nd_ = 0;
+  r.pax_ = -1;
   char* base = static_cast<char*>(data_);
+  // pointer axis (out_from_axes_ptr, array_impl.hh:964-1031): axes before it move the table
+  // position, an integer index on it dereferences the row, a slice moves the table offset;
+  // axes after it move the row offset while the pointer axis is kept
+  bool live_ptr = pax_ >= 0;
+  int64_t poff = poff_, roff = roff_;
   int src = 0;
   auto push = [&r](int64_t e, int64_t s) {
     if (r.nd_ >= NCA_MAX_NDIM) throw Error(NCA_ERR_INVALID, "too many axes");
@@ -551,17 +621,36 @@ NCDevArray NCDevArray::view_from_indices(const std::vector<IndexItem>& items) co
     r.st_[r.nd_] = s;
     ++r.nd_;
   };
+  auto keep_axis = [&](int a) {
+    if (a == pax_) r.pax_ = r.nd_;
+    push(sh_[a], st_[a]);
+  };
+  auto move = [&](int a, int64_t bytes) {
+    if (live_ptr && a > pax_) roff += bytes;
+    else base += bytes;
+  };
   for (const IndexItem& it : items) {
     if (std::holds_alternative<Ellipsis>(it)) {
       const int fill = nd_ - consumed;
-      for (int k = 0; k < fill; ++k, ++src) push(sh_[src], st_[src]);
+      for (int k = 0; k < fill; ++k, ++src) keep_axis(src);
     } else if (std::holds_alternative<NewAxis>(it)) {
       push(1, 0);
     } else if (const int64_t* pi = std::get_if<int64_t>(&it)) {
       int64_t i = *pi;
       if (i < -sh_[src] || i >= sh_[src]) throw Error(NCA_ERR_INVALID, "Index out of bounds!");
       if (i < 0) i += sh_[src];
-      base += i * st_[src];
+      if (live_ptr && src == pax_) {
+        NCDevArray probe = *this;
+        probe.data_ = base;
+        probe.poff_ = poff;
+        probe.roff_ = 0;
+        base = static_cast<char*>(probe.row(i));
+        base += roff;
+        roff = 0;
+        live_ptr = false;
+      } else {
+        move(src, i * st_[src]);
+      }
       ++src;
     } else {
       const Slice& s = std::get<Slice>(it);
@@ -581,13 +670,26 @@ NCDevArray NCDevArray::view_from_indices(const std::vector<IndexItem>& items) co
       int64_t len = 0;
       if (step > 0 && stop > start) len = (stop - start + step - 1) / step;
       if (step < 0 && start > stop) len = (start - stop - step - 1) / (-step);
-      if (len > 0) base += start * st_[src];
+      if (live_ptr && src == pax_) {
+        if (len > 0) poff += start * st_[src];
+        r.pax_ = r.nd_;
+      } else if (len > 0) {
+        move(src, start * st_[src]);
+      }
       push(len, st_[src] * step);
       ++src;
     }
   }
-  for (; src < nd_; ++src) push(sh_[src], st_[src]);
+  for (; src < nd_; ++src) keep_axis(src);
   r.data_ = base;
+  if (live_ptr) {
+    r.poff_ = poff;
+    r.roff_ = roff;
+  } else {
+    r.pax_ = -1;
+    r.poff_ = r.roff_ = 0;
+    r.rows_.reset();
+  }
   return r;
 }
 
@@ -612,6 +714,12 @@ NCDevArray NCDevArray::reshape(const Shape& shape) const {
 NCDevArray NCDevArray::copy_as_shape(const Shape& shape) const { return copy().reshape(shape); }
 
 NCDevArray NCDevArray::squeeze() const {
+  if (pax_ >= 0) {
+    // a length-1 pointer axis collapses like any other (array_impl.hh:1146-1172)
+    std::vector<IndexItem> items;
+    for (int d = 0; d < nd_; ++d) items.push_back(sh_[d] == 1 ? IndexItem(int64_t{0}) : IndexItem(Slice()));
+    return view_from_indices(items);
+  }
   NCDevArray r = *this;
   r.nd_ = 0;
   for (int d = 0; d < nd_; ++d)
@@ -689,14 +797,24 @@ Scalar NCDevArray::item() const {
   return scalar_from_bytes(dt_, buf);
 }
 
-Scalar NCDevArray::full(ReduceOp op, Accum mode) const { return HIPEngine::execute_full_reduce(*this, op, mode); }
+Scalar NCDevArray::full(ReduceOp op, Accum mode) const {
+  if (pax_ >= 0) {
+    // sum/max/min read the rows in place through the fused expression reduction; the
+    // others reduce a staged dense copy
+    if (op == ReduceOp::sum) return Expr(*this).sum(mode);
+    if (op == ReduceOp::max) return Expr(*this).max();
+    if (op == ReduceOp::min) return Expr(*this).min();
+    return HIPEngine::execute_full_reduce(dense(), op, mode);
+  }
+  return HIPEngine::execute_full_reduce(*this, op, mode);
+}
 
 NCDevArray NCDevArray::axes_reduce(ReduceOp op, const std::vector<int64_t>& axes, Accum mode) const {
   Shape kept;
   for (int d = 0; d < nd_; ++d)
     if (std::find(axes.begin(), axes.end(), static_cast<int64_t>(d)) == axes.end()) kept.push_back(sh_[d]);
   NCDevArray r(kept, reduce_result_dtype(op, dt_));
-  HIPEngine::execute_reduce_axes(*this, axes, op, r, mode);
+  HIPEngine::execute_reduce_axes(dense(), axes, op, r, mode);
   return r;
 }
 
@@ -838,8 +956,11 @@ void HIPEngine::execute_copy_into(const NCDevArray& src, void* host, DType host_
 
 void HIPEngine::execute_scatter_add(const NCDevArray& dest, const NCDevArray& idx, const NCDevArray& src, void* stream) {
   ensure_init();
-  const nca_view d = dest.cview(), i = idx.cview(), s = src.cview();
+  // pointer-axis operands go through staged dense copies
+  const NCDevArray dd = dest.dense(), id = idx.dense(), sd = src.dense();
+  const nca_view d = dd.cview(), i = id.cview(), s = sd.cview();
   check(nca_hip_scatter_add(&d, &i, &s, stream), "scatter_add");
+  if (dest.pointer_axis() >= 0) execute_assign(dest, dd, stream);
 }
 
 // ---- stencils ------------------------------------------------------------------------------
@@ -883,11 +1004,15 @@ void stencil_apply(uint64_t h, const std::vector<Scalar>& scalars, const NCDevAr
                    void* stream, Boundary b, const std::vector<void*>& ext_args) {
   ensure_init();
   std::vector<nca_scalar> sc = pack_scalars(scalars, nullptr);
-  const nca_view s = src.cview(), d = dest.cview();
+  // pointer-axis arrays go through staged dense copies
+  const NCDevArray sd = src.dense();
+  const NCDevArray dd = dest.pointer_axis() >= 0 ? NCDevArray(dest.shape_vec(), dest.dtype()) : dest;
+  const nca_view s = sd.cview(), d = dd.cview();
   check(nca_hip_stencil_apply(h, &s, nullptr, 0, sc.data(), static_cast<int32_t>(scalars.size()), &d,
                               static_cast<int32_t>(b), ext_args.empty() ? nullptr : ext_args.data(),
                               static_cast<int32_t>(ext_args.size()), stream),
         "stencil apply");
+  if (dest.pointer_axis() >= 0) HIPEngine::execute_assign(dest, dd, stream);
 }
 
 uint64_t stencil_adjoint(uint64_t h) {

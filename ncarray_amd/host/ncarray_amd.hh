@@ -137,6 +137,8 @@ class Storage {
   Storage(const Storage&) = delete;
   Storage& operator=(const Storage&) = delete;
   void* ptr() const { return ptr_; }
+  // Keep `k` alive as long as this storage (the rows behind a row-pointer table).
+  void hold(std::shared_ptr<void> k) { keep_ = std::move(k); }
 
  private:
   void* ptr_ = nullptr;
@@ -251,6 +253,15 @@ class NCDevArray {
   // A view of device memory owned elsewhere (strides in bytes).
   static NCDevArray wrap(void* data, const Shape& shape, const Shape& strides_bytes, DType dtype,
                          std::shared_ptr<void> keep = {});
+  // NCDevArrayRef over separately allocated rows (the list constructor, pylist_to_ref,
+  // utilities.hh:273-314): axis 0 is a pointer axis over `rows`, which must share one shape,
+  // stride layout and dtype.  The row-pointer table is copied to device memory; the rows are
+  // kept alive by the result.
+  static NCDevArray from_rows(const std::vector<NCDevArray>& rows);
+  // The same over raw device row pointers with one row layout (ArrayImpl's vector<void*>
+  // constructor, array_impl.hh:388-432, pointer axis 0); `keep` holds the rows alive.
+  static NCDevArray from_row_pointers(const std::vector<void*>& rows, const Shape& row_shape,
+                                      const Shape& row_strides_bytes, DType dtype, std::shared_ptr<void> keep = {});
 
   // ArrayImpl::operator=(Expr) (array_operations.hh:42-63): evaluate into this array's
   // storage and dtype.  An empty array is first allocated with the expression's shape/dtype.
@@ -267,7 +278,14 @@ class NCDevArray {
   int itemsize() const { return ncarray_amd::itemsize(dt_); }
   int64_t nbytes() const { return size() * itemsize(); }
   DType dtype() const { return dt_; }
-  bool is_contiguous() const;
+  bool is_contiguous() const;  // false for pointer-axis arrays (array_impl.hh:1090-1107)
+  int pointer_axis() const { return pax_; }
+  bool is_pointer_axis(int d) const { return d == pax_; }
+  // Row pointer at pointer-axis index i (of a pointer-axis array), offset by the row offset.
+  void* row(int64_t i) const;
+  // This array, or for a pointer-axis array a new C-order owner with its values (the staged
+  // copy that operations without pointer-axis kernels read).
+  NCDevArray dense() const;
   bool empty() const { return storage_ == nullptr; }
   void* data() const { return data_; }
   const std::shared_ptr<Storage>& storage() const { return storage_; }
@@ -355,6 +373,13 @@ class NCDevArray {
   int nd_ = 0;
   std::array<int64_t, NCA_MAX_NDIM> sh_{};
   std::array<int64_t, NCA_MAX_NDIM> st_{};
+  // NC pointer axis (NCOffsetsPolicy, layout.hh:451-480): data_ points into the device
+  // row-pointer table, st_[pax_] counts table entries, poff_ is the table offset (the
+  // reference's m_offsets[P]) and roff_ the byte offset added to each row pointer.
+  int pax_ = -1;
+  int64_t poff_ = 0;
+  int64_t roff_ = 0;
+  std::shared_ptr<const std::vector<void*>> rows_;  // host copy of the table
 };
 
 // ---- engine (GPUEngine, gpuengine.hh:64-638) ------------------------------------------------

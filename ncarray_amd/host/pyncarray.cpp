@@ -191,7 +191,12 @@ PyOperand resolve(const py::handle& o) {
   else if (py::isinstance<Expr>(o)) r.expr = o.cast<Expr>();
   else if (py::isinstance<py::array>(o)) r.arr = from_host(o.cast<py::array>(), std::nullopt);
   else if (is_scalar(o)) r.sc = to_scalar(o);
-  else if (py::hasattr(o, "__cuda_array_interface__")) r.arr = py::module_::import("ncarray_amd._ncarray").attr("NCDevArray").attr("from_cuda_array")(o).cast<NCDevArray>();
+  else if (py::hasattr(o, "__cuda_array_interface__")) {
+    // torch exports bfloat16 with the bare typestr "|V2": name the dtype for it
+    py::object as = py::none();
+    if (py::hasattr(o, "dtype") && std::string(py::str(o.attr("dtype"))) == "torch.bfloat16") as = py::cast(DType::bfloat16);
+    r.arr = py::module_::import("ncarray_amd._ncarray").attr("NCDevArray").attr("from_cuda_array")(o, as).cast<NCDevArray>();
+  }
   else throw py::type_error("unsupported operand type " + std::string(py::str(py::type::of(o))));
   return r;
 }
@@ -427,10 +432,21 @@ PYBIND11_MODULE(_ncarray, m) {
           py::arg("shape"), py::arg("dtype") = DType::float32)
       .def_static("from_host", [](const py::array& a, std::optional<DType> d) { return from_host(a, d); },
                   py::arg("array"), py::arg("dtype") = py::none(), "Copy a host (numpy) array to a new device array.")
-      .def_static("from_cuda_array", [](const py::object& obj) {
+      .def_static("from_cuda_array", [](const py::object& obj, std::optional<DType> as) {
         py::dict ai = obj.attr("__cuda_array_interface__");
         py::tuple shp = ai["shape"];
-        const DType d = from_typestr(ai["typestr"].cast<std::string>());
+        const std::string ts = ai["typestr"].cast<std::string>();
+        DType d;
+        if (as) {
+          // an explicit dtype names what a bare typestr cannot (e.g. "|V2" for bfloat16)
+          if (static_cast<int>(py::dtype(ts).itemsize()) != itemsize(*as))
+            throw py::type_error("typestr " + ts + " does not match the size of " + dtype_name(*as));
+          d = *as;
+        } else {
+          if (ts.size() >= 2 && ts[1] == 'V')
+            throw py::type_error("typestr " + ts + " does not name a dtype; pass dtype= (e.g. DType.bfloat16)");
+          d = from_typestr(ts);
+        }
         std::vector<int64_t> shape, strides;
         for (const py::handle& s : shp) shape.push_back(s.cast<int64_t>());
         if (ai.contains("strides") && !ai["strides"].is_none()) {
@@ -452,7 +468,8 @@ PYBIND11_MODULE(_ncarray, m) {
           delete static_cast<py::object*>(p);
         });
         return NCDevArray::wrap(ptr, shape, strides, d, keep);
-      }, py::arg("obj"), "Zero-copy view of an object exposing __cuda_array_interface__ (e.g. a torch tensor).")
+      }, py::arg("obj"), py::arg("dtype") = py::none(),
+                  "Zero-copy view of an object exposing __cuda_array_interface__ (e.g. a torch tensor).")
       .def_static("_placeholder", [](const std::vector<int64_t>& shape, DType d) {
         std::vector<int64_t> strides(shape.size(), 0);
         int64_t s = itemsize(d);
@@ -477,7 +494,14 @@ PYBIND11_MODULE(_ncarray, m) {
       .def_property_readonly("dtype", &NCDevArray::dtype)
       .def_property_readonly("is_contiguous", &NCDevArray::is_contiguous)
       .def_property_readonly("data_ptr", [](const NCDevArray& a) { return reinterpret_cast<uintptr_t>(a.data()); })
+      .def_static("from_rows", &NCDevArray::from_rows, py::arg("rows"),
+                  "A pointer-axis array (axis 0) over separately allocated device rows of one shape, stride "
+                  "layout and dtype (NCDevArrayRef of a list); the rows are kept alive.")
+      .def_property_readonly("pointer_axis", &NCDevArray::pointer_axis, "The NC pointer axis, or -1.")
+      .def("dense", &NCDevArray::dense, "This array, or a C-order copy of a pointer-axis array.")
       .def_property_readonly("__cuda_array_interface__", [](const NCDevArray& a) {
+        if (a.pointer_axis() >= 0)
+          throw Error(NCA_ERR_UNSUPPORTED, "a pointer-axis array has no __cuda_array_interface__ (use dense())");
         py::dict d;
         d["shape"] = to_tuple(a.shape(), a.ndim());
         d["typestr"] = typestr(a.dtype());

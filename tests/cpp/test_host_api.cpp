@@ -374,6 +374,40 @@ TEST(StencilTest, LaplacianZeroBoundary) {
   EXPECT_EQ(bad, 0);
 }
 
+// NCDevArrayRef over separately allocated rows (pylist_to_ref, utilities.hh:273-314): a
+// pointer axis in front, indexing through it, expressions and reductions over the rows.
+TEST(NCPointerAxis, RowsIndexingAndExpressions) {
+  const int R = 3, N = 5;
+  std::vector<nca::NCDevArray> rows;
+  for (int r = 0; r < R; ++r) {
+    nca::NCDevArray a(Shape{N});
+    a = a.iota() + static_cast<float>(10 * r);  // row r holds 10r, 10r+1, ...
+    rows.push_back(a);
+  }
+  nca::NCDevArray ref = nca::NCDevArray::from_rows(rows);
+  EXPECT_EQ(ref.ndim(), 2);
+  EXPECT_EQ(ref.pointer_axis(), 0);
+  EXPECT_TRUE(!ref.is_contiguous());
+  EXPECT_FLOAT_EQ(nca::scalar_to_double(ref(1, 3).item()), 13.0);
+  EXPECT_FLOAT_EQ(nca::scalar_to_double(ref(-1, -1).item()), 24.0);
+  nca::NCDevArray tail = ref(nca::Slice(1, nca::Slice::kNone), nca::Slice(1, 4));  // rows 1-2, cols 1-3
+  EXPECT_EQ(tail.pointer_axis(), 0);
+  std::vector<float> t = host_f32(tail);
+  EXPECT_TRUE(t == (std::vector<float>{11, 12, 13, 21, 22, 23}));
+  EXPECT_FLOAT_EQ(nca::scalar_to_double(ref.sum()), 3 * 10.0 + 30 * 5.0);
+  EXPECT_FLOAT_EQ(nca::scalar_to_double(ref.max()), 24.0);
+  EXPECT_EQ(nca::scalar_to_double(ref.argmax()), 14.0);
+  nca::NCDevArray s0 = ref.sum(std::vector<int64_t>{0});
+  std::vector<float> sv = host_f32(s0);
+  EXPECT_TRUE(sv == (std::vector<float>{30, 33, 36, 39, 42}));
+  ref(1) *= 2.0f;  // writes through row 1
+  std::vector<float> r1 = host_f32(rows[1]);
+  EXPECT_TRUE(r1 == (std::vector<float>{20, 22, 24, 26, 28}));
+  nca::NCDevArray twice(ref * 2.0f);
+  EXPECT_TRUE(twice.is_contiguous() && twice.pointer_axis() == -1);
+  EXPECT_FLOAT_EQ(nca::scalar_to_double(twice(2, 4).item()), 48.0);
+}
+
 int main(int argc, char** argv) {
   const char* filter = argc > 1 ? argv[1] : nullptr;
   int n_cases = 0, n_failed_cases = 0;
