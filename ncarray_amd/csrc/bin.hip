@@ -2,14 +2,14 @@
 // host/elementwise.hh:482-505) -- dest[idx[i]] += src[i] for 0 <= idx[i] < dest.size,
 // other indices skipped -- rebuilt as per-frame LDS-privatised histograms.
 //
-// nca_hip_bin: grid (chunks, frames).  A block zeroes an LDS histogram (uint32 count +
-// float64 or float32 sum per bin), streams its pixel chunk with 16-byte loads of pixels
-// and bins, folds runs of equal consecutive bins in registers (neighbouring pixels of a
-// detector bin map mostly share a bin) and adds each run to LDS with one atomic pair.
-// The block then writes its histogram to a partial slot; a combine kernel adds the
-// chunk partials of each (frame, bin) in chunk order, so counts are exact and the sums
-// do not depend on block scheduling (float64 in LDS makes the in-block order immaterial
-// at float32 output precision).
+// nca_hip_bin: grid (chunks, frame groups).  A block zeroes LDS tables (float64 sum and
+// zero-pixel count per bin and frame), streams its pixel chunk with 16-byte loads of pixels
+// (non-temporal) and bins, and adds every pixel to its bin's sum with one LDS atomic.  The
+// valid pixels of each bin are counted from the map alone, once per call, spread over the
+// blocks of the first frame groups.  Blocks write their tables to partial slots; a combine
+// kernel adds the chunk partials of each (frame, bin) in chunk order, so counts are exact and
+// the sums do not depend on block scheduling (float64 in LDS makes the in-block order
+// immaterial at float32 output precision).
 //
 // nca_hip_bin_backward: grad_frames[f, p] = grad_sums[f, map[p]] (0 out of range): the
 // gradient row of each frame is staged in LDS and gathered.
@@ -24,121 +24,131 @@
 
 namespace nca {
 
-template <typename T> struct BinT;
-template <> struct BinT<float> { static constexpr int32_t dt = NCA_FLOAT32; };
-template <> struct BinT<double> { static constexpr int32_t dt = NCA_FLOAT64; };
-template <> struct BinT<nca_bf16> { static constexpr int32_t dt = NCA_BFLOAT16; };
-template <> struct BinT<nca_f16> { static constexpr int32_t dt = NCA_FLOAT16; };
-template <> struct BinT<int> { static constexpr int32_t dt = NCA_INT32; };
-template <> struct BinT<unsigned short> { static constexpr int32_t dt = NCA_UINT16; };
-template <> struct BinT<unsigned int> { static constexpr int32_t dt = NCA_UINT32; };
+// Counting modes of the histogram kernel.
+enum { BIN_CNT_NONE = 0, BIN_CNT_ALL = 1, BIN_CNT_NONZERO = 2 };
 
-// value type folded into the histogram sums
-template <typename T> struct BinV { typedef float V; };
-template <> struct BinV<double> { typedef double V; };
-template <> struct BinV<int> { typedef double V; };
-template <> struct BinV<unsigned int> { typedef double V; };
-
-template <typename T, typename I, typename A, bool NONZERO, int FB>
-__global__ __launch_bounds__(512) void k_bin(const T* __restrict__ frames, const I* __restrict__ map, nca_i64 P, int nb,
-                                             nca_i64 chunk, nca_i64 F, unsigned* __restrict__ pcnt, A* __restrict__ psum) {
-  // one block: a chunk of pixels of FB consecutive frames; the map is read once for all FB
-  typedef typename BinV<T>::V V;
+// One block: a chunk of pixels of FB consecutive frames (the map is read once for all FB).
+// Every pixel adds its value to the LDS float64 sum of its bin with one atomic (out-of-range
+// bins go to a dummy slot nb, so the body has no branch on the bin).  Counting is split so
+// that no per-pixel count atomic is needed: pixels equal to zero are counted per frame (a
+// branch that is rarely taken), and the pixels of each valid bin are counted once per call
+// from the map alone -- the blocks of the first S frame groups each count one S-th of their
+// chunk.  The combine kernel forms count_all = valid and count_nonzero = valid - zeros.
+// A radial detector map changes bin about once per pixel along a row, so folding runs of
+// equal bins does not pay; the atomics run at LDS rate while the frame loads (non-temporal,
+// read once) stay in flight.
+template <typename T, typename I, int FB, bool SUMS, int CNT>
+__global__ __launch_bounds__(1024) void k_bin(const T* __restrict__ frames, const I* __restrict__ map, nca_i64 P,
+                                              int nb, nca_i64 chunk, nca_i64 F, int S, nca_i64 slice,
+                                              double* __restrict__ psum, unsigned* __restrict__ pzero,
+                                              unsigned* __restrict__ pvalid) {
   extern __shared__ unsigned char lds_raw[];
-  A* hs = reinterpret_cast<A*>(lds_raw);                                           // [FB][nb]
-  unsigned* hc = reinterpret_cast<unsigned*>(lds_raw + sizeof(A) * (size_t)FB * nb);  // [FB][nb]
-  for (int b = threadIdx.x; b < FB * nb; b += blockDim.x) { hs[b] = A(0); hc[b] = 0u; }
+  const int nb1 = nb + 1;
+  double* hs = reinterpret_cast<double*>(lds_raw);                                       // [FB][nb1] sums
+  unsigned* hz = reinterpret_cast<unsigned*>(lds_raw + (SUMS ? sizeof(double) * FB * nb1 : 0));  // [FB][nb1] zeros
+  unsigned* hv = hz + (CNT == BIN_CNT_NONZERO ? FB * nb1 : 0);                           // [nb1] valid pixels
+  const bool count_valid = CNT != BIN_CNT_NONE && static_cast<int>(blockIdx.y) < S;
+  for (int b = threadIdx.x; b < FB * nb1; b += blockDim.x) {
+    if (SUMS) hs[b] = 0.0;
+    if (CNT == BIN_CNT_NONZERO) hz[b] = 0u;
+  }
+  if (count_valid)
+    for (int b = threadIdx.x; b < nb1; b += blockDim.x) hv[b] = 0u;
   __syncthreads();
   const nca_i64 f0 = (nca_i64)blockIdx.y * FB;
   const int nf = (F - f0 < FB) ? static_cast<int>(F - f0) : FB;
   const int c = blockIdx.x;
   const nca_i64 p0 = (nca_i64)c * chunk;
   const nca_i64 p1 = (p0 + chunk < P) ? p0 + chunk : P;
-  // runs of equal bins are folded in the accumulator type A (float64), so every pixel value
-  // enters the sum exactly
-  auto flush = [&](int fb, int bin, A s, unsigned n) {
-    if (bin >= 0 && n) {
-      atomicAdd(&hs[fb * nb + bin], s);
-      atomicAdd(&hc[fb * nb + bin], n);
-    }
+  auto slot = [&](I m) { return (m < 0 || m >= nb) ? nb : static_cast<int>(m); };
+  auto add = [&](int fb, int b, double v) {
+    if (SUMS) atomicAdd(&hs[fb * nb1 + b], v);
+    if (CNT == BIN_CNT_NONZERO && v == 0.0) atomicAdd(&hz[fb * nb1 + b], 1u);
   };
-  // vector body: 4 pixels per thread per vector (aligned: host guarantees p0 % 4 == 0); the
-  // loads of U vectors are issued before any of them is folded into the histogram, so each
-  // thread keeps U * FB frame loads in flight across the LDS atomics
-  constexpr int U = (FB == 1) ? 4 : 2;
-  const nca_i64 nvec = (p1 - p0) / 4;
-  for (nca_i64 v0 = threadIdx.x; v0 < nvec; v0 += (nca_i64)U * blockDim.x) {
-    nca_vec<I, 4> m[U];
-    nca_vec<T, 4> x[U][FB];
+  if (SUMS || CNT == BIN_CNT_NONZERO) {
+    // vector body: 4 pixels per vector (host guarantees p0 % 4 == 0 and aligned rows); U vectors
+    // per frame are loaded before any is folded in, so U * FB loads stay in flight
+    constexpr int U = (FB >= 4) ? 1 : (4 / FB);
+    const nca_i64 nvec = (p1 - p0) / 4;
+    for (nca_i64 v0 = threadIdx.x; v0 < nvec; v0 += (nca_i64)U * blockDim.x) {
+      nca_vec<I, 4> m[U];
+      nca_vec<T, 4> x[U][FB];
 #pragma unroll
-    for (int u = 0; u < U; ++u) {
-      const nca_i64 v = v0 + (nca_i64)u * blockDim.x;
-      const nca_i64 p = p0 + 4 * (v < nvec ? v : 0);
-      m[u] = *reinterpret_cast<const nca_vec<I, 4>*>(map + p);
+      for (int u = 0; u < U; ++u) {
+        const nca_i64 v = v0 + (nca_i64)u * blockDim.x;
+        const nca_i64 p = p0 + 4 * (v < nvec ? v : 0);
+        m[u] = *reinterpret_cast<const nca_vec<I, 4>*>(map + p);
 #pragma unroll
-      for (int fb = 0; fb < FB; ++fb)
-        x[u][fb] = *reinterpret_cast<const nca_vec<T, 4>*>(frames + (f0 + (fb < nf ? fb : 0)) * P + p);
-    }
+        for (int fb = 0; fb < FB; ++fb)
+          x[u][fb] = nca_load_nt(reinterpret_cast<const nca_vec<T, 4>*>(frames + (f0 + (fb < nf ? fb : 0)) * P + p));
+      }
 #pragma unroll
-    for (int u = 0; u < U; ++u) {
-      if (v0 + (nca_i64)u * blockDim.x >= nvec) break;
-      int bins[4];
-#pragma unroll
-      for (int j = 0; j < 4; ++j) bins[j] = (m[u].v[j] < 0 || m[u].v[j] >= nb) ? -1 : static_cast<int>(m[u].v[j]);
-#pragma unroll
-      for (int fb = 0; fb < FB; ++fb) {
-        if (fb >= nf) break;
-        int cur = bins[0];
-        A s = 0;
-        unsigned n = 0;
+      for (int u = 0; u < U; ++u) {
+        const bool in = v0 + (nca_i64)u * blockDim.x < nvec;
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
-          const V val = nca_cvt<V, T>::from(x[u][fb].v[j]);
-          if (bins[j] != cur) { flush(fb, cur, s, n); cur = bins[j]; s = 0; n = 0; }
-          s += static_cast<A>(val);
-          n += NONZERO ? (val != V(0) ? 1u : 0u) : 1u;
+          const int b = in ? slot(m[u].v[j]) : nb;
+#pragma unroll
+          for (int fb = 0; fb < FB; ++fb)
+            if (fb < nf) add(fb, b, nca_cvt<double, T>::from(x[u][fb].v[j]));
         }
-        flush(fb, cur, s, n);
       }
     }
-  }
-  for (nca_i64 p = p0 + 4 * nvec + threadIdx.x; p < p1; p += blockDim.x) {
-    const I bi = map[p];
-    if (bi < 0 || bi >= nb) continue;
-    for (int fb = 0; fb < nf; ++fb) {
-      const V val = nca_cvt<V, T>::from(frames[(f0 + fb) * P + p]);
-      flush(fb, static_cast<int>(bi), static_cast<A>(val), NONZERO ? (val != V(0) ? 1u : 0u) : 1u);
+    for (nca_i64 p = p0 + 4 * nvec + threadIdx.x; p < p1; p += blockDim.x) {
+      const int b = slot(map[p]);
+      for (int fb = 0; fb < nf; ++fb) add(fb, b, nca_cvt<double, T>::from(frames[(f0 + fb) * P + p]));
     }
   }
+  if (count_valid) {
+    // valid pixels per bin of sub-slice blockIdx.y of this chunk (slice % 4 == 0)
+    const nca_i64 q0 = p0 + (nca_i64)blockIdx.y * slice;
+    const nca_i64 q1 = (q0 + slice < p1) ? q0 + slice : p1;
+    const nca_i64 nq = (q1 > q0) ? (q1 - q0) / 4 : 0;
+    for (nca_i64 v = threadIdx.x; v < nq; v += blockDim.x) {
+      const nca_vec<I, 4> m = *reinterpret_cast<const nca_vec<I, 4>*>(map + q0 + 4 * v);
+#pragma unroll
+      for (int j = 0; j < 4; ++j) atomicAdd(&hv[slot(m.v[j])], 1u);
+    }
+    for (nca_i64 p = q0 + 4 * nq + threadIdx.x; p < q1; p += blockDim.x) atomicAdd(&hv[slot(map[p])], 1u);
+  }
   __syncthreads();
-  // partial slots of (frame, chunk)
+  // partial slots of (frame, chunk) and of (chunk, sub-slice)
   for (int fb = 0; fb < nf; ++fb) {
     const nca_i64 base = ((f0 + fb) * gridDim.x + c) * (nca_i64)nb;
     for (int b = threadIdx.x; b < nb; b += blockDim.x) {
-      if (psum) psum[base + b] = hs[fb * nb + b];
-      if (pcnt) pcnt[base + b] = hc[fb * nb + b];
+      if (SUMS) psum[base + b] = hs[fb * nb1 + b];
+      if (CNT == BIN_CNT_NONZERO) pzero[base + b] = hz[fb * nb1 + b];
     }
+  }
+  if (count_valid) {
+    const nca_i64 base = ((nca_i64)c * S + blockIdx.y) * nb;
+    for (int b = threadIdx.x; b < nb; b += blockDim.x) pvalid[base + b] = hv[b];
   }
 }
 
-template <typename A, typename CO, typename SO>
-__global__ __launch_bounds__(256) void k_bin_combine(const unsigned* __restrict__ pcnt, const A* __restrict__ psum,
-                                                     int chunks, int nb, nca_i64 F, CO* __restrict__ counts,
-                                                     nca_i64 cst0, nca_i64 cst1, SO* __restrict__ sums, nca_i64 sst0,
-                                                     nca_i64 sst1) {
+// counts[f, b] = valid[b] (- zeros[f, b]); sums[f, b] = sum of the chunk partials in chunk order
+// (deterministic whatever the block schedule).
+template <typename CO, typename SO>
+__global__ __launch_bounds__(256) void k_bin_combine(const double* __restrict__ psum, const unsigned* __restrict__ pzero,
+                                                     const unsigned* __restrict__ pvalid, int chunks, int nvalid, int nb,
+                                                     nca_i64 F, CO* __restrict__ counts, nca_i64 cst0, nca_i64 cst1,
+                                                     SO* __restrict__ sums, nca_i64 sst0, nca_i64 sst1) {
   const nca_i64 k = (nca_i64)blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= F * nb) return;
   const nca_i64 f = k / nb;
   const int b = static_cast<int>(k - f * nb);
-  unsigned long long cnt = 0;
-  double s = 0.0;
-  for (int c = 0; c < chunks; ++c) {
-    const nca_i64 idx = (f * chunks + c) * (nca_i64)nb + b;
-    if (pcnt) cnt += pcnt[idx];
-    if (psum) s += static_cast<double>(psum[idx]);
+  if (counts) {
+    long long cnt = 0;
+    for (int q = 0; q < nvalid; ++q) cnt += pvalid[(nca_i64)q * nb + b];
+    if (pzero)
+      for (int c = 0; c < chunks; ++c) cnt -= pzero[(f * chunks + c) * (nca_i64)nb + b];
+    counts[f * cst0 + b * cst1] = static_cast<CO>(cnt);
   }
-  if (counts) counts[f * cst0 + b * cst1] = static_cast<CO>(cnt);
-  if (sums) nca_st(&sums[f * sst0 + b * sst1], s);
+  if (sums) {
+    double s = 0.0;
+    for (int c = 0; c < chunks; ++c) s += psum[(f * chunks + c) * (nca_i64)nb + b];
+    nca_st(&sums[f * sst0 + b * sst1], s);
+  }
 }
 
 template <typename G, typename I, typename O, int FB>
@@ -284,25 +294,34 @@ int nca_hip_bin(const nca_view* frames, const nca_view* binmap, int32_t n_bins, 
     if (sums) NCA_REQUIRE(sums->dtype == NCA_FLOAT32 || sums->dtype == NCA_FLOAT64, "sums must be float32 or float64");
     hipStream_t s = as_stream(stream);
     DeviceState& dev = current_device();
-    const size_t per_frame = (sizeof(double) + sizeof(unsigned)) * static_cast<size_t>(n_bins);
-    NCA_REQUIRE(per_frame <= static_cast<size_t>(dev.lds_per_block), "too many bins for an LDS histogram");
-    const int fb = frames_per_block(per_frame, static_cast<size_t>(dev.lds_per_block), F, 1, 4);
-    const size_t lds = per_frame * fb;
+    const int cnt_mode = !counts ? BIN_CNT_NONE : (count_pred == NCA_COUNT_NONZERO ? BIN_CNT_NONZERO : BIN_CNT_ALL);
+    const bool do_sums = sums != nullptr;
+    const size_t nb1 = static_cast<size_t>(n_bins) + 1;
+    // LDS per frame of a block (sums, zero counts) and per block (valid-pixel counts)
+    const size_t per_frame = ((do_sums ? sizeof(double) : 0) + (cnt_mode == BIN_CNT_NONZERO ? sizeof(unsigned) : 0)) * nb1;
+    const size_t per_block = (cnt_mode != BIN_CNT_NONE ? sizeof(unsigned) : 0) * nb1;
+    NCA_REQUIRE(per_frame + per_block <= static_cast<size_t>(dev.lds_per_block), "too many bins for an LDS histogram");
+    const int fb = frames_per_block(std::max<size_t>(per_frame, 1), static_cast<size_t>(dev.lds_per_block) - per_block, F,
+                                    1, 4);
+    const size_t lds = per_frame * fb + per_block;
     const int64_t groups = (F + fb - 1) / fb;
-    // chunks per frame group: about 4 blocks per CU overall, chunk a multiple of 4 pixels
-    int64_t chunks = std::max<int64_t>(1, (static_cast<int64_t>(dev.n_cus) * 4 + groups - 1) / std::max<int64_t>(groups, 1));
+    // chunks per frame group: about 2 blocks of 1024 threads per CU overall, chunk a multiple of 4 pixels
+    int64_t chunks = std::max<int64_t>(1, (static_cast<int64_t>(dev.n_cus) * 2 + groups - 1) / std::max<int64_t>(groups, 1));
     chunks = std::min<int64_t>(chunks, std::max<int64_t>(1, P / 4096));
     int64_t chunk = (P + chunks - 1) / chunks;
     chunk = (chunk + 3) / 4 * 4;
     chunks = (P + chunk - 1) / chunk;
     NCA_REQUIRE(F < 65536, "at most 65535 frames per call");
+    // valid-pixel counts: the blocks of frame groups [0, S) each count one S-th of their chunk
+    const int S = static_cast<int>(std::min<int64_t>(8, std::max<int64_t>(groups, 1)));
+    const int64_t slice = ((chunk + S - 1) / S + 3) / 4 * 4;
     const size_t slots = static_cast<size_t>(F) * chunks * n_bins;
-    Workspace& w = workspace(s, 1, slots * (sizeof(double) + sizeof(unsigned)));
+    const size_t vslots = static_cast<size_t>(chunks) * S * n_bins;
+    Workspace& w = workspace(s, 1, slots * (sizeof(double) + sizeof(unsigned)) + vslots * sizeof(unsigned));
     double* psum = static_cast<double*>(w.big);
-    unsigned* pcnt = reinterpret_cast<unsigned*>(psum + slots);
-    if (P == 0 || F == 0) {
-      // empty frames: zero outputs through the combine kernel with zero chunks
-    }
+    unsigned* pzero = reinterpret_cast<unsigned*>(psum + slots);
+    unsigned* pvalid = pzero + slots;
+    const bool work = P > 0 && F > 0;
     dispatch_frame_type(frames->dtype, [&](auto* tp) {
       typedef std::remove_pointer_t<decltype(tp)> T;
       dispatch_map_type(binmap->dtype, [&](auto* ip) {
@@ -310,30 +329,40 @@ int nca_hip_bin(const nca_view* frames, const nca_view* binmap, int32_t n_bins, 
         NCA_REQUIRE(reinterpret_cast<uintptr_t>(frames->data) % (4 * sizeof(T)) == 0 &&
                         reinterpret_cast<uintptr_t>(binmap->data) % (4 * sizeof(I)) == 0 && (P % 4 == 0 || F == 1),
                     "binning needs 4-element aligned frames and map (and pixels per frame % 4 == 0)");
-        if (P > 0 && F > 0) {
-          dim3 grid(static_cast<unsigned>(chunks), static_cast<unsigned>(groups));
-          auto go = [&](auto fbc) {
-            constexpr int FBC = decltype(fbc)::value;
-            if (count_pred == NCA_COUNT_NONZERO)
-              hipLaunchKernelGGL((k_bin<T, I, double, true, FBC>), grid, dim3(512), lds, s, (const T*)frames->data,
-                                 (const I*)binmap->data, (nca_i64)P, n_bins, (nca_i64)chunk, (nca_i64)F, pcnt, psum);
-            else
-              hipLaunchKernelGGL((k_bin<T, I, double, false, FBC>), grid, dim3(512), lds, s, (const T*)frames->data,
-                                 (const I*)binmap->data, (nca_i64)P, n_bins, (nca_i64)chunk, (nca_i64)F, pcnt, psum);
-          };
-          if (fb == 4) go(std::integral_constant<int, 4>{});
-          else if (fb == 2) go(std::integral_constant<int, 2>{});
-          else go(std::integral_constant<int, 1>{});
-        }
+        if (!work) return;
+        const dim3 grid(static_cast<unsigned>(chunks), static_cast<unsigned>(groups));
+        auto go = [&](auto fbc, auto sc, auto cc) {
+          constexpr int FBC = decltype(fbc)::value;
+          constexpr bool SC = decltype(sc)::value;
+          constexpr int CC = decltype(cc)::value;
+          hipLaunchKernelGGL((k_bin<T, I, FBC, SC, CC>), grid, dim3(1024), lds, s, (const T*)frames->data,
+                             (const I*)binmap->data, (nca_i64)P, n_bins, (nca_i64)chunk, (nca_i64)F, S, (nca_i64)slice,
+                             psum, pzero, pvalid);
+        };
+        auto with_cnt = [&](auto fbc, auto sc) {
+          if (cnt_mode == BIN_CNT_NONZERO) go(fbc, sc, std::integral_constant<int, BIN_CNT_NONZERO>{});
+          else if (cnt_mode == BIN_CNT_ALL) go(fbc, sc, std::integral_constant<int, BIN_CNT_ALL>{});
+          else go(fbc, sc, std::integral_constant<int, BIN_CNT_NONE>{});
+        };
+        auto with_sums = [&](auto fbc) {
+          if (do_sums) with_cnt(fbc, std::true_type{});
+          else with_cnt(fbc, std::false_type{});
+        };
+        if (fb == 4) with_sums(std::integral_constant<int, 4>{});
+        else if (fb == 2) with_sums(std::integral_constant<int, 2>{});
+        else with_sums(std::integral_constant<int, 1>{});
       });
     });
-    const int nchunks = (P > 0) ? static_cast<int>(chunks) : 0;
+    // empty frames or map: zero chunks, so the combine writes zero counts and sums
+    const int nchunks = work ? static_cast<int>(chunks) : 0;
+    const int nvalid = work ? static_cast<int>(chunks) * S : 0;
     dim3 g2(static_cast<unsigned>((F * n_bins + 255) / 256));
     auto launch_combine = [&](auto* cp, auto* sp) {
       typedef std::remove_pointer_t<decltype(cp)> CO;
       typedef std::remove_pointer_t<decltype(sp)> SO;
-      hipLaunchKernelGGL((k_bin_combine<double, CO, SO>), g2, dim3(256), 0, s, counts ? pcnt : nullptr,
-                         sums ? psum : nullptr, nchunks, n_bins, (nca_i64)F, counts ? (CO*)counts->data : nullptr,
+      if (g2.x == 0) return;
+      hipLaunchKernelGGL((k_bin_combine<CO, SO>), g2, dim3(256), 0, s, psum, cnt_mode == BIN_CNT_NONZERO ? pzero : nullptr,
+                         pvalid, nchunks, nvalid, n_bins, (nca_i64)F, counts ? (CO*)counts->data : nullptr,
                          (nca_i64)cst0, (nca_i64)cst1, sums ? (SO*)sums->data : nullptr, (nca_i64)sst0, (nca_i64)sst1);
     };
     const bool c64 = counts && counts->dtype == NCA_INT64;

@@ -205,4 +205,26 @@ __device__ __forceinline__ void nca_store_nt(nca_vec<T, N>* p, const nca_vec<T, 
   }
 }
 
+// Streaming (non-temporal) load of a whole vector: for data this kernel reads once.
+template <typename T, int N>
+__device__ __forceinline__ nca_vec<T, N> nca_load_nt(const nca_vec<T, N>* p) {
+  constexpr int B = static_cast<int>(sizeof(T)) * N;
+  nca_vec<T, N> v;
+  if constexpr (B == 16) {
+    const nca_u32x4 w = __builtin_nontemporal_load(reinterpret_cast<const nca_u32x4*>(p));
+    __builtin_memcpy(&v, &w, 16);
+  } else if constexpr (B == 32) {
+    const nca_u32x4 w0 = __builtin_nontemporal_load(reinterpret_cast<const nca_u32x4*>(p));
+    const nca_u32x4 w1 = __builtin_nontemporal_load(reinterpret_cast<const nca_u32x4*>(p) + 1);
+    __builtin_memcpy(&v, &w0, 16);
+    __builtin_memcpy(reinterpret_cast<char*>(&v) + 16, &w1, 16);
+  } else if constexpr (B == 8) {
+    const nca_u32x2 w = __builtin_nontemporal_load(reinterpret_cast<const nca_u32x2*>(p));
+    __builtin_memcpy(&v, &w, 8);
+  } else {
+    v = *p;
+  }
+  return v;
+}
+
 #endif  // NCA_DEVICE_HH

@@ -101,3 +101,105 @@ def test_scatter_add_reference(dev):
     capi.call("nca_hip_scatter_add", ctypes.byref(ti.view(d)), ctypes.byref(ti.view(torch.from_numpy(idx).to(dev))),
               ctypes.byref(ti.view(torch.from_numpy(src).to(dev))), None)
     assert np.array_equal(d.cpu().numpy(), O.scatter_add_f32(dest0, idx, src))
+
+
+def _bin_outputs(capi, ti, frames_d, map_d, nb, pred, want_counts=True, want_sums=True):
+    F = frames_d.shape[0] if frames_d.dim() > 1 else 1
+    shape = (F, nb) if frames_d.dim() > 1 else (nb,)
+    counts = torch.full(shape, -7, dtype=torch.int64, device=frames_d.device) if want_counts else None
+    sums = torch.full(shape, -7.0, dtype=torch.float64, device=frames_d.device) if want_sums else None
+    capi.call("nca_hip_bin", ctypes.byref(ti.view(frames_d)), ctypes.byref(ti.view(map_d)), nb, pred,
+              ctypes.byref(ti.view(counts)) if want_counts else None,
+              ctypes.byref(ti.view(sums)) if want_sums else None, None)
+    return (counts.cpu().numpy() if want_counts else None), (sums.cpu().numpy() if want_sums else None)
+
+
+@pytest.mark.parametrize("pred", [0, 1])
+@pytest.mark.parametrize("want", ["counts", "sums", "both"])
+def test_counts_or_sums_alone(dev, pred, want):
+    """Each output alone equals the same output of the combined call (and the oracle); with
+    COUNT_ALL and no sums the kernel counts from the map only."""
+    capi, ti = _c()
+    F, ny, nx, nb = 21, 40, 52, 37  # F not a multiple of the valid-count groups
+    fr = make_frames(F, ny, nx, seed=11, zero_frac=0.05)
+    bm = O.radial_binmap(ny, nx, 17, 30, nb)
+    bm[3, :7] = -1
+    bm[9, 4:9] = nb + 2
+    c, s = _bin_outputs(capi, ti, torch.from_numpy(fr).to(dev), torch.from_numpy(bm).to(dev), nb, pred,
+                        want in ("counts", "both"), want in ("sums", "both"))
+    rc, rs = O.bin_f32(fr, bm, nb, nonzero=bool(pred))
+    if c is not None:
+        assert np.array_equal(c, rc)
+    if s is not None:
+        assert np.all(np.abs(s - rs) <= 1e-12 * np.abs(rs) + 1e-300)
+
+
+@pytest.mark.parametrize("tdt,mdt", [(torch.bfloat16, torch.int32), (torch.float16, torch.int64),
+                                     (torch.int32, torch.int32), (torch.float64, torch.int64),
+                                     (torch.float32, torch.int64)])
+def test_frame_and_map_dtypes(dev, tdt, mdt):
+    capi, ti = _c()
+    F, ny, nx, nb = 6, 48, 40, 29
+    rng = np.random.default_rng(4)
+    if tdt == torch.int32:
+        frn = rng.integers(-3, 50, (F, ny, nx)).astype(np.int32)
+        frd = torch.from_numpy(frn).to(dev)
+        vals = frn.astype(np.float64)
+    else:
+        fr = make_frames(F, ny, nx, seed=5, zero_frac=0.03)
+        frd = torch.from_numpy(fr).to(dev).to(tdt)  # float32 -> float64 is exact
+        vals = frd.double().cpu().numpy()
+    bm = O.radial_binmap(ny, nx, 20, 21, nb)
+    c, s = _bin_outputs(capi, ti, frd, torch.from_numpy(bm).to(dev).to(mdt), nb, 1)
+    m = bm.ravel()
+    for f in range(F):
+        w = vals[f].ravel()
+        assert np.array_equal(c[f], np.bincount(m[w != 0], minlength=nb))
+        ref = np.bincount(m, weights=w, minlength=nb)
+        assert np.allclose(s[f], ref, rtol=1e-12, atol=0)
+
+
+def test_single_frame_ragged_pixels(dev):
+    """One 1-D frame whose pixel count is not a multiple of 4 (scalar tail paths)."""
+    capi, ti = _c()
+    P, nb = 4099 * 3 + 1, 13
+    rng = np.random.default_rng(8)
+    fr = rng.exponential(2.0, P).astype(np.float32)
+    fr[rng.random(P) < 0.1] = 0.0
+    bm = rng.integers(-2, nb + 2, P).astype(np.int32)
+    c, s = _bin_outputs(capi, ti, torch.from_numpy(fr).to(dev), torch.from_numpy(bm).to(dev), nb, 1)
+    rc, rs = O.bin_f32(fr.reshape(1, P), bm, nb, nonzero=True)
+    assert np.array_equal(c, rc[0])
+    assert np.all(np.abs(s - rs[0]) <= 1e-12 * np.abs(rs[0]))
+
+
+def test_nan_and_signed_zero_predicate(dev):
+    """NaN is a nonzero pixel (counted, and it makes its bin's sum NaN); -0.0 and +0.0 are zeros."""
+    capi, ti = _c()
+    nb = 4
+    bm = np.array([0, 0, 1, 1, 2, 2, 3, 3] * 4, dtype=np.int32)
+    fr = np.ones((2, bm.size), dtype=np.float32)
+    fr[0, 0] = np.nan
+    fr[0, 2] = -0.0
+    fr[1, 3] = 0.0
+    fr[1, 4] = np.inf
+    c, s = _bin_outputs(capi, ti, torch.from_numpy(fr).to(dev), torch.from_numpy(bm).to(dev), nb, 1)
+    rc, rs = O.bin_f32(fr, bm, nb, nonzero=True)
+    assert np.array_equal(c, rc)
+    assert np.array_equal(np.isnan(s), np.isnan(rs))
+    fin = ~np.isnan(rs)
+    assert np.array_equal(s[fin], rs[fin])
+
+
+@pytest.mark.parametrize("fpb", ["2", "4"])
+def test_frames_per_block_override(dev, monkeypatch, fpb):
+    """Blocks that share the map vector over 2 or 4 frames give the same results."""
+    capi, ti = _c()
+    monkeypatch.setenv("NCA_HIP_BIN_FRAMES", fpb)
+    F, ny, nx, nb = 11, 64, 64, 50
+    fr = make_frames(F, ny, nx, seed=9)
+    bm = O.radial_binmap(ny, nx, 32, 32, nb)
+    c, s = _bin_outputs(capi, ti, torch.from_numpy(fr).to(dev), torch.from_numpy(bm).to(dev), nb, 1)
+    rc, rs = O.bin_f32(fr, bm, nb, nonzero=True)
+    assert np.array_equal(c, rc)
+    assert np.all(np.abs(s - rs) <= 1e-12 * np.abs(rs))
