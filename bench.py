@@ -231,7 +231,7 @@ def main():
             "ms_per_step": round(bf_mean, 4),
             "algorithmic_bytes": bytes_bf16,
             "GBps": round(bytes_bf16 / (bf_mean * 1e-3) / 1e9, 2),
-            "accumulate": "float32 compensated (Neumaier) per thread, float64 across z runs",
+            "accumulate": "float32 compensated (Kahan) per thread, float64 across z runs",
         },
         "errors_vs_fp64": errors,
         "other_workloads": extra,

@@ -255,17 +255,18 @@ def laplacian_c0_backward(x, y, g):
     xcode, ycode = _code(x), _code(y)
     lap, adj = laplacian_stencil(xcode, capi.FLOAT32)
     a = torch.empty(x.shape, dtype=torch.float32, device=x.device)
-    lap.apply(ti.view(x), ti.view(a), capi.BOUNDARY_ZERO, scalars=(6.0,), stream=s)
+    va = ti.view(a)
+    lap.apply(ti.view(x), va, capi.BOUNDARY_ZERO, scalars=(6.0,), stream=s)
     tape = c0_tape(capi.FLOAT32, ycode, capi.FLOAT32)
     ga = torch.empty_like(a)
     gy = torch.empty_like(y)
     gf = g if g.dtype == torch.float32 else g.float()
-    ins = [ti.view(a), ti.view(y), ti.view(a), ti.view(a), ti.view(y)]
-    grads = [ti.view(ga), ti.view(gy), ti.view(ga), ti.view(ga), ti.view(gy)]
-    tape.vjp(ins, grads, [1, 1, 1, 1, 1], grad_scalar=gf.data_ptr(), scalars=(0.5,), stream=s)
+    vy, vga, vgy = ti.view(y), ti.view(ga), ti.view(gy)
+    tape.vjp([va, vy, va, va, vy], [vga, vgy, vga, vga, vgy], [1, 1, 1, 1, 1], grad_scalar=gf.data_ptr(),
+             scalars=(0.5,), stream=s)
     # the adjoint of the (x dtype -> float32) Laplacian reads float32 and writes x's dtype
     dx = torch.empty(x.shape, dtype=x.dtype, device=x.device)
-    adj.apply(ti.view(ga), ti.view(dx), capi.BOUNDARY_ZERO, scalars=(6.0,), stream=s)
+    adj.apply(vga, ti.view(dx), capi.BOUNDARY_ZERO, scalars=(6.0,), stream=s)
     return dx, gy
 
 

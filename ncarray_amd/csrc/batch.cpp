@@ -37,6 +37,7 @@ struct BatchPlan {
   int tile = 0, vec = 0, wpc = 0;
   int n_tiles = 0;
   int aligned = 1;  // every pointer 16-byte aligned
+  std::vector<int> alias;  // operand k reads the same arrays as operand alias[k] in every item
   // device tables
   void* d_items = nullptr;      // per item: n_in in ptrs, out ptr, n
   int* d_tile_item = nullptr;
@@ -61,7 +62,7 @@ static BatchPlan& plan_of(uint64_t h) {
   return *it->second;
 }
 
-std::string gen_batch_source(const Tape& t, int fuse, int mode, int vec, bool with_out) {
+std::string gen_batch_source(const Tape& t, int fuse, int mode, int vec, bool with_out, const std::vector<int>& alias_in) {
   const int n_in = static_cast<int>(t.arr_dtypes.size());
   const std::string C = ctype_of(t.compute_dtype);
   const std::string D = ctype_of(t.dest_dtype);
@@ -70,8 +71,9 @@ std::string gen_batch_source(const Tape& t, int fuse, int mode, int vec, bool wi
   const bool fsum = is_float(rcd);
   const std::string SA = fsum ? RC : (dtype_info(rcd).is_signed ? "long long" : "unsigned long long");
   const bool comp = fsum && mode == NCA_ACCUM_COMPENSATED;
+  // operands that alias an earlier one (the same arrays in every item) are loaded once
   std::vector<int> alias(n_in);
-  for (int k = 0; k < n_in; ++k) alias[k] = k;
+  for (int k = 0; k < n_in; ++k) alias[k] = (static_cast<int>(alias_in.size()) == n_in) ? alias_in[k] : k;
   std::vector<std::string> lnames, snames, roots;
   for (int k = 0; k < n_in; ++k) lnames.push_back("u" + std::to_string(k));
   for (size_t i = 0; i < t.scalar_dtypes.size(); ++i) snames.push_back("s" + std::to_string(i));
@@ -93,8 +95,9 @@ std::string gen_batch_source(const Tape& t, int fuse, int mode, int vec, bool wi
   os << "    const nca_i64 n_it = items[it].n;\n";
   os << "    const nca_i64 len = (n_it - st0 < (nca_i64)tile) ? n_it - st0 : (nca_i64)tile;\n";
   for (int k = 0; k < n_in; ++k)
-    os << "    const " << ctype_of(t.arr_dtypes[k]) << "* __restrict__ p" << k << " = reinterpret_cast<const "
-       << ctype_of(t.arr_dtypes[k]) << "*>(items[it].in[" << k << "]) + st0;\n";
+    if (alias[k] == k)
+      os << "    const " << ctype_of(t.arr_dtypes[k]) << "* __restrict__ p" << k << " = reinterpret_cast<const "
+         << ctype_of(t.arr_dtypes[k]) << "*>(items[it].in[" << k << "]) + st0;\n";
   os << "    " << D << "* __restrict__ out = items[it].out ? reinterpret_cast<" << D << "*>(items[it].out) + st0 : nullptr;\n";
   if (fuse & NCA_FUSE_SUM) {
     os << "    " << SA << " acc_s = 0;\n";
@@ -108,22 +111,44 @@ std::string gen_batch_source(const Tape& t, int fuse, int mode, int vec, bool wi
   if (fuse & NCA_FUSE_MAX) upd << "          acc_mx = nca_maxop(acc_mx, rv);\n";
   if (fuse & NCA_FUSE_MIN) upd << "          acc_mn = nca_minop(acc_mn, rv);\n";
   upd << "        }\n";
+  // vector body: two vectors per input in flight per thread (both issued before either is
+  // used), streaming loads and stores (every element is read and written once)
+  auto emit_vec = [&](const std::string& sfx, const std::string& idx) {
+    os << "      {\n      nca_vec<" << D << ", V> r;\n#pragma unroll\n      for (int j = 0; j < V; ++j) {\n";
+    for (int k = 0; k < n_in; ++k)
+      if (alias[k] == k)
+        os << "        const " << C << " u" << k << " = nca_cvt<" << C << ", " << ctype_of(t.arr_dtypes[k]) << ">::from(x" << k
+           << sfx << ".v[j]);\n";
+    if (t.uses_idx) os << "        const nca_i64 lin = st0 + " << idx << " * V + j;\n";
+    os << body;
+    os << "        const " << D << " st = nca_store_cvt<" << D << ", " << C << ">::to(" << roots[0] << ");\n";
+    os << "        r.v[j] = st;\n" << upd.str() << "      }\n";
+    if (with_out)
+      os << "      if (out) nca_store_nt(reinterpret_cast<nca_vec<" << D << ", V>*>(out + " << idx << " * V), r);\n";
+    os << "      }\n";
+  };
+  auto emit_loads = [&](const std::string& sfx, const std::string& idx) {
+    for (int k = 0; k < n_in; ++k)
+      if (alias[k] == k)
+        os << "      const nca_vec<" << ctype_of(t.arr_dtypes[k]) << ", V> x" << k << sfx << " = nca_load_nt(reinterpret_cast<const nca_vec<"
+           << ctype_of(t.arr_dtypes[k]) << ", V>*>(p" << k << " + " << idx << " * V));\n";
+  };
   os << "    const nca_i64 nv = len / V;\n";
-  os << "    for (nca_i64 i = threadIdx.x; i < nv; i += blockDim.x) {\n";
-  for (int k = 0; k < n_in; ++k)
-    os << "      const nca_vec<" << ctype_of(t.arr_dtypes[k]) << ", V> x" << k << " = *reinterpret_cast<const nca_vec<"
-       << ctype_of(t.arr_dtypes[k]) << ", V>*>(p" << k << " + i * V);\n";
-  os << "      nca_vec<" << D << ", V> r;\n#pragma unroll\n      for (int j = 0; j < V; ++j) {\n";
-  for (int k = 0; k < n_in; ++k)
-    os << "        const " << C << " u" << k << " = nca_cvt<" << C << ", " << ctype_of(t.arr_dtypes[k]) << ">::from(x" << k << ".v[j]);\n";
-  if (t.uses_idx) os << "        const nca_i64 lin = st0 + i * V + j;\n";
-  os << body;
-  os << "        const " << D << " st = nca_store_cvt<" << D << ", " << C << ">::to(" << roots[0] << ");\n";
-  os << "        r.v[j] = st;\n" << upd.str() << "      }\n";
-  if (with_out) os << "      if (out) *reinterpret_cast<nca_vec<" << D << ", V>*>(out + i * V) = r;\n";
+  os << "    nca_i64 i = threadIdx.x;\n";
+  os << "    for (; i + (nca_i64)blockDim.x < nv; i += 2 * (nca_i64)blockDim.x) {\n";
+  os << "      const nca_i64 i1 = i + (nca_i64)blockDim.x;\n";
+  emit_loads("a", "i");
+  emit_loads("b", "i1");
+  emit_vec("a", "i");
+  emit_vec("b", "i1");
+  os << "    }\n";
+  os << "    if (i < nv) {\n";
+  emit_loads("a", "i");
+  emit_vec("a", "i");
   os << "    }\n";
   os << "    for (nca_i64 i = nv * V + threadIdx.x; i < len; i += blockDim.x) {\n";
-  for (int k = 0; k < n_in; ++k) os << "        const " << C << " u" << k << " = nca_ld<" << C << ">(p" << k << " + i);\n";
+  for (int k = 0; k < n_in; ++k)
+    if (alias[k] == k) os << "        const " << C << " u" << k << " = nca_ld<" << C << ">(p" << k << " + i);\n";
   if (t.uses_idx) os << "        const nca_i64 lin = st0 + i;\n";
   os << body;
   os << "        const " << D << " st = nca_store_cvt<" << D << ", " << C << ">::to(" << roots[0] << ");\n";
@@ -258,9 +283,12 @@ static void batch_launch_impl(BatchPlan& p, const nca_scalar* scalars, int n_sca
   ScalarSlots sc = make_scalar_slots(t, scalars, n_scalars);
   const int vec = p.aligned ? p.vec : 1;
   std::string key = "batch|" + std::to_string(vec) + "|" + std::to_string(p.fuse) + "|" + std::to_string(p.mode) + "|" +
-                    std::to_string(p.with_out);
-  hipFunction_t fn = program_kernel(ep, key, [&]() { return gen_batch_source(t, p.fuse, p.mode, vec, p.with_out); }, "nca_batch");
-  hipFunction_t comb = program_kernel(ep, key + "|combine", [&]() { return gen_batch_source(t, p.fuse, p.mode, vec, p.with_out); },
+                    std::to_string(p.with_out) + "|a";
+  for (int a : p.alias) key += std::to_string(a) + ",";
+  hipFunction_t fn = program_kernel(ep, key, [&]() { return gen_batch_source(t, p.fuse, p.mode, vec, p.with_out, p.alias); },
+                                    "nca_batch");
+  hipFunction_t comb = program_kernel(ep, key + "|combine",
+                                      [&]() { return gen_batch_source(t, p.fuse, p.mode, vec, p.with_out, p.alias); },
                                       "nca_batch_combine");
   DeviceState& dev = current_device();
   int blocks = std::max(1, dev.n_cus * p.wpc / 4);
@@ -278,7 +306,10 @@ static void batch_launch_impl(BatchPlan& p, const nca_scalar* scalars, int n_sca
   params.push_back(&tile);
   params.push_back(&part);
   for (auto& x : sc.slots) params.push_back(&x);
-  if (p.n_tiles > 0) jit_launch(fn, dim3(blocks), dim3(256), 0, s, params);
+  if (p.n_tiles > 0) {
+    ProfScope ps("nca_batch", s);
+    jit_launch(fn, dim3(blocks), dim3(256), 0, s, params);
+  }
   if (p.fuse && p.n_items > 0) {
     std::vector<void*> cp;
     void* first = p.d_item_first;
@@ -293,6 +324,7 @@ static void batch_launch_impl(BatchPlan& p, const nca_scalar* scalars, int n_sca
     cp.push_back(&sums);
     cp.push_back(&maxs);
     cp.push_back(&mins);
+    ProfScope ps("nca_batch_combine", s);
     jit_launch(comb, dim3((p.n_items + 255) / 256), dim3(256), 0, s, cp);
   }
 }
@@ -335,6 +367,20 @@ int nca_hip_batch_plan(uint64_t expr_handle, int32_t n_items, int32_t n_in, cons
     p->dev = dev.id;
     for (auto v : p->sizes) NCA_REQUIRE(v >= 0, "negative item size");
     for (auto ptr : p->in_ptrs) if (reinterpret_cast<uintptr_t>(ptr) % 16) p->aligned = 0;
+    p->alias.resize(n_in);
+    {
+      const Tape& tp = expr_program(expr_handle).tape;
+      for (int k = 0; k < n_in; ++k) {
+        p->alias[k] = k;
+        for (int j = 0; j < k && p->alias[k] == k; ++j) {
+          if (p->alias[j] != j || tp.arr_dtypes[j] != tp.arr_dtypes[k]) continue;
+          bool same = true;
+          for (int i = 0; i < n_items && same; ++i)
+            same = p->in_ptrs[static_cast<size_t>(i) * n_in + j] == p->in_ptrs[static_cast<size_t>(i) * n_in + k];
+          if (same) p->alias[k] = j;
+        }
+      }
+    }
     for (auto ptr : p->out_ptrs) if (reinterpret_cast<uintptr_t>(ptr) % 16) p->aligned = 0;
     int dt = 1, dv = 0, dw = 0;
     int tt = 65536, tv = 4, tw = 16;

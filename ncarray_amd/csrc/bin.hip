@@ -28,12 +28,14 @@ namespace nca {
 enum { BIN_CNT_NONE = 0, BIN_CNT_ALL = 1, BIN_CNT_NONZERO = 2 };
 
 // One block: a chunk of pixels of FB consecutive frames (the map is read once for all FB).
-// Every pixel adds its value to the LDS float64 sum of its bin with one atomic (out-of-range
-// bins go to a dummy slot nb, so the body has no branch on the bin).  Counting is split so
-// that no per-pixel count atomic is needed: pixels equal to zero are counted per frame (a
-// branch that is rarely taken), and the pixels of each valid bin are counted once per call
+// Every pixel makes one float64 LDS atomic and no branch: a nonzero pixel adds its value to
+// its bin's sum, a zero pixel adds 1.0 to its bin's zero count (a float64 table: exact up to
+// 2^53; leaving +-0 out of a sum that starts at +0 does not change it), and out-of-range bins
+// go to a dummy slot nb.  Counting is split so that no per-pixel count atomic is needed: zero
+// pixels are counted per frame as above, and the pixels of each valid bin are counted once per call
 // from the map alone -- the blocks of the first S frame groups each count one S-th of their
-// chunk.  The combine kernel forms count_all = valid and count_nonzero = valid - zeros.
+// chunk (S up to 256, so the extra work is spread evenly); k_bin_valid_total adds those rows
+// and the combine kernel forms count_all = valid and count_nonzero = valid - zeros.
 // A radial detector map changes bin about once per pixel along a row, so folding runs of
 // equal bins does not pay; the atomics run at LDS rate while the frame loads (non-temporal,
 // read once) stay in flight.
@@ -44,13 +46,19 @@ __global__ __launch_bounds__(1024) void k_bin(const T* __restrict__ frames, cons
                                               unsigned* __restrict__ pvalid) {
   extern __shared__ unsigned char lds_raw[];
   const int nb1 = nb + 1;
+  // zero counts share the sums' float64 atomic when both are wanted (ZD), else uint32
+  constexpr bool ZD = SUMS && CNT == BIN_CNT_NONZERO;
+  constexpr size_t ZB = ZD ? sizeof(double) : sizeof(unsigned);
   double* hs = reinterpret_cast<double*>(lds_raw);                                       // [FB][nb1] sums
-  unsigned* hz = reinterpret_cast<unsigned*>(lds_raw + (SUMS ? sizeof(double) * FB * nb1 : 0));  // [FB][nb1] zeros
-  unsigned* hv = hz + (CNT == BIN_CNT_NONZERO ? FB * nb1 : 0);                           // [nb1] valid pixels
+  unsigned char* zraw = lds_raw + (SUMS ? sizeof(double) * FB * nb1 : 0);
+  double* hzd = reinterpret_cast<double*>(zraw);                                         // [FB][nb1] zeros (ZD)
+  unsigned* hz = reinterpret_cast<unsigned*>(zraw);                                      // [FB][nb1] zeros (!ZD)
+  unsigned* hv = reinterpret_cast<unsigned*>(zraw + (CNT == BIN_CNT_NONZERO ? ZB * FB * nb1 : 0));  // [nb1] valid
   const bool count_valid = CNT != BIN_CNT_NONE && static_cast<int>(blockIdx.y) < S;
   for (int b = threadIdx.x; b < FB * nb1; b += blockDim.x) {
     if (SUMS) hs[b] = 0.0;
-    if (CNT == BIN_CNT_NONZERO) hz[b] = 0u;
+    if (ZD) hzd[b] = 0.0;
+    else if (CNT == BIN_CNT_NONZERO) hz[b] = 0u;
   }
   if (count_valid)
     for (int b = threadIdx.x; b < nb1; b += blockDim.x) hv[b] = 0u;
@@ -62,8 +70,13 @@ __global__ __launch_bounds__(1024) void k_bin(const T* __restrict__ frames, cons
   const nca_i64 p1 = (p0 + chunk < P) ? p0 + chunk : P;
   auto slot = [&](I m) { return (m < 0 || m >= nb) ? nb : static_cast<int>(m); };
   auto add = [&](int fb, int b, double v) {
-    if (SUMS) atomicAdd(&hs[fb * nb1 + b], v);
-    if (CNT == BIN_CNT_NONZERO && v == 0.0) atomicAdd(&hz[fb * nb1 + b], 1u);
+    if (ZD) {
+      const bool z = v == 0.0;
+      atomicAdd(z ? &hzd[fb * nb1 + b] : &hs[fb * nb1 + b], z ? 1.0 : v);
+    } else {
+      if (SUMS) atomicAdd(&hs[fb * nb1 + b], v);
+      if (CNT == BIN_CNT_NONZERO && v == 0.0) atomicAdd(&hz[fb * nb1 + b], 1u);
+    }
   };
   if (SUMS || CNT == BIN_CNT_NONZERO) {
     // vector body: 4 pixels per vector (host guarantees p0 % 4 == 0 and aligned rows); U vectors
@@ -117,7 +130,8 @@ __global__ __launch_bounds__(1024) void k_bin(const T* __restrict__ frames, cons
     const nca_i64 base = ((f0 + fb) * gridDim.x + c) * (nca_i64)nb;
     for (int b = threadIdx.x; b < nb; b += blockDim.x) {
       if (SUMS) psum[base + b] = hs[fb * nb1 + b];
-      if (CNT == BIN_CNT_NONZERO) pzero[base + b] = hz[fb * nb1 + b];
+      if (ZD) pzero[base + b] = static_cast<unsigned>(hzd[fb * nb1 + b]);
+      else if (CNT == BIN_CNT_NONZERO) pzero[base + b] = hz[fb * nb1 + b];
     }
   }
   if (count_valid) {
@@ -126,11 +140,40 @@ __global__ __launch_bounds__(1024) void k_bin(const T* __restrict__ frames, cons
   }
 }
 
+// valid[b] = sum over the `rows` valid-count rows (chunk x sub-slice) of the histogram blocks.
+// Block: 16 bins x 16 row lanes; a wave reads 4 rows of 16 consecutive bins.
+__global__ __launch_bounds__(256) void k_bin_valid_total(const unsigned* __restrict__ pvalid, int rows, int nb,
+                                                         unsigned long long* __restrict__ valid) {
+  __shared__ unsigned long long part[16][17];
+  const int j = threadIdx.x & 15, lane = threadIdx.x >> 4;
+  const int b = blockIdx.x * 16 + j;
+  unsigned long long t = 0;
+  if (b < nb) {
+    // 8 independent loads in flight per thread
+    int r = lane;
+    for (; r + 7 * 16 < rows; r += 8 * 16) {
+      unsigned q[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) q[u] = pvalid[(nca_i64)(r + 16 * u) * nb + b];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) t += q[u];
+    }
+    for (; r < rows; r += 16) t += pvalid[(nca_i64)r * nb + b];
+  }
+  part[lane][j] = t;
+  __syncthreads();
+  if (lane == 0 && b < nb) {
+    unsigned long long s = 0;
+    for (int q = 0; q < 16; ++q) s += part[q][j];
+    valid[b] = s;
+  }
+}
+
 // counts[f, b] = valid[b] (- zeros[f, b]); sums[f, b] = sum of the chunk partials in chunk order
 // (deterministic whatever the block schedule).
 template <typename CO, typename SO>
 __global__ __launch_bounds__(256) void k_bin_combine(const double* __restrict__ psum, const unsigned* __restrict__ pzero,
-                                                     const unsigned* __restrict__ pvalid, int chunks, int nvalid, int nb,
+                                                     const unsigned long long* __restrict__ valid, int chunks, int nb,
                                                      nca_i64 F, CO* __restrict__ counts, nca_i64 cst0, nca_i64 cst1,
                                                      SO* __restrict__ sums, nca_i64 sst0, nca_i64 sst1) {
   const nca_i64 k = (nca_i64)blockIdx.x * blockDim.x + threadIdx.x;
@@ -138,8 +181,7 @@ __global__ __launch_bounds__(256) void k_bin_combine(const double* __restrict__ 
   const nca_i64 f = k / nb;
   const int b = static_cast<int>(k - f * nb);
   if (counts) {
-    long long cnt = 0;
-    for (int q = 0; q < nvalid; ++q) cnt += pvalid[(nca_i64)q * nb + b];
+    long long cnt = valid ? static_cast<long long>(valid[b]) : 0;
     if (pzero)
       for (int c = 0; c < chunks; ++c) cnt -= pzero[(f * chunks + c) * (nca_i64)nb + b];
     counts[f * cst0 + b * cst1] = static_cast<CO>(cnt);
@@ -298,7 +340,8 @@ int nca_hip_bin(const nca_view* frames, const nca_view* binmap, int32_t n_bins, 
     const bool do_sums = sums != nullptr;
     const size_t nb1 = static_cast<size_t>(n_bins) + 1;
     // LDS per frame of a block (sums, zero counts) and per block (valid-pixel counts)
-    const size_t per_frame = ((do_sums ? sizeof(double) : 0) + (cnt_mode == BIN_CNT_NONZERO ? sizeof(unsigned) : 0)) * nb1;
+    const size_t zero_bytes = cnt_mode == BIN_CNT_NONZERO ? (do_sums ? sizeof(double) : sizeof(unsigned)) : 0;
+    const size_t per_frame = ((do_sums ? sizeof(double) : 0) + zero_bytes) * nb1;
     const size_t per_block = (cnt_mode != BIN_CNT_NONE ? sizeof(unsigned) : 0) * nb1;
     NCA_REQUIRE(per_frame + per_block <= static_cast<size_t>(dev.lds_per_block), "too many bins for an LDS histogram");
     const int fb = frames_per_block(std::max<size_t>(per_frame, 1), static_cast<size_t>(dev.lds_per_block) - per_block, F,
@@ -313,14 +356,18 @@ int nca_hip_bin(const nca_view* frames, const nca_view* binmap, int32_t n_bins, 
     chunks = (P + chunk - 1) / chunk;
     NCA_REQUIRE(F < 65536, "at most 65535 frames per call");
     // valid-pixel counts: the blocks of frame groups [0, S) each count one S-th of their chunk
-    const int S = static_cast<int>(std::min<int64_t>(8, std::max<int64_t>(groups, 1)));
+    const int S = static_cast<int>(std::min<int64_t>(256, std::max<int64_t>(groups, 1)));
     const int64_t slice = ((chunk + S - 1) / S + 3) / 4 * 4;
     const size_t slots = static_cast<size_t>(F) * chunks * n_bins;
     const size_t vslots = static_cast<size_t>(chunks) * S * n_bins;
-    Workspace& w = workspace(s, 1, slots * (sizeof(double) + sizeof(unsigned)) + vslots * sizeof(unsigned));
+    Workspace& w = workspace(s, 1, slots * (sizeof(double) + sizeof(unsigned)) + vslots * sizeof(unsigned) +
+                                       static_cast<size_t>(n_bins) * sizeof(unsigned long long) + 8);
     double* psum = static_cast<double*>(w.big);
     unsigned* pzero = reinterpret_cast<unsigned*>(psum + slots);
     unsigned* pvalid = pzero + slots;
+    // 8-byte aligned after the 4-byte tables
+    unsigned long long* valid =
+        reinterpret_cast<unsigned long long*>((reinterpret_cast<uintptr_t>(pvalid + vslots) + 7) & ~uintptr_t(7));
     const bool work = P > 0 && F > 0;
     dispatch_frame_type(frames->dtype, [&](auto* tp) {
       typedef std::remove_pointer_t<decltype(tp)> T;
@@ -335,6 +382,7 @@ int nca_hip_bin(const nca_view* frames, const nca_view* binmap, int32_t n_bins, 
           constexpr int FBC = decltype(fbc)::value;
           constexpr bool SC = decltype(sc)::value;
           constexpr int CC = decltype(cc)::value;
+          ProfScope ps("nca_bin", s);
           hipLaunchKernelGGL((k_bin<T, I, FBC, SC, CC>), grid, dim3(1024), lds, s, (const T*)frames->data,
                              (const I*)binmap->data, (nca_i64)P, n_bins, (nca_i64)chunk, (nca_i64)F, S, (nca_i64)slice,
                              psum, pzero, pvalid);
@@ -353,16 +401,23 @@ int nca_hip_bin(const nca_view* frames, const nca_view* binmap, int32_t n_bins, 
         else with_sums(std::integral_constant<int, 1>{});
       });
     });
-    // empty frames or map: zero chunks, so the combine writes zero counts and sums
+    // empty frames or map: zero chunks and no valid rows, so the combine writes zero counts and sums
     const int nchunks = work ? static_cast<int>(chunks) : 0;
-    const int nvalid = work ? static_cast<int>(chunks) * S : 0;
+    const int rows = work ? static_cast<int>(chunks) * S : 0;
+    if (cnt_mode != BIN_CNT_NONE) {
+      ProfScope ps("nca_bin_valid_total", s);
+      hipLaunchKernelGGL(k_bin_valid_total, dim3(static_cast<unsigned>((n_bins + 15) / 16)), dim3(256), 0, s, pvalid, rows,
+                         n_bins, valid);
+    }
     dim3 g2(static_cast<unsigned>((F * n_bins + 255) / 256));
     auto launch_combine = [&](auto* cp, auto* sp) {
       typedef std::remove_pointer_t<decltype(cp)> CO;
       typedef std::remove_pointer_t<decltype(sp)> SO;
       if (g2.x == 0) return;
+      ProfScope ps("nca_bin_combine", s);
       hipLaunchKernelGGL((k_bin_combine<CO, SO>), g2, dim3(256), 0, s, psum, cnt_mode == BIN_CNT_NONZERO ? pzero : nullptr,
-                         pvalid, nchunks, nvalid, n_bins, (nca_i64)F, counts ? (CO*)counts->data : nullptr,
+                         cnt_mode != BIN_CNT_NONE ? valid : nullptr, nchunks, n_bins, (nca_i64)F,
+                         counts ? (CO*)counts->data : nullptr,
                          (nca_i64)cst0, (nca_i64)cst1, sums ? (SO*)sums->data : nullptr, (nca_i64)sst0, (nca_i64)sst1);
     };
     const bool c64 = counts && counts->dtype == NCA_INT64;
@@ -411,6 +466,7 @@ int nca_hip_bin_backward(const nca_view* grad_sums, const nca_view* binmap, cons
       const dim3 grid(static_cast<unsigned>(chunks), static_cast<unsigned>(groups));
       auto go = [&](auto fbc) {
         constexpr int FBC = decltype(fbc)::value;
+        ProfScope ps("nca_bin_backward", s);
         hipLaunchKernelGGL((k_bin_backward<G, I, O, FBC>), grid, dim3(256), lds, s, (const G*)grad_sums->data,
                            (nca_i64)gst0, (nca_i64)gst1, (const I*)binmap->data, (nca_i64)P, nb, (O*)grad_frames->data,
                            (nca_i64)chunk, (nca_i64)F);
@@ -460,6 +516,7 @@ int nca_hip_scatter_add(const nca_view* dest, const nca_view* idx, const nca_vie
       typedef std::remove_pointer_t<decltype(dp)> D;
       typedef std::remove_pointer_t<decltype(ip)> I;
       typedef std::remove_pointer_t<decltype(sp)> S;
+      ProfScope ps("nca_scatter_add", s);
       hipLaunchKernelGGL((k_scatter_add<D, I, S>), dim3(blocks), dim3(256), 0, s, (D*)dest->data, (nca_i64)nd,
                          (const I*)idx->data, (const S*)src->data, (nca_i64)n);
     };

@@ -772,7 +772,10 @@ int nca_hip_expr_vjp(uint64_t handle, const nca_view* ins, int32_t n_in, const n
     for (auto& s : sc.slots) params.push_back(&s);
     DeviceState& dev = current_device();
     int blocks = grid_blocks(n / vec > 0 ? n / vec : 1, 256, dev.n_cus * 16);
-    if (n > 0) jit_launch(fn, dim3(blocks), dim3(256), 0, as_stream(stream), params);
+    if (n > 0) {
+      ProfScope ps("nca_expr_vjp", as_stream(stream));
+      jit_launch(fn, dim3(blocks), dim3(256), 0, as_stream(stream), params);
+    }
     check_launch("vjp");
     finish_call(stream);
   });

@@ -52,8 +52,10 @@ int grid_blocks(int64_t work_items, int threads, int max_blocks);
 hipFunction_t program_kernel(ExprProgram& p, const std::string& key,
                              const std::function<std::string()>& gen, const std::string& kname);
 
-// Ragged-batch kernel source (batch.cpp).
-std::string gen_batch_source(const Tape& t, int fuse, int mode, int vec, bool with_out);
+// Ragged-batch kernel source (batch.cpp); alias[k] names the operand whose arrays operand k
+// reads (empty: none alias).
+std::string gen_batch_source(const Tape& t, int fuse, int mode, int vec, bool with_out,
+                             const std::vector<int>& alias = {});
 
 }  // namespace nca
 

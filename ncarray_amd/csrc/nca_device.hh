@@ -93,6 +93,17 @@ __device__ __forceinline__ void nca_neumaier_add(F& s, F& c, F x) {
   c += e;
   s = t;
 }
+// Kahan summation with the compensation kept as the correction to add (the represented
+// value is s + c, as for nca_neumaier_add): 4 operations instead of Neumaier's 7, the same
+// first-order bound (2 eps sum|x|).  Used where the accumulation is on the critical ALU
+// path (the march stencil over 2-byte storage).
+template <typename F>
+__device__ __forceinline__ void nca_kahan_add(F& s, F& c, F x) {
+  const F y = x + c;
+  const F t = s + y;
+  c = y - (t - s);
+  s = t;
+}
 // Knuth's TwoSum: s + e == a + b exactly.
 template <typename F>
 __device__ __forceinline__ void nca_two_sum(F a, F b, F& s, F& e) {

@@ -787,6 +787,7 @@ template <int DT> struct RunAxes {
 
 template <int DT> struct RunFinalizeScalar {
   static void run(int kind, const void* p, int n, void* out, hipStream_t s) {
+    ProfScope ps("nca_finalize_scalar", s);
     if (kind == NCA_FUSE_SUM)
       hipLaunchKernelGGL((k_finalize_scalar<DT, NCA_FUSE_SUM>), dim3(1), dim3(256), 0, s, p, n, out);
     else if (kind == NCA_FUSE_MAX)

@@ -276,7 +276,7 @@ static std::string gen_march(const StencilProgram& p, const StencilFuse& fz, int
       os << "      const " << RC << " rv" << j << " = nca_cvt<" << RC << ", " << D << ">::from(r.v[" << j << "]);\n";
     if (fz.axis0)
       for (int j = 0; j < vx; ++j) {
-        if (comp) os << "      nca_neumaier_add(cs[" << j << "], cc[" << j << "], (float)rv" << j << ");\n";
+        if (comp) os << "      nca_kahan_add(cs[" << j << "], cc[" << j << "], (float)rv" << j << ");\n";
         else os << "      cs[" << j << "] += (float)rv" << j << ";\n";
       }
     // pairwise trees over the vx values
@@ -293,7 +293,7 @@ static std::string gen_march(const StencilProgram& p, const StencilFuse& fz, int
     };
     if (fz.fuse & NCA_FUSE_SUM) {
       os << "      const " << RC << " qs = " << tree("nca_add2") << ";\n";
-      if (comp) os << "      nca_neumaier_add(acc_s, acc_c, qs);\n";
+      if (comp) os << "      nca_kahan_add(acc_s, acc_c, qs);\n";
       else os << "      acc_s += qs;\n";
     }
     if (fz.fuse & NCA_FUSE_MAX) os << "      acc_mx = nca_maxop(acc_mx, " << tree("nca_maxop") << ");\n";
@@ -633,7 +633,7 @@ static std::string gen_march_q(const StencilProgram& p, const StencilFuse& fz, i
           os << "        const " << RC << " rv" << j << " = nca_cvt<" << RC << ", " << D << ">::from(res.v[" << j << "]);\n";
         if (fz.axis0)
           for (int j = 0; j < VX; ++j) {
-            if (comp) os << "        nca_neumaier_add(cs[" << r << "][" << j << "], cc[" << r << "][" << j << "], (float)rv" << j << ");\n";
+            if (comp) os << "        nca_kahan_add(cs[" << r << "][" << j << "], cc[" << r << "][" << j << "], (float)rv" << j << ");\n";
             else os << "        cs[" << r << "][" << j << "] += (float)rv" << j << ";\n";
           }
         auto tree = [&](const std::string& f) {
@@ -649,7 +649,7 @@ static std::string gen_march_q(const StencilProgram& p, const StencilFuse& fz, i
         };
         if (fz.fuse & NCA_FUSE_SUM) {
           os << "        const " << RC << " qs = " << tree("nca_add2") << ";\n";
-          if (comp) os << "        nca_neumaier_add(acc_s, acc_c, qs);\n";
+          if (comp) os << "        nca_kahan_add(acc_s, acc_c, qs);\n";
           else os << "        acc_s += qs;\n";
         }
         if (fz.fuse & NCA_FUSE_MAX) os << "        acc_mx = nca_maxop(acc_mx, " << tree("nca_maxop") << ");\n";
