@@ -15,6 +15,7 @@ dominant kernel, and the CPU host baseline (the oracle's restatement of the refe
 host path, timed on this machine's cores).
 """
 import argparse
+import gc
 import json
 import os
 import sys
@@ -46,6 +47,9 @@ def parse():
                     help="also run a plain 512 MiB device copy (for PMC byte-counter calibration)")
     ap.add_argument("--only-headline", action="store_true",
                     help="skip the other workloads (expression, binning, autograd, ragged batch)")
+    ap.add_argument("--workloads", default="c0,c2,c3,c4",
+                    help="comma list of the other workloads to run: c0 (fused expression + sum), c2 (binning), "
+                         "c3 (autograd pipeline), c4 (ragged batch)")
     return ap.parse_args()
 
 
@@ -86,13 +90,22 @@ def main():
             fn()
         torch.cuda.synchronize()
         evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(steps)]
-        t0 = time.perf_counter()
-        for a, b in evs:
-            a.record(stream)
-            fn()
-            b.record(stream)
-        torch.cuda.synchronize()
-        wall = time.perf_counter() - t0
+        # Python's cyclic garbage collector is off while timing (as timeit does): a full
+        # collection in the middle of a host-bound loop (the eager autograd step) stalls the
+        # host for tens of milliseconds and would be charged to one step
+        gc_was_on = gc.isenabled()
+        gc.disable()
+        try:
+            t0 = time.perf_counter()
+            for a, b in evs:
+                a.record(stream)
+                fn()
+                b.record(stream)
+            torch.cuda.synchronize()
+            wall = time.perf_counter() - t0
+        finally:
+            if gc_was_on:
+                gc.enable()
         ms = [a.elapsed_time(b) for a, b in evs]
         return ms, wall
 
@@ -173,8 +186,11 @@ def main():
     # ---- the other workloads: parity-tested elsewhere, measured here ----
     extra = {}
     if not args.only_headline:
+        chosen = {w.strip() for w in args.workloads.split(",") if w.strip()}
         for name, fn in (("c0_fused_expression_sum", bench_c0), ("c2_binning", bench_c2),
                          ("c3_autograd_pipeline", bench_c3), ("c4_ragged_batch", bench_c4)):
+            if name.split("_")[0] not in chosen:
+                continue
             try:
                 extra[name] = fn(torch, capi, tapes, ti, dev, sptr, time_loop)
             except Exception as e:  # report, keep the headline line
@@ -333,11 +349,26 @@ def bench_c2(torch, capi, tapes, ti, dev, sptr, time_loop):
     bwd = lambda: capi.call("nca_hip_bin_backward", ctypes.byref(vg), ctypes.byref(vm), ctypes.byref(vgf), sptr)  # noqa: E731
     fm, _ = time_loop(fwd, 10, 2)
     bm_, _ = time_loop(bwd, 10, 2)
+    # per-kernel device time (HIP events the library records around each launch)
+    capi.profile_enable(True)
+    for _ in range(10):
+        fwd()
+        bwd()
+    torch.cuda.synchronize()
+    recs = capi.profile_read()
+    capi.profile_enable(False)
+    per = {}
+    for name, kms in recs:
+        per.setdefault(name, []).append(kms)
     n = F * ny * nx
     fwd_bytes = 4 * n + 4 * ny * nx + 2 * 4 * F * nb
     bwd_bytes = 4 * n + 4 * ny * nx + 4 * F * nb
     return {"fwd_ms": round(float(np.mean(fm)), 4), "fwd_GBps": gbps(fwd_bytes, np.mean(fm)),
             "bwd_ms": round(float(np.mean(bm_)), 4), "bwd_GBps": gbps(bwd_bytes, np.mean(bm_)),
+            "kernel_ms": {k: round(float(np.mean(v)), 4) for k, v in per.items()},
+            # the histogram kernel reads every pixel and the map once (4 B each)
+            "histogram_kernel_frac_of_8TBps": round((4 * n + 4 * ny * nx) / (np.mean(per["nca_bin"]) * 1e-3) / 1e9
+                                                    / HBM_PEAK_GBPS, 4),
             "fwd_bytes": fwd_bytes, "bwd_bytes": bwd_bytes,
             "counts_total_equals_nonzero_pixels": bool(int(counts.sum()) == int((fr != 0).sum()))}
 
@@ -391,8 +422,9 @@ def bench_c3(torch, capi, tapes, ti, dev, sptr, time_loop):
                     "device_fwd_bwd_frac_of_8TBps": frac(fb_bytes, kms),
                     "device_fwd_bwd_kernels": sorted({n for n, _ in recs}),
                     "loss": float(loss_eager)}
-    res["note"] = ("bytes are algorithmic (each input read once, each output written once); the backward "
-                   "recomputes the Laplacian, runs the derivative expression and the transposed stencil; "
+    res["note"] = ("bytes are algorithmic (each input read once, each output written once); the backward is "
+                   "one march pass (the Laplacian recomputed, both gradients of the C0 expression written) and "
+                   "the transposed stencil; fwd_bwd_ms is host-bound (torch autograd and Python per step); "
                    "graph_fwd replays a hipGraph of 10 forwards (no Python launch overhead); device_fwd_bwd "
                    "sums the step's kernel durations (HIP events per launch); 134 MB working set: "
                    "cache-resident")

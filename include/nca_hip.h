@@ -290,6 +290,24 @@ int nca_hip_stencil_apply_reduce(uint64_t handle, const nca_view* src,
  * symbolically (they may depend on the scalars, not on the loads).  Unit step only;
  * boundary ZERO treats grad_dst outside its shape as 0, NONE requires in-range. */
 int nca_hip_stencil_adjoint_create(uint64_t stencil_handle, uint64_t* handle);
+/* Backward of loss = sum(expr(stencil(src), extras...)) in one march pass over src (no
+ * counterpart in the reference, whose expressions and stencils have no derivatives; it
+ * backs the fused stencil-expression loss of the autograd layer).  At every point the
+ * pass computes a = stencil(src) and writes w = d expr / d a * g and, for every
+ * grad_extras[j] whose data is not NULL, d expr / d extra_j * g; g is read from the
+ * device pointer grad_scalar (the expression's compute type; NULL means 1).
+ * expr_operand[k] names what expression operand k reads: -1 the stencil value, j >= 0
+ * extra j.  For a stencil linear in its loads the gradient of src is the stencil's
+ * adjoint (nca_hip_stencil_adjoint_create) applied to w.  Requires a stencil without
+ * extras or hooks whose z offsets have no x/y part, 3-D C-contiguous arrays of one
+ * shape with the last extent a multiple of the vector width, and boundary ZERO. */
+int nca_hip_stencil_expr_vjp(uint64_t stencil_handle, uint64_t expr_handle,
+                             const int32_t* expr_operand, int32_t n_operand,
+                             const nca_view* src, const nca_view* extras, int32_t n_extra,
+                             const nca_scalar* stencil_scalars, int32_t n_stencil_scalars,
+                             const nca_scalar* expr_scalars, int32_t n_expr_scalars,
+                             const void* grad_scalar, const nca_view* w,
+                             const nca_view* grad_extras, int32_t boundary, void* stream);
 int nca_hip_stencil_source(uint64_t handle, int32_t variant, char* buf, size_t n);
 
 /* ================================================================================ */

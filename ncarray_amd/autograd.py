@@ -8,8 +8,8 @@ Every forward and backward runs a HIP kernel of libnca_hip through the C ABI:
   * SumFunction / MeanFunction: full reductions; backward = a broadcast kernel.
   * BinWeightedSumFunction: per-frame weighted bin sums; backward = a gather kernel.
   * LaplacianC0Loss: loss = sum(a*b + exp(-a*a) - 0.5*b) with a = Laplacian(x), b = y, in
-    ONE forward kernel (stencil + expression + sum, nothing materialised); backward
-    recomputes a, runs the derivative expression and the transposed stencil.
+    ONE forward kernel (stencil + expression + sum, nothing materialised); backward = one
+    march pass that recomputes a and writes dloss/da and dloss/db, then the transposed stencil.
 torch only supplies tensors (device memory) and streams.
 """
 import ctypes
@@ -249,18 +249,40 @@ class LaplacianC0Loss(torch.autograd.Function):
         return laplacian_c0_backward(xc, yc, g.contiguous()) + (None,)
 
 
+def _march_vjp_ok(x, y):
+    """The one-pass backward needs what the march kernel needs: 3-D C-contiguous arrays whose
+    last extent and base addresses hold whole 16-byte vectors of x's elements."""
+    vx = 16 // x.element_size()
+    return (x.dim() == 3 and x.is_contiguous() and y.is_contiguous() and x.shape[-1] % vx == 0
+            and x.data_ptr() % 16 == 0 and y.data_ptr() % (vx * y.element_size()) == 0)
+
+
 def laplacian_c0_backward(x, y, g):
-    """(dx, dy) of sum(C0(Laplacian(x), y)) times the 0-d upstream gradient g."""
+    """(dx, dy) of sum(C0(Laplacian(x), y)) times the 0-d upstream gradient g.
+
+    One march pass computes a = Laplacian(x) on the fly and writes w = dC0/da * g and
+    dy = dC0/db * g (nca_hip_stencil_expr_vjp); the transposed Laplacian of w is dx.  Shapes the
+    march kernel does not take go through the three-kernel composition (Laplacian, derivative
+    expression, transposed stencil)."""
     s = _stream()
     xcode, ycode = _code(x), _code(y)
     lap, adj = laplacian_stencil(xcode, capi.FLOAT32)
+    tape = c0_tape(capi.FLOAT32, ycode, capi.FLOAT32)
+    gf = g if g.dtype == torch.float32 else g.float()
+    if _march_vjp_ok(x, y):
+        w = torch.empty(x.shape, dtype=torch.float32, device=x.device)
+        gy = torch.empty_like(y)
+        vw = ti.view(w)
+        lap.expr_vjp(tape, [-1, 0, -1, -1, 0], ti.view(x), vw, extras=[ti.view(y)], grad_extras=[ti.view(gy)],
+                     scalars=(6.0,), expr_scalars=(0.5,), grad_scalar=gf.data_ptr(), stream=s)
+        dx = torch.empty(x.shape, dtype=x.dtype, device=x.device)
+        adj.apply(vw, ti.view(dx), capi.BOUNDARY_ZERO, scalars=(6.0,), stream=s)
+        return dx, gy
     a = torch.empty(x.shape, dtype=torch.float32, device=x.device)
     va = ti.view(a)
     lap.apply(ti.view(x), va, capi.BOUNDARY_ZERO, scalars=(6.0,), stream=s)
-    tape = c0_tape(capi.FLOAT32, ycode, capi.FLOAT32)
     ga = torch.empty_like(a)
     gy = torch.empty_like(y)
-    gf = g if g.dtype == torch.float32 else g.float()
     vy, vga, vgy = ti.view(y), ti.view(ga), ti.view(gy)
     tape.vjp([va, vy, va, va, vy], [vga, vgy, vga, vga, vgy], [1, 1, 1, 1, 1], grad_scalar=gf.data_ptr(),
              scalars=(0.5,), stream=s)

@@ -426,7 +426,21 @@ static MarchPlan march_plan(const StencilProgram& p, int64_t nx = int64_t(1) << 
   return m;
 }
 
-static std::string gen_march_q(const StencilProgram& p, const StencilFuse& fz, int32_t dst_dtype, const MarchPlan& mp) {
+// Backward of loss = sum(expr(stencil(src), extras)) in one march pass (vjp mode of the queue
+// march): the stencil value a is computed as in the forward, then the expression's symbolic
+// derivatives with respect to a and to the extras are evaluated at a and stored, each scaled
+// by the upstream gradient g read from a device pointer (1 when the pointer is null).
+struct MarchVjp {
+  Tape expr;                        // the pointwise expression (derivative nodes are appended)
+  std::vector<int> operand;         // expression operand k reads: -1 the stencil value, j >= 0 extra j
+  std::vector<int32_t> extra_dtypes;
+  std::vector<int> roots;           // derivative roots, one per output (-1: exactly zero)
+  std::vector<int32_t> out_dtypes;  // output m's dtype (output 0: d/da; then the extras asked for)
+  std::vector<int> alias;           // expression operand k -> first operand reading the same value
+};
+
+static std::string gen_march_q(const StencilProgram& p, const StencilFuse& fz, int32_t dst_dtype, const MarchPlan& mp,
+                               const MarchVjp* vj = nullptr) {
   const Tape& t = p.tape;
   const MarchGeom g = geom_of(p);
   const std::string C = ctype_of(t.compute_dtype);
@@ -441,6 +455,23 @@ static std::string gen_march_q(const StencilProgram& p, const StencilFuse& fz, i
   const int32_t rcd = compute_dtype(dst_dtype);
   const std::string RC = ctype_of(rcd);
   const bool comp = fz.mode == NCA_ACCUM_COMPENSATED;
+  // vjp mode: the expression's derivative code, its loads renamed to the stencil value / extras
+  const std::vector<int32_t>& xdt = vj ? vj->extra_dtypes : p.extra_dtypes;
+  std::string vbody;
+  std::vector<std::string> vnames;
+  if (vj) {
+    std::vector<std::string> en(vj->operand.size()), rn(vj->expr.scalar_dtypes.size());
+    for (size_t k = 0; k < en.size(); ++k)
+      en[k] = vj->operand[k] < 0 ? roots[0] : "u" + std::to_string(p.n_off + vj->operand[k]);
+    for (size_t i = 0; i < rn.size(); ++i) rn[i] = "r" + std::to_string(i);
+    std::vector<int> live;
+    for (int r : vj->roots)
+      if (r >= 0) live.push_back(r);
+    std::vector<std::string> ln;
+    vbody = emit_nodes(vj->expr, live, vj->alias, en, rn, "lin", "q", ln);
+    size_t li = 0;
+    for (int r : vj->roots) vnames.push_back(r >= 0 ? ln[li++] : "(" + C + ")0");
+  }
   const int VX = mp.vx, RPT = mp.rpt, RZ = g.rz, RY = g.ry, RX = g.rx;
   const int Q = 2 * RZ + 2;
   const int TXT = mp.txt, TYT = 256 / mp.txt;
@@ -449,14 +480,24 @@ static std::string gen_march_q(const StencilProgram& p, const StencilFuse& fz, i
   std::ostringstream os;
   os << "#define VX " << VX << "\n#define TX " << TX << "\n#define TY " << TY << "\n#define LH " << LH << "\n#define LW " << LW
      << "\n#define PADL " << PADL << "\n#define RY " << RY << "\n#define RX " << RX << "\n";
-  os << "extern \"C\" __global__ __launch_bounds__(256) void " << march_name(p, fz, dst_dtype) << "(const " << S
-     << "* __restrict__ src";
-  for (size_t j = 0; j < p.extra_dtypes.size(); ++j) os << ", const " << ctype_of(p.extra_dtypes[j]) << "* __restrict__ e" << j;
-  os << ", " << D << "* __restrict__ dst, nca_i64 NZ, nca_i64 NY, nca_i64 NX, int ntx, int nty, int zlen"
-     << ", double* __restrict__ colpart, void* __restrict__ colout, void* __restrict__ psum, void* __restrict__ pmax"
-     << ", void* __restrict__ pmin";
+  const std::string kname = vj ? std::string("nca_stencil_march_vjp_") + dtype_info(p.src_dtype).name : march_name(p, fz, dst_dtype);
+  os << "extern \"C\" __global__ __launch_bounds__(256) void " << kname << "(const " << S << "* __restrict__ src";
+  for (size_t j = 0; j < xdt.size(); ++j) os << ", const " << ctype_of(xdt[j]) << "* __restrict__ e" << j;
+  if (vj) {
+    for (size_t m = 0; m < vj->out_dtypes.size(); ++m) os << ", " << ctype_of(vj->out_dtypes[m]) << "* __restrict__ o" << m;
+    os << ", nca_i64 NZ, nca_i64 NY, nca_i64 NX, int ntx, int nty, int zlen";
+  } else {
+    os << ", " << D << "* __restrict__ dst, nca_i64 NZ, nca_i64 NY, nca_i64 NX, int ntx, int nty, int zlen"
+       << ", double* __restrict__ colpart, void* __restrict__ colout, void* __restrict__ psum, void* __restrict__ pmax"
+       << ", void* __restrict__ pmin";
+  }
   for (size_t i = 0; i < t.scalar_dtypes.size(); ++i) os << ", " << C << " s" << i;
+  if (vj) {
+    for (size_t i = 0; i < vj->expr.scalar_dtypes.size(); ++i) os << ", " << C << " r" << i;
+    os << ", const " << C << "* __restrict__ gin";
+  }
   os << ") {\n";
+  if (vj) os << "  const " << C << " gsc = gin ? *gin : (" << C << ")1;\n";
   // LDS rows are 16-byte aligned (not aligned to a whole VX vector when VX * sizeof(C) > 16)
   os << "  struct __attribute__((aligned(16))) lvec { " << C << " v[VX]; };\n";
   os << "  __shared__ __attribute__((aligned(16))) " << C << " lds[2][LH][LW];\n";
@@ -596,13 +637,17 @@ static std::string gen_march_q(const StencilProgram& p, const StencilFuse& fz, i
           os << "      const " << C << " R" << nm(kv.first) << "_" << m << " = lds[" << buf << "][RY + ty + " << TYT * r << " + ("
              << kv.first << ")][PADL + VX * tx + VX - 1 + " << m << "];\n";
       }
-      for (size_t jx = 0; jx < p.extra_dtypes.size(); ++jx) {
-        const std::string E = ctype_of(p.extra_dtypes[jx]);
+      for (size_t jx = 0; jx < xdt.size(); ++jx) {
+        const std::string E = ctype_of(xdt[jx]);
         os << "      nca_vec<" << E << ", VX> X" << jx << " = {};\n"
            << "      if (valid" << r << ") X" << jx << " = *reinterpret_cast<const nca_vec<" << E << ", VX>*>(e" << jx
            << " + soff + ioff" << r << ");\n";
       }
-      os << "      nca_vec<" << D << ", VX> res;\n";
+      if (vj)
+        for (size_t m = 0; m < vj->out_dtypes.size(); ++m)
+          os << "      nca_vec<" << ctype_of(vj->out_dtypes[m]) << ", VX> res" << m << ";\n";
+      else
+        os << "      nca_vec<" << D << ", VX> res;\n";
       for (int j = 0; j < VX; ++j) {
         os << "      {\n";
         for (int kk = 0; kk < p.n_off; ++kk) {
@@ -619,14 +664,26 @@ static std::string gen_march_q(const StencilProgram& p, const StencilFuse& fz, i
           }
           os << "        const " << C << " u" << kk << " = " << v << ";\n";
         }
-        for (size_t jx = 0; jx < p.extra_dtypes.size(); ++jx)
-          os << "        const " << C << " u" << (p.n_off + jx) << " = nca_cvt<" << C << ", " << ctype_of(p.extra_dtypes[jx])
+        for (size_t jx = 0; jx < xdt.size(); ++jx)
+          os << "        const " << C << " u" << (p.n_off + jx) << " = nca_cvt<" << C << ", " << ctype_of(xdt[jx])
              << ">::from(X" << jx << ".v[" << j << "]);\n";
-        if (t.uses_idx) os << "        const nca_i64 lin = (nca_i64)soff + ioff" << r << " + " << j << ";\n";
+        if (t.uses_idx || (vj && vj->expr.uses_idx))
+          os << "        const nca_i64 lin = (nca_i64)soff + ioff" << r << " + " << j << ";\n";
         os << body;
-        os << "        res.v[" << j << "] = nca_store_cvt<" << D << ", " << C << ">::to(" << roots[0] << ");\n"
-           << "      }\n";
+        if (vj) {
+          os << vbody;
+          for (size_t m = 0; m < vj->out_dtypes.size(); ++m)
+            os << "        res" << m << ".v[" << j << "] = nca_store_cvt<" << ctype_of(vj->out_dtypes[m]) << ", " << C
+               << ">::to((" << C << ")(gsc * " << vnames[m] << "));\n";
+        } else {
+          os << "        res.v[" << j << "] = nca_store_cvt<" << D << ", " << C << ">::to(" << roots[0] << ");\n";
+        }
+        os << "      }\n";
       }
+      if (vj)
+        for (size_t m = 0; m < vj->out_dtypes.size(); ++m)
+          os << "      if (valid" << r << ") nca_store_nt(reinterpret_cast<nca_vec<" << ctype_of(vj->out_dtypes[m])
+             << ", VX>*>(o" << m << " + soff + ioff" << r << "), res" << m << ");\n";
       if (fz.axis0 || fz.fuse) {
         os << "      if (valid" << r << ") {\n";
         for (int j = 0; j < VX; ++j)
@@ -830,6 +887,24 @@ static hipFunction_t stencil_kernel(StencilProgram& p, const std::string& key, c
   return fn;
 }
 
+// Resident 256-thread blocks per CU of a loaded kernel (its registers and LDS), queried once per
+// kernel: the march grids are one resident wave of blocks.
+static int blocks_per_cu(hipFunction_t fn) {
+  static std::mutex mu;
+  static std::unordered_map<hipFunction_t, int> cache;
+  {
+    std::lock_guard<std::mutex> lk(mu);
+    auto it = cache.find(fn);
+    if (it != cache.end()) return it->second;
+  }
+  int per_cu = 0;
+  HIP_CHECK(hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, 256, 0));
+  if (per_cu < 1) per_cu = 1;
+  std::lock_guard<std::mutex> lk(mu);
+  cache[fn] = per_cu;
+  return per_cu;
+}
+
 // ---- apply ------------------------------------------------------------------------------------------------
 static void stencil_apply_impl(uint64_t handle, const nca_view* src, const nca_view* extras, int n_extra,
                                const nca_scalar* scalars, int n_scalars, const nca_view* dst, int boundary,
@@ -913,9 +988,7 @@ static void stencil_apply_impl(uint64_t handle, const nca_view* src, const nca_v
     // One resident wave of blocks: (tiles x z runs) = CUs x blocks per CU for this kernel's
     // registers and LDS, so no block waits for another to finish (no tail), and runs are
     // as long as possible (fewer halo planes).
-    int per_cu = 0;
-    HIP_CHECK(hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, 256, 0));
-    if (per_cu < 1) per_cu = 1;
+    const int per_cu = blocks_per_cu(fn);
     const int64_t resident = static_cast<int64_t>(dev.n_cus) * per_cu;
     int64_t nzc = std::max<int64_t>(1, std::min<int64_t>(resident / std::max<int64_t>(tiles, 1), NZ));
     int zlen = static_cast<int>((NZ + nzc - 1) / nzc);
@@ -1103,6 +1176,134 @@ int nca_hip_stencil_apply_reduce(uint64_t handle, const nca_view* src, const nca
     stencil_apply_impl(handle, src, extras, n_extra, scalars, n_scalars, dst, boundary, nullptr, 0, axis0_sum, fuse,
                        accum_mode, sum_out, max_out, min_out, as_stream(stream));
     check_launch("fused stencil");
+    finish_call(stream);
+  });
+}
+
+int nca_hip_stencil_expr_vjp(uint64_t stencil_handle, uint64_t expr_handle, const int32_t* expr_operand,
+                             int32_t n_operand, const nca_view* src, const nca_view* extras, int32_t n_extra,
+                             const nca_scalar* stencil_scalars, int32_t n_stencil_scalars, const nca_scalar* expr_scalars,
+                             int32_t n_expr_scalars, const void* grad_scalar, const nca_view* w,
+                             const nca_view* grad_extras, int32_t boundary, void* stream) {
+  return api_call([&] {
+    StencilProgram& p = stencil_program(stencil_handle);
+    const Tape& et = expr_program(expr_handle).tape;
+    NCA_REQUIRE(src && w && (n_operand == 0 || expr_operand), "null argument");
+    NCA_REQUIRE(p.extra_dtypes.empty() && !p.has_ext, "the stencil of a stencil-expression vjp reads only its source");
+    NCA_REQUIRE(p.ndim == 3, "the stencil-expression vjp is 3-D");
+    NCA_REQUIRE(boundary == NCA_BOUNDARY_ZERO, "the stencil-expression vjp needs the zero boundary");
+    NCA_REQUIRE(n_operand == static_cast<int32_t>(et.arr_dtypes.size()), "one operand code per expression operand");
+    NCA_REQUIRE(n_extra >= 0 && n_extra <= 8 && (n_extra == 0 || extras), "0..8 extra arrays");
+    NCA_REQUIRE(et.compute_dtype == p.tape.compute_dtype, "the expression and the stencil must compute in one dtype");
+    validate_view(*src, "src");
+    validate_view(*w, "w");
+    NCA_REQUIRE(src->dtype == p.src_dtype, std::string("src dtype must be ") + dtype_info(p.src_dtype).name);
+    NCA_REQUIRE(src->ndim == 3 && same_shape(*src, *w) && is_c_contiguous(*src) && is_c_contiguous(*w),
+                "src and w must be C-contiguous 3-D arrays of one shape");
+    NCA_REQUIRE(is_float(w->dtype), "w must be floating point");
+    MarchVjp vj;
+    vj.expr = et;
+    vj.operand.assign(expr_operand, expr_operand + n_operand);
+    int first_a = -1;
+    for (int k = 0; k < n_operand; ++k) {
+      NCA_REQUIRE(vj.operand[k] >= -1 && vj.operand[k] < n_extra, "expression operand code out of range");
+      if (vj.operand[k] < 0 && first_a < 0) first_a = k;
+    }
+    NCA_REQUIRE(first_a >= 0, "the expression must read the stencil value");
+    for (int j = 0; j < n_extra; ++j) {
+      validate_view(extras[j], "extra");
+      NCA_REQUIRE(same_shape(extras[j], *w) && is_c_contiguous(extras[j]), "extras must be C-contiguous with w's shape");
+      vj.extra_dtypes.push_back(extras[j].dtype);
+    }
+    for (int k = 0; k < n_operand; ++k)
+      if (vj.operand[k] >= 0)
+        NCA_REQUIRE(et.arr_dtypes[k] == extras[vj.operand[k]].dtype, "extra dtype differs from the expression operand's");
+    vj.alias.resize(n_operand);
+    for (int k = 0; k < n_operand; ++k) {
+      vj.alias[k] = k;
+      for (int q = 0; q < k; ++q)
+        if (vj.operand[q] == vj.operand[k]) { vj.alias[k] = q; break; }
+    }
+    // outputs: w = d expr / d a, then d expr / d extra_j for every grad_extras[j] with data
+    std::vector<void*> outs{w->data};
+    vj.roots.push_back(differentiate(vj.expr, vj.expr.root, vj.alias[first_a], vj.alias));
+    vj.out_dtypes.push_back(w->dtype);
+    for (int j = 0; j < n_extra && grad_extras; ++j) {
+      if (!grad_extras[j].data) continue;
+      validate_view(grad_extras[j], "grad_extra");
+      NCA_REQUIRE(same_shape(grad_extras[j], *w) && is_c_contiguous(grad_extras[j]) && is_float(grad_extras[j].dtype),
+                  "extra gradients must be C-contiguous floating-point arrays of w's shape");
+      int var = -1;
+      for (int k = 0; k < n_operand && var < 0; ++k)
+        if (vj.operand[k] == j) var = vj.alias[k];
+      vj.roots.push_back(var >= 0 ? differentiate(vj.expr, vj.expr.root, var, vj.alias) : -1);
+      vj.out_dtypes.push_back(grad_extras[j].dtype);
+      outs.push_back(grad_extras[j].data);
+    }
+    const int64_t NZ = w->shape[0], NY = w->shape[1], NX = w->shape[2];
+    const MarchPlan mp = march_plan(p, NX);
+    const MarchGeom g = geom_of(p);
+    NCA_REQUIRE(mp.queue && g.rz <= 3 && g.ry <= 3 && g.rx <= 3,
+                "the stencil-expression vjp needs a stencil whose z offsets have no x or y part (radius <= 3)");
+    NCA_REQUIRE(NY * NX < (int64_t(1) << 31), "plane too large");
+    const int vx = mp.vx;
+    auto aligned = [&](const void* ptr, int32_t dt) { return reinterpret_cast<uintptr_t>(ptr) % (vx * itemsize(dt)) == 0; };
+    bool ok = NX % vx == 0 && aligned(src->data, src->dtype);
+    for (int j = 0; j < n_extra; ++j) ok = ok && aligned(extras[j].data, extras[j].dtype);
+    for (size_t m = 0; m < outs.size(); ++m) ok = ok && aligned(outs[m], vj.out_dtypes[m]);
+    NCA_REQUIRE(ok, "the stencil-expression vjp needs the last extent a multiple of " + std::to_string(vx) +
+                        " elements and arrays aligned to whole vectors");
+    const int64_t n = view_size(*w);
+    if (n == 0) return;
+    hipStream_t s = as_stream(stream);
+    DeviceState& dev = current_device();
+    std::string key = "vjp|" + et.signature + "|";
+    for (int o : vj.operand) key += std::to_string(o) + ",";
+    key += "|";
+    for (auto d : vj.extra_dtypes) key += std::to_string(d) + ",";
+    key += "|";
+    for (size_t m = 0; m < vj.roots.size(); ++m) key += std::to_string(vj.out_dtypes[m]) + (vj.roots[m] < 0 ? "z," : ",");
+    key += "|" + std::to_string(mp.vx) + "," + std::to_string(mp.rpt) + "," + std::to_string(mp.txt);
+    const std::string kname = std::string("nca_stencil_march_vjp_") + dtype_info(p.src_dtype).name;
+    hipFunction_t fn = stencil_kernel(
+        p, key,
+        [&]() {
+          StencilFuse fz;
+          fz.with_dst = false;
+          return gen_march_q(p, fz, w->dtype, mp, &vj);
+        },
+        kname);
+    const int txw = mp.txt * vx, tyw = (256 / mp.txt) * mp.rpt;
+    const int ntx = static_cast<int>((NX + txw - 1) / txw), nty = static_cast<int>((NY + tyw - 1) / tyw);
+    const int64_t tiles = static_cast<int64_t>(ntx) * nty;
+    const int per_cu = blocks_per_cu(fn);
+    const int64_t resident = static_cast<int64_t>(dev.n_cus) * per_cu;
+    int64_t nzc = std::max<int64_t>(1, std::min<int64_t>(resident / std::max<int64_t>(tiles, 1), NZ));
+    const int zlen = static_cast<int>((NZ + nzc - 1) / nzc);
+    nzc = (NZ + zlen - 1) / zlen;
+    const int64_t blocks = tiles * nzc;
+    NCA_REQUIRE(blocks < (int64_t(1) << 31), "grid too large");
+    ScalarSlots ss = make_scalar_slots(p.tape, stencil_scalars, n_stencil_scalars);
+    ScalarSlots es = make_scalar_slots(et, expr_scalars, n_expr_scalars);
+    std::deque<const void*> ptrs;
+    std::vector<void*> params;
+    ptrs.push_back(src->data);
+    params.push_back(&ptrs.back());
+    for (int j = 0; j < n_extra; ++j) { ptrs.push_back(extras[j].data); params.push_back(&ptrs.back()); }
+    for (void* o : outs) { ptrs.push_back(o); params.push_back(&ptrs.back()); }
+    int64_t dims[3] = {NZ, NY, NX};
+    for (auto& d : dims) params.push_back(&d);
+    int ints[3] = {ntx, nty, zlen};
+    for (auto& i : ints) params.push_back(&i);
+    for (auto& x : ss.slots) params.push_back(&x);
+    for (auto& x : es.slots) params.push_back(&x);
+    ptrs.push_back(grad_scalar);
+    params.push_back(&ptrs.back());
+    {
+      ProfScope ps(kname.c_str(), s);
+      jit_launch(fn, dim3(static_cast<unsigned>(blocks)), dim3(256), 0, s, params);
+    }
+    check_launch("stencil_expr_vjp");
     finish_call(stream);
   });
 }

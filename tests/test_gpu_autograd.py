@@ -142,3 +142,74 @@ def test_unfused_composition_matches_fused(dev):
     A.sum(A.expr(tape, (0.5,), a, y2, a, a, y2)).backward()
     assert torch.allclose(x1.grad, x2.grad, rtol=1e-5, atol=1e-5)
     assert torch.allclose(y1.grad, y2.grad, rtol=1e-6, atol=1e-6)
+
+
+@pytest.mark.parametrize("dtype", [torch.float32, torch.bfloat16])
+def test_stencil_expr_vjp_matches_unfused(dev, dtype):
+    """nca_hip_stencil_expr_vjp (one march pass: Laplacian, derivative expression, two outputs)
+    gives the same bits as the Laplacian kernel followed by the expression's vjp kernel, with an
+    upstream gradient other than 1."""
+    A, capi, tapes = _mods()
+    from ncarray_amd import torch_interop as ti
+    g = torch.Generator(device=dev).manual_seed(11)
+    x = torch.randn((20, 24, 40), generator=g, device=dev).to(dtype)
+    y = torch.randn((20, 24, 40), generator=g, device=dev).to(dtype)
+    gs = torch.tensor(2.5, dtype=torch.float32, device=dev)
+    xc, yc = ti.TORCH_TO_CODE[dtype], ti.TORCH_TO_CODE[dtype]
+    lap, _ = A.laplacian_stencil(xc, capi.FLOAT32)
+    tape = A.c0_tape(capi.FLOAT32, yc, capi.FLOAT32)
+    w = torch.empty(x.shape, dtype=torch.float32, device=dev)
+    gy = torch.empty_like(y)
+    lap.expr_vjp(tape, [-1, 0, -1, -1, 0], ti.view(x), ti.view(w), extras=[ti.view(y)], grad_extras=[ti.view(gy)],
+                 scalars=(6.0,), expr_scalars=(0.5,), grad_scalar=gs.data_ptr())
+    a = torch.empty(x.shape, dtype=torch.float32, device=dev)
+    lap.apply(ti.view(x), ti.view(a), capi.BOUNDARY_ZERO, scalars=(6.0,))
+    ga, gy2 = torch.empty_like(a), torch.empty_like(y)
+    va, vy = ti.view(a), ti.view(y)
+    tape.vjp([va, vy, va, va, vy], [ti.view(ga), ti.view(gy2), ti.view(ga), ti.view(ga), ti.view(gy2)], [1] * 5,
+             grad_scalar=gs.data_ptr(), scalars=(0.5,))
+    torch.cuda.synchronize()
+    assert torch.equal(w, ga)
+    assert torch.equal(gy, gy2)
+    # w alone (no extra gradient asked for), g = 1 when the pointer is null
+    w1 = torch.empty_like(w)
+    lap.expr_vjp(tape, [-1, 0, -1, -1, 0], ti.view(x), ti.view(w1), extras=[ti.view(y)], grad_extras=[None],
+                 scalars=(6.0,), expr_scalars=(0.5,))
+    torch.cuda.synchronize()
+    assert torch.equal(w1 * 2.5, ga)
+
+
+def test_fused_loss_backward_fallback_shape(dev):
+    """A last extent that is not a whole number of vectors takes the three-kernel backward;
+    both backward paths agree with the FP64 gradient."""
+    A, capi, tapes = _mods()
+    rng = np.random.default_rng(4)
+    x = rng.standard_normal((16, 20, 18)).astype(np.float32)
+    y = rng.standard_normal((16, 20, 18)).astype(np.float32)
+    xt = torch.from_numpy(x).to(dev).requires_grad_()
+    yt = torch.from_numpy(y).to(dev).requires_grad_()
+    assert not A._march_vjp_ok(xt.detach(), yt.detach())
+    A.laplacian_c0_loss(xt, yt).backward()
+    gx64, gy64 = grad64(x, y)
+    assert np.max(np.abs(xt.grad.cpu().numpy() - gx64)) <= 1e-4 * max(1.0, np.max(np.abs(gx64)))
+    assert np.max(np.abs(yt.grad.cpu().numpy() - gy64)) <= 1e-5 * max(1.0, np.max(np.abs(gy64)))
+
+
+def test_stencil_expr_vjp_rejects(dev):
+    A, capi, tapes = _mods()
+    from ncarray_amd import torch_interop as ti
+    x = torch.randn((8, 8, 8), device=dev)
+    y = torch.randn((8, 8, 8), device=dev)
+    w = torch.empty_like(x)
+    lap, _ = A.laplacian_stencil(capi.FLOAT32, capi.FLOAT32)
+    tape = A.c0_tape(capi.FLOAT32, capi.FLOAT32, capi.FLOAT32)
+    args = dict(extras=[ti.view(y)], scalars=(6.0,), expr_scalars=(0.5,))
+    with pytest.raises(capi.NcaError, match="zero boundary"):
+        lap.expr_vjp(tape, [-1, 0, -1, -1, 0], ti.view(x), ti.view(w), boundary=capi.BOUNDARY_NONE, **args)
+    with pytest.raises(capi.NcaError, match="read the stencil value"):
+        lap.expr_vjp(tape, [0, 0, 0, 0, 0], ti.view(x), ti.view(w), **args)
+    with pytest.raises(capi.NcaError, match="operand code"):
+        lap.expr_vjp(tape, [-1, 3, -1, -1, 0], ti.view(x), ti.view(w), **args)
+    fused = A.fused_loss_stencil(capi.FLOAT32, capi.FLOAT32)  # has an extra array of its own
+    with pytest.raises(capi.NcaError, match="reads only its source"):
+        fused.expr_vjp(tape, [-1, 0, -1, -1, 0], ti.view(x), ti.view(w), **args)
