@@ -175,23 +175,27 @@ std::string gen_batch_source(const Tape& t, int fuse, int mode, int vec, bool wi
   if (fuse & NCA_FUSE_MIN)
     os << "      " << RC << " m2 = red_mn[0];\n      for (int w = 1; w < 4; ++w) m2 = nca_minop(m2, red_mn[w]);\n      *(" << RC << "*)(slot + 16) = m2;\n";
   os << "    }\n    __syncthreads();\n  }\n}\n";
-  // combine: one thread per item, its tiles in order
+  // one wave per item: lane l folds the item's tile slots l, l + 64, ... in order, then a fixed
+  // butterfly combines the lanes: deterministic for a given plan
   const int32_t sum_dt = reduce_result_dtype(NCA_REDUCE_SUM, t.dest_dtype);
   os << "extern \"C\" __global__ __launch_bounds__(256) void nca_batch_combine(const int* __restrict__ first, int n_items, "
         "int n_tiles, const char* __restrict__ part, " << ctype_of(sum_dt) << "* __restrict__ sums, " << D
      << "* __restrict__ maxs, " << D << "* __restrict__ mins) {\n"
-     << "  const int i = blockIdx.x * blockDim.x + threadIdx.x;\n  if (i >= n_items) return;\n"
+     << "  const int i = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;\n  if (i >= n_items) return;\n"
      << "  const int t0 = first[i], t1 = (i + 1 < n_items) ? first[i + 1] : n_tiles;\n";
   if (fuse & NCA_FUSE_SUM) {
-    if (fsum) os << "  double s = 0.0;\n  for (int t = t0; t < t1; ++t) s += *(const double*)(part + 24 * (size_t)t);\n  nca_st(sums + i, s);\n";
-    else os << "  " << SA << " s = 0;\n  for (int t = t0; t < t1; ++t) s += *(const " << SA << "*)(part + 24 * (size_t)t);\n  sums[i] = (" << ctype_of(sum_dt) << ")s;\n";
+    const std::string ST = fsum ? std::string("double") : SA;
+    os << "  " << ST << " s = 0;\n  for (int t = t0 + lane; t < t1; t += 64) s += *(const " << ST
+       << "*)(part + 24 * (size_t)t);\n  s = nca_wave_sum(s);\n";
+    if (fsum) os << "  if (lane == 0) nca_st(sums + i, s);\n";
+    else os << "  if (lane == 0) sums[i] = (" << ctype_of(sum_dt) << ")s;\n";
   }
   if (fuse & NCA_FUSE_MAX)
-    os << "  " << RC << " m = nca_lim<" << RC << ">::lowest();\n  for (int t = t0; t < t1; ++t) m = nca_maxop(m, *(const " << RC
-       << "*)(part + 24 * (size_t)t + 8));\n  nca_st(maxs + i, m);\n";
+    os << "  " << RC << " m = nca_lim<" << RC << ">::lowest();\n  for (int t = t0 + lane; t < t1; t += 64) m = nca_maxop(m, *(const "
+       << RC << "*)(part + 24 * (size_t)t + 8));\n  m = nca_wave_max(m);\n  if (lane == 0) nca_st(maxs + i, m);\n";
   if (fuse & NCA_FUSE_MIN)
-    os << "  " << RC << " m2 = nca_lim<" << RC << ">::max();\n  for (int t = t0; t < t1; ++t) m2 = nca_minop(m2, *(const " << RC
-       << "*)(part + 24 * (size_t)t + 16));\n  nca_st(mins + i, m2);\n";
+    os << "  " << RC << " m2 = nca_lim<" << RC << ">::max();\n  for (int t = t0 + lane; t < t1; t += 64) m2 = nca_minop(m2, *(const "
+       << RC << "*)(part + 24 * (size_t)t + 16));\n  m2 = nca_wave_min(m2);\n  if (lane == 0) nca_st(mins + i, m2);\n";
   os << "}\n";
   return os.str();
 }
@@ -325,7 +329,7 @@ static void batch_launch_impl(BatchPlan& p, const nca_scalar* scalars, int n_sca
     cp.push_back(&maxs);
     cp.push_back(&mins);
     ProfScope ps("nca_batch_combine", s);
-    jit_launch(comb, dim3((p.n_items + 255) / 256), dim3(256), 0, s, cp);
+    jit_launch(comb, dim3((p.n_items + 3) / 4), dim3(256), 0, s, cp);  // one wave per item
   }
 }
 
@@ -383,7 +387,8 @@ int nca_hip_batch_plan(uint64_t expr_handle, int32_t n_items, int32_t n_in, cons
     }
     for (auto ptr : p->out_ptrs) if (reinterpret_cast<uintptr_t>(ptr) % 16) p->aligned = 0;
     int dt = 1, dv = 0, dw = 0;
-    int tt = 65536, tv = 4, tw = 16;
+    // defaults measured on MI355X for the 4096-array C0 batch: 8192-element tiles, 8 waves per CU
+    int tt = 8192, tv = 4, tw = 8;
     if (db_lookup(batch_db_key(*p, t), dt, dv, dw)) { tt = dt; tv = dv; tw = dw; }
     p->tile = tile_elems > 0 ? tile_elems : tt;
     p->vec = vec > 0 ? vec : tv;
@@ -495,7 +500,7 @@ int nca_hip_batch_autotune(uint64_t plan, const nca_scalar* scalars, int32_t n_s
     BatchPlan& p = plan_of(plan);
     ExprProgram& ep = expr_program(p.expr);
     if (reps < 1) reps = 3;
-    const int tiles[] = {16384, 32768, 65536, 131072, 262144};
+    const int tiles[] = {4096, 8192, 16384, 32768, 65536, 131072};
     const int vecs[] = {2, 4, 8};
     const int wpcs[] = {8, 16, 32};
     hipEvent_t e0, e1;

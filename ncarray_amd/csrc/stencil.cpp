@@ -888,8 +888,13 @@ static hipFunction_t stencil_kernel(StencilProgram& p, const std::string& key, c
 }
 
 // Resident 256-thread blocks per CU of a loaded kernel (its registers and LDS), queried once per
-// kernel: the march grids are one resident wave of blocks.
+// kernel: the march grids are one resident wave of blocks.  NCA_HIP_MARCH_BLOCKS_PER_CU (1..16)
+// overrides the query, for tuning.
 static int blocks_per_cu(hipFunction_t fn) {
+  if (const char* e = std::getenv("NCA_HIP_MARCH_BLOCKS_PER_CU")) {
+    const int v = std::atoi(e);
+    if (v >= 1 && v <= 16) return v;
+  }
   static std::mutex mu;
   static std::unordered_map<hipFunction_t, int> cache;
   {
