@@ -147,6 +147,15 @@ __device__ __forceinline__ void nca_wave_comp(F& s, F& c) {
 template <typename T> __device__ __forceinline__ T nca_maxop(T a, T b) { return (b > a) ? b : a; }
 template <typename T> __device__ __forceinline__ T nca_add2(T a, T b) { return a + b; }
 template <typename T> __device__ __forceinline__ T nca_minop(T a, T b) { return (b < a) ? b : a; }
+// max/min that never return a NaN operand when the other is a number (IEEE maxNum/minNum,
+// one v_max/v_min instruction for floating types); for pairwise trees over values that may hold
+// NaN, where nca_maxop(NaN, x) would return the NaN and drop x.  Integers: nca_maxop/nca_minop.
+template <typename T> __device__ __forceinline__ T nca_maxnum(T a, T b) { return nca_maxop(a, b); }
+template <typename T> __device__ __forceinline__ T nca_minnum(T a, T b) { return nca_minop(a, b); }
+template <> __device__ __forceinline__ float nca_maxnum<float>(float a, float b) { return __builtin_fmaxf(a, b); }
+template <> __device__ __forceinline__ float nca_minnum<float>(float a, float b) { return __builtin_fminf(a, b); }
+template <> __device__ __forceinline__ double nca_maxnum<double>(double a, double b) { return __builtin_fmax(a, b); }
+template <> __device__ __forceinline__ double nca_minnum<double>(double a, double b) { return __builtin_fmin(a, b); }
 template <typename T>
 __device__ __forceinline__ T nca_wave_max(T v) {
 #pragma unroll

@@ -296,8 +296,8 @@ static std::string gen_march(const StencilProgram& p, const StencilFuse& fz, int
       if (comp) os << "      nca_kahan_add(acc_s, acc_c, qs);\n";
       else os << "      acc_s += qs;\n";
     }
-    if (fz.fuse & NCA_FUSE_MAX) os << "      acc_mx = nca_maxop(acc_mx, " << tree("nca_maxop") << ");\n";
-    if (fz.fuse & NCA_FUSE_MIN) os << "      acc_mn = nca_minop(acc_mn, " << tree("nca_minop") << ");\n";
+    if (fz.fuse & NCA_FUSE_MAX) os << "      acc_mx = nca_maxop(acc_mx, " << tree("nca_maxnum") << ");\n";
+    if (fz.fuse & NCA_FUSE_MIN) os << "      acc_mn = nca_minop(acc_mn, " << tree("nca_minnum") << ");\n";
     os << "    }\n";
   }
   if (fz.with_dst)
@@ -709,8 +709,8 @@ static std::string gen_march_q(const StencilProgram& p, const StencilFuse& fz, i
           if (comp) os << "        nca_kahan_add(acc_s, acc_c, qs);\n";
           else os << "        acc_s += qs;\n";
         }
-        if (fz.fuse & NCA_FUSE_MAX) os << "        acc_mx = nca_maxop(acc_mx, " << tree("nca_maxop") << ");\n";
-        if (fz.fuse & NCA_FUSE_MIN) os << "        acc_mn = nca_minop(acc_mn, " << tree("nca_minop") << ");\n";
+        if (fz.fuse & NCA_FUSE_MAX) os << "        acc_mx = nca_maxop(acc_mx, " << tree("nca_maxnum") << ");\n";
+        if (fz.fuse & NCA_FUSE_MIN) os << "        acc_mn = nca_minop(acc_mn, " << tree("nca_minnum") << ");\n";
         os << "      }\n";
       }
       if (fz.with_dst)

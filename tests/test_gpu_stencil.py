@@ -254,3 +254,35 @@ def test_march_general_offsets(dev, offs, src_code, shape):
     assert torch.equal(out2, out)
     assert float(mx) == float(ref.max())
     assert np.allclose(col.cpu().numpy(), ref.astype(np.float64).sum(axis=0), rtol=1e-12, atol=1e-9)
+
+
+@pytest.mark.parametrize("dt", ["float32", "bfloat16"])
+@pytest.mark.parametrize("march", [True, False], ids=["march", "direct"])
+def test_fused_max_min_skip_nan(dev, dt, march):
+    """Fused max/min of a stencil output that holds NaN: NaN is never selected and every other
+    value is still considered (the host fold's rule), also when the NaN and the extreme value
+    sit in the same vector of one thread.  The NaN input at (4, 4, 0) makes the Laplacian NaN at
+    (3, 4, 0) but not at (3, 4, 1); input -1000 at (3, 4, 1) puts the maximum +6000 + ... there
+    and the minimum, -1000 + ..., at its neighbours."""
+    capi, tapes, ti = _c()
+    shape = (8, 8, 64) if march else (8, 8, 62)
+    x = np.random.default_rng(3).standard_normal(shape).astype(np.float32)
+    x[4, 4, 0] = np.nan
+    x[3, 4, 1] = -1000.0
+    code = capi.FLOAT32 if dt == "float32" else capi.BFLOAT16
+    tdt = torch.float32 if dt == "float32" else torch.bfloat16
+    xd = torch.from_numpy(x).to(dev).to(tdt)
+    xs = xd.float().cpu().numpy()  # the stored input (bf16-rounded)
+    st = capi.Stencil(tapes.LAPLACIAN_OFFSETS, tapes.laplacian(), code, capi.FLOAT32, code, [capi.FLOAT32])
+    lap = torch.empty_like(xd)
+    mx = torch.empty((), dtype=tdt, device=dev)
+    mn = torch.empty((), dtype=tdt, device=dev)
+    st.apply_reduce(ti.view(xd), ti.view(lap), capi.BOUNDARY_ZERO, fuse=capi.FUSE_MAX | capi.FUSE_MIN,
+                    mode=capi.ACCUM_COMPENSATED, max_out=mx.data_ptr(), min_out=mn.data_ptr(), scalars=(6.0,))
+    stored = lap.float().cpu().numpy()
+    assert np.isnan(stored[3, 4, 0]) and not np.isnan(stored[3, 4, 1])
+    ref = O.stencil_f32(xs, shape, tapes.LAPLACIAN_OFFSETS, tapes.laplacian(), [6.0], True)
+    if dt == "float32":
+        assert np.array_equal(stored, ref, equal_nan=True)
+    assert mx.float().item() == np.nanmax(stored) and mx.float().item() > 5000
+    assert mn.float().item() == np.nanmin(stored)
