@@ -157,7 +157,7 @@ std::string gen_batch_source(const Tape& t, int fuse, int mode, int vec, bool wi
   // tile partial
   if (fuse & NCA_FUSE_SUM) {
     if (fsum) {
-      if (comp) os << "    nca_wave_comp(acc_s, acc_c);\n    if (lane == 0) red_sd[wid] = (double)acc_s + (double)acc_c;\n";
+      if (comp) os << "    nca_wave_comp(acc_s, acc_c);\n    if (lane == 0) red_sd[wid] = nca_comp_value(acc_s, acc_c);\n";
       else os << "    acc_s = nca_wave_sum(acc_s);\n    if (lane == 0) red_sd[wid] = (double)acc_s;\n";
     } else {
       os << "    acc_s = nca_wave_sum(acc_s);\n    if (lane == 0) red_si[wid] = acc_s;\n";

@@ -376,7 +376,7 @@ static ReduceCode reduce_code(int32_t dest, const ReduceSpec& rs) {
     << "  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = blockDim.x >> 6;\n";
   if (rs.fuse & NCA_FUSE_SUM) {
     if (fsum) {
-      if (rs.mode == NCA_ACCUM_COMPENSATED) f << "  nca_wave_comp(acc_s, acc_c);\n  if (lane == 0) red_sd[wid] = (double)acc_s + (double)acc_c;\n";
+      if (rs.mode == NCA_ACCUM_COMPENSATED) f << "  nca_wave_comp(acc_s, acc_c);\n  if (lane == 0) red_sd[wid] = nca_comp_value(acc_s, acc_c);\n";
       else f << "  acc_s = nca_wave_sum(acc_s);\n  if (lane == 0) red_sd[wid] = (double)acc_s;\n";
     } else {
       f << "  acc_s = nca_wave_sum(acc_s);\n  if (lane == 0) red_si[wid] = acc_s;\n";

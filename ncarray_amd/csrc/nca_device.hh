@@ -94,15 +94,27 @@ __device__ __forceinline__ void nca_neumaier_add(F& s, F& c, F x) {
   s = t;
 }
 // Kahan summation with the compensation kept as the correction to add (the represented
-// value is s + c, as for nca_neumaier_add): 4 operations instead of Neumaier's 7, the same
+// value is s + c, as for nca_neumaier_add): fewer operations than Neumaier's 7, the same
 // first-order bound (2 eps sum|x|).  Used where the accumulation is on the critical ALU
 // path (the march stencil over 2-byte storage).
+// Once the running sum is infinite or NaN the correction (inf - inf) would be NaN and, fed
+// back through y, turn the sum itself into NaN; the correction is then dropped, so the sum
+// follows a plain sum's IEEE value (two operations more).
 template <typename F>
 __device__ __forceinline__ void nca_kahan_add(F& s, F& c, F x) {
   const F y = x + c;
   const F t = s + y;
-  c = y - (t - s);
+  const F e = y - (t - s);
+  c = __builtin_isfinite(t) ? e : F(0);
   s = t;
+}
+// The float64 value of a compensated pair.  When the running sum is infinite or NaN the
+// compensation is meaningless (it holds inf - inf) and the value is the sum itself, as a plain
+// sum gives: a sum that meets +inf is +inf, one that meets +inf and -inf is NaN.
+template <typename F>
+__device__ __forceinline__ double nca_comp_value(F s, F c) {
+  const double ds = static_cast<double>(s);
+  return __builtin_isfinite(ds) ? ds + static_cast<double>(c) : ds;
 }
 // Knuth's TwoSum: s + e == a + b exactly.
 template <typename F>

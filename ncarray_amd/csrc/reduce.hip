@@ -85,7 +85,7 @@ template <int DT, bool COMP> struct AccSum {
     else s += o.s;
   }
   __device__ void to_slot(Slot& sl) const {
-    if constexpr (RT<DT>::fp) sl.a.d = static_cast<double>(s) + static_cast<double>(c);
+    if constexpr (RT<DT>::fp) sl.a.d = nca_comp_value(s, c);
     else sl.a.u = static_cast<unsigned long long>(s);
     sl.idx = 0;
   }

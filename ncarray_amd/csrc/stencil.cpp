@@ -325,18 +325,19 @@ static std::string gen_march(const StencilProgram& p, const StencilFuse& fz, int
   // the global reductions (combined in block order by the same kernel).
   if (fz.axis0) {
     const std::string CT = ctype_of(fz.axis0_dtype);
+    const std::string cv = comp ? "nca_comp_value(cs[j], cc[j])" : "(double)cs[j]";
     os << "  if (valid) {\n"
-       << "    if (colpart) {\n#pragma unroll\n      for (int j = 0; j < VX; ++j) colpart[((nca_i64)tzi * NY + gy) * NX + gx + j] = (double)cs[j]"
-       << (comp ? " + (double)cc[j]" : "") << ";\n"
+       << "    if (colpart) {\n#pragma unroll\n      for (int j = 0; j < VX; ++j) colpart[((nca_i64)tzi * NY + gy) * NX + gx + j] = "
+       << cv << ";\n"
        << "    } else {\n#pragma unroll\n      for (int j = 0; j < VX; ++j) nca_st(reinterpret_cast<" << CT
-       << "*>(colout) + gy * NX + gx + j, (double)cs[j]" << (comp ? " + (double)cc[j]" : "") << ");\n"
+       << "*>(colout) + gy * NX + gx + j, " << cv << ");\n"
        << "    }\n  }\n";
   }
   if (fz.fuse) {
     os << "  __shared__ double red_sd[4];\n  __shared__ " << RC << " red_mx[4];\n  __shared__ " << RC << " red_mn[4];\n"
        << "  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;\n";
     if (fz.fuse & NCA_FUSE_SUM) {
-      if (comp) os << "  nca_wave_comp(acc_s, acc_c);\n  if (lane == 0) red_sd[wid] = (double)acc_s + (double)acc_c;\n";
+      if (comp) os << "  nca_wave_comp(acc_s, acc_c);\n  if (lane == 0) red_sd[wid] = nca_comp_value(acc_s, acc_c);\n";
       else os << "  acc_s = nca_wave_sum(acc_s);\n  if (lane == 0) red_sd[wid] = (double)acc_s;\n";
     }
     if (fz.fuse & NCA_FUSE_MAX) os << "  acc_mx = nca_wave_max(acc_mx);\n  if (lane == 0) red_mx[wid] = acc_mx;\n";
@@ -744,20 +745,21 @@ static std::string gen_march_q(const StencilProgram& p, const StencilFuse& fz, i
   // epilogue (as in gen_march)
   if (fz.axis0) {
     const std::string CT = ctype_of(fz.axis0_dtype);
-    for (int r = 0; r < RPT; ++r)
+    for (int r = 0; r < RPT; ++r) {
+      const std::string cv = comp ? "nca_comp_value(cs[" + I(r) + "][j], cc[" + I(r) + "][j])" : "(double)cs[" + I(r) + "][j]";
       os << "  if (valid" << r << ") {\n"
          << "    if (colpart) {\n#pragma unroll\n      for (int j = 0; j < VX; ++j) colpart[((nca_i64)tzi * NY + gy" << r
-         << ") * NX + gx + j] = (double)cs[" << r << "][j]" << (comp ? " + (double)cc[" + I(r) + "][j]" : "") << ";\n"
+         << ") * NX + gx + j] = " << cv << ";\n"
          << "    } else {\n#pragma unroll\n      for (int j = 0; j < VX; ++j) nca_st(reinterpret_cast<" << CT
-         << "*>(colout) + gy" << r << " * NX + gx + j, (double)cs[" << r << "][j]"
-         << (comp ? " + (double)cc[" + I(r) + "][j]" : "") << ");\n"
+         << "*>(colout) + gy" << r << " * NX + gx + j, " << cv << ");\n"
          << "    }\n  }\n";
+    }
   }
   if (fz.fuse) {
     os << "  __shared__ double red_sd[4];\n  __shared__ " << RC << " red_mx[4];\n  __shared__ " << RC << " red_mn[4];\n"
        << "  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;\n";
     if (fz.fuse & NCA_FUSE_SUM) {
-      if (comp) os << "  nca_wave_comp(acc_s, acc_c);\n  if (lane == 0) red_sd[wid] = (double)acc_s + (double)acc_c;\n";
+      if (comp) os << "  nca_wave_comp(acc_s, acc_c);\n  if (lane == 0) red_sd[wid] = nca_comp_value(acc_s, acc_c);\n";
       else os << "  acc_s = nca_wave_sum(acc_s);\n  if (lane == 0) red_sd[wid] = (double)acc_s;\n";
     }
     if (fz.fuse & NCA_FUSE_MAX) os << "  acc_mx = nca_wave_max(acc_mx);\n  if (lane == 0) red_mx[wid] = acc_mx;\n";
@@ -853,7 +855,7 @@ static std::string gen_direct(const StencilProgram& p, int32_t dst_dtype, bool z
     os << "  __shared__ double red_sd[4];\n  __shared__ " << RC << " red_mx[4];\n  __shared__ " << RC << " red_mn[4];\n"
        << "  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;\n";
     if (fuse & NCA_FUSE_SUM) {
-      if (comp) os << "  nca_wave_comp(acc_s, acc_c);\n  if (lane == 0) red_sd[wid] = (double)acc_s + (double)acc_c;\n";
+      if (comp) os << "  nca_wave_comp(acc_s, acc_c);\n  if (lane == 0) red_sd[wid] = nca_comp_value(acc_s, acc_c);\n";
       else os << "  acc_s = nca_wave_sum(acc_s);\n  if (lane == 0) red_sd[wid] = (double)acc_s;\n";
     }
     if (fuse & NCA_FUSE_MAX) os << "  acc_mx = nca_wave_max(acc_mx);\n  if (lane == 0) red_mx[wid] = acc_mx;\n";

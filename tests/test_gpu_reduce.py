@@ -174,3 +174,58 @@ def test_axis0_sum_512cube(dev):
     mx = full(x, capi.REDUCE_MAX).item()
     assert mx == float(xn.max())
     del x
+
+
+@pytest.mark.parametrize("dtype", [torch.float32, torch.bfloat16, torch.float64])
+def test_compensated_sums_of_infinities(dev, dtype):
+    """A compensated sum that meets +inf is +inf and one that meets +inf and -inf is NaN, as the
+    plain sum (the compensation term of an infinite sum is inf - inf and must not leak)."""
+    capi, ti = _c()
+    n = 100003
+    base = torch.linspace(-1, 1, n, device=dev, dtype=torch.float32).to(dtype)
+    x1 = base.clone()
+    x1[777] = float("inf")
+    x2 = x1.clone()
+    x2[90000] = float("-inf")
+    for mode in (capi.ACCUM_PLAIN, capi.ACCUM_COMPENSATED):
+        assert full(x1, capi.REDUCE_SUM, mode).item() == float("inf"), mode
+        assert np.isnan(full(x2, capi.REDUCE_SUM, mode).item()), mode
+        assert np.isnan(full(x1 - x1, capi.REDUCE_SUM, mode).item()), mode  # inf - inf
+        # axis sums: columns 0 (inf), 1 (finite), 2 (inf and -inf)
+        m = base[:3000].reshape(1000, 3).clone()
+        m[10, 0] = float("inf")
+        m[20, 2] = float("inf")
+        m[30, 2] = float("-inf")
+        s = axes(m.contiguous(), [0], capi.REDUCE_SUM, mode).float().numpy()
+        assert s[0] == np.inf and np.isfinite(s[1]) and np.isnan(s[2]), (mode, s)
+
+
+def test_fused_compensated_sums_of_infinities(dev):
+    """The same through the fused expression reduction and the march stencil's axis-0 and global sums."""
+    from ncarray_amd import tapes
+    capi, ti = _c()
+    a = torch.linspace(-1, 1, 1 << 16, device=dev)
+    a[5] = float("inf")
+    b = torch.zeros_like(a)
+    out = torch.empty_like(a)
+    s = torch.empty((), device=dev)
+    P = capi.pack
+    tape = capi.Tape([P(capi.OP_LOAD_NCARR, 0), P(capi.OP_LOAD_NCARR, 1), P(capi.OP_ADD)], [capi.FLOAT32] * 2, [],
+                     capi.FLOAT32, capi.FLOAT32)
+    tape.launch_reduce([ti.view(a), ti.view(b)], ti.view(out), capi.FUSE_SUM, capi.ACCUM_COMPENSATED, s.data_ptr(),
+                       None, None)
+    torch.cuda.synchronize()
+    assert s.item() == float("inf")
+    # march stencil (identity tape on one offset): the column through the inf sums to inf, others stay finite
+    x = torch.rand((64, 16, 64), device=dev)
+    x[7, 3, 5] = float("inf")
+    st = capi.Stencil([[0, 0, 0]], [P(capi.OP_LOAD_NCARR, 0)], capi.FLOAT32, capi.FLOAT32, capi.FLOAT32)
+    y = torch.empty_like(x)
+    col = torch.empty((16, 64), device=dev)
+    tot = torch.empty((), device=dev)
+    st.apply_reduce(ti.view(x), ti.view(y), capi.BOUNDARY_ZERO, axis0=ti.view(col), fuse=capi.FUSE_SUM,
+                    mode=capi.ACCUM_COMPENSATED, sum_out=tot.data_ptr())
+    torch.cuda.synchronize()
+    c = col.cpu().numpy()
+    assert c[3, 5] == np.inf and np.isfinite(np.delete(c.ravel(), 3 * 64 + 5)).all()
+    assert tot.item() == float("inf")
