@@ -409,6 +409,9 @@ def bench_c3(torch, capi, tapes, ti, dev, sptr, time_loop):
         with torch.no_grad():
             loss_eager = A.laplacian_c0_loss(x, y).clone()
         gfm, _ = graph_loop(torch, fwd, 10, 10)  # forward only (launched from this thread)
+        # forward + backward without torch's autograd (the same kernels, launched from this
+        # thread), 10 steps per hipGraph replay
+        gfbm, _ = graph_loop(torch, lambda: A.laplacian_c0_value_and_grad(x, y), 10, 10)
         fwd_bytes = 2 * item * n3                  # read x, y
         fb_bytes = fwd_bytes + 4 * item * n3       # + read x, y, write dx, dy
         frac = lambda nb, ms: round(nb / (ms * 1e-3) / 1e9 / HBM_PEAK_GBPS, 4)  # noqa: E731
@@ -418,6 +421,8 @@ def bench_c3(torch, capi, tapes, ti, dev, sptr, time_loop):
                     "fwd_bwd_frac_of_8TBps": frac(fb_bytes, np.mean(bm)),
                     "graph_fwd_ms": round(gfm, 4), "graph_fwd_GBps": gbps(fwd_bytes, gfm),
                     "graph_fwd_frac_of_8TBps": frac(fwd_bytes, gfm),
+                    "graph_fwd_bwd_ms": round(gfbm, 4), "graph_fwd_bwd_GBps": gbps(fb_bytes, gfbm),
+                    "graph_fwd_bwd_frac_of_8TBps": frac(fb_bytes, gfbm),
                     "device_fwd_bwd_ms": round(kms, 4), "device_fwd_bwd_GBps": gbps(fb_bytes, kms),
                     "device_fwd_bwd_frac_of_8TBps": frac(fb_bytes, kms),
                     "device_fwd_bwd_kernels": sorted({n for n, _ in recs}),
@@ -425,7 +430,8 @@ def bench_c3(torch, capi, tapes, ti, dev, sptr, time_loop):
     res["note"] = ("bytes are algorithmic (each input read once, each output written once); the backward is "
                    "one march pass (the Laplacian recomputed, both gradients of the C0 expression written) and "
                    "the transposed stencil; fwd_bwd_ms is host-bound (torch autograd and Python per step); "
-                   "graph_fwd replays a hipGraph of 10 forwards (no Python launch overhead); device_fwd_bwd "
+                   "graph_fwd replays a hipGraph of 10 forwards (no Python launch overhead), graph_fwd_bwd one of 10 "
+                   "forward+backward steps (laplacian_c0_value_and_grad); device_fwd_bwd "
                    "sums the step's kernel durations (HIP events per launch); 134 MB working set: "
                    "cache-resident")
     return res

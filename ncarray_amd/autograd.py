@@ -10,6 +10,8 @@ Every forward and backward runs a HIP kernel of libnca_hip through the C ABI:
   * LaplacianC0Loss: loss = sum(a*b + exp(-a*a) - 0.5*b) with a = Laplacian(x), b = y, in
     ONE forward kernel (stencil + expression + sum, nothing materialised); backward = one
     march pass that recomputes a and writes dloss/da and dloss/db, then the transposed stencil.
+    laplacian_c0_value_and_grad runs the same three kernels without torch's autograd, from the
+    calling thread, so a whole step can be captured in a hipGraph.
 torch only supplies tensors (device memory) and streams.
 """
 import ctypes
@@ -257,8 +259,8 @@ def _march_vjp_ok(x, y):
             and x.data_ptr() % 16 == 0 and y.data_ptr() % (vx * y.element_size()) == 0)
 
 
-def laplacian_c0_backward(x, y, g):
-    """(dx, dy) of sum(C0(Laplacian(x), y)) times the 0-d upstream gradient g.
+def laplacian_c0_backward(x, y, g=None):
+    """(dx, dy) of sum(C0(Laplacian(x), y)) times the 0-d upstream gradient g (None: 1).
 
     One march pass computes a = Laplacian(x) on the fly and writes w = dC0/da * g and
     dy = dC0/db * g (nca_hip_stencil_expr_vjp); the transposed Laplacian of w is dx.  Shapes the
@@ -268,16 +270,21 @@ def laplacian_c0_backward(x, y, g):
     xcode, ycode = _code(x), _code(y)
     lap, adj = laplacian_stencil(xcode, capi.FLOAT32)
     tape = c0_tape(capi.FLOAT32, ycode, capi.FLOAT32)
-    gf = g if g.dtype == torch.float32 else g.float()
+    if g is None:
+        gf = None
+    else:
+        gf = g if g.dtype == torch.float32 else g.float()
     if _march_vjp_ok(x, y):
         w = torch.empty(x.shape, dtype=torch.float32, device=x.device)
         gy = torch.empty_like(y)
         vw = ti.view(w)
         lap.expr_vjp(tape, [-1, 0, -1, -1, 0], ti.view(x), vw, extras=[ti.view(y)], grad_extras=[ti.view(gy)],
-                     scalars=(6.0,), expr_scalars=(0.5,), grad_scalar=gf.data_ptr(), stream=s)
+                     scalars=(6.0,), expr_scalars=(0.5,), grad_scalar=None if gf is None else gf.data_ptr(), stream=s)
         dx = torch.empty(x.shape, dtype=x.dtype, device=x.device)
         adj.apply(vw, ti.view(dx), capi.BOUNDARY_ZERO, scalars=(6.0,), stream=s)
         return dx, gy
+    if gf is None:
+        gf = torch.ones((), dtype=torch.float32, device=x.device)
     a = torch.empty(x.shape, dtype=torch.float32, device=x.device)
     va = ti.view(a)
     lap.apply(ti.view(x), va, capi.BOUNDARY_ZERO, scalars=(6.0,), stream=s)
@@ -294,3 +301,19 @@ def laplacian_c0_backward(x, y, g):
 
 def laplacian_c0_loss(x, y, compensated=True):
     return LaplacianC0Loss.apply(x, y, capi.ACCUM_COMPENSATED if compensated else capi.ACCUM_PLAIN)
+
+
+def laplacian_c0_value_and_grad(x, y, compensated=True, grad_scale=None):
+    """(loss, dx, dy) of loss = sum(C0(Laplacian(x), y)) without torch's autograd: the forward
+    kernel and the two-kernel backward, all launched from the calling thread on its current
+    stream, so a whole training step can be captured in one hipGraph.  grad_scale: optional 0-d
+    float32 device tensor scaling the gradients (the upstream gradient; default 1)."""
+    _init(x)
+    xc, yc = x.detach().contiguous(), y.detach().contiguous()
+    st = fused_loss_stencil(_code(xc), _code(yc))
+    loss = torch.empty((), dtype=torch.float32, device=x.device)
+    st.apply_reduce(ti.view(xc), ti.shape_view(list(xc.shape), capi.FLOAT32), capi.BOUNDARY_ZERO,
+                    fuse=capi.FUSE_SUM, mode=capi.ACCUM_COMPENSATED if compensated else capi.ACCUM_PLAIN,
+                    sum_out=loss.data_ptr(), extras=[ti.view(yc)], scalars=(6.0, 0.5), stream=_stream())
+    dx, dy = laplacian_c0_backward(xc, yc, grad_scale)
+    return loss, dx, dy

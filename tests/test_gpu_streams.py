@@ -137,3 +137,30 @@ def test_graph_of_autograd_step(dev, dtype):
         # eager work between replays (not touching the graph's static outputs)
         A.laplacian_c0_loss(x.detach(), y.detach())
         torch.cuda.synchronize()
+
+
+@pytest.mark.parametrize("dtype", [torch.float32, torch.bfloat16])
+def test_graph_of_loss_value_and_grad(dev, dtype):
+    """A whole forward + backward step of the fused Laplacian-C0 loss, launched from this thread
+    (laplacian_c0_value_and_grad, no torch autograd), captured once and replayed: every replay
+    equals the eager step, with eager library work between the replays."""
+    from ncarray_amd import autograd as A
+    gen = torch.Generator(device="cuda").manual_seed(7)
+    x = torch.randn((32, 48, 64), generator=gen, device="cuda").to(dtype)
+    y = torch.randn((32, 48, 64), generator=gen, device="cuda").to(dtype)
+    ref = A.laplacian_c0_value_and_grad(x, y)
+    torch.cuda.synchronize()
+    ref = tuple(t.clone() for t in ref)
+    holder = {}
+    g = _capture(lambda: holder.__setitem__("out", A.laplacian_c0_value_and_grad(x, y)))
+    for k in range(3):
+        g.replay()
+        torch.cuda.synchronize()
+        for a, b in zip(holder["out"], ref):
+            assert torch.equal(a, b), k
+        A.laplacian_c0_value_and_grad(x * 2, y)  # eager work between replays
+        torch.cuda.synchronize()
+    # the same gradients as torch autograd over the library's Function
+    xa, ya = x.clone().requires_grad_(), y.clone().requires_grad_()
+    A.laplacian_c0_loss(xa, ya).backward()
+    assert torch.equal(xa.grad, ref[1]) and torch.equal(ya.grad, ref[2])
