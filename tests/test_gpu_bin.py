@@ -203,3 +203,27 @@ def test_frames_per_block_override(dev, monkeypatch, fpb):
     rc, rs = O.bin_f32(fr, bm, nb, nonzero=True)
     assert np.array_equal(c, rc)
     assert np.all(np.abs(s - rs) <= 1e-12 * np.abs(rs))
+
+
+def test_frame_offsets_beyond_2_31(dev):
+    """2049 uint16 frames of 2^20 pixels (2.15e9 pixels): the last frames start past element 2^31,
+    and their counts and sums are exact (closed forms: bin = p mod 10, every pixel 1 except a
+    marked block of the last frame)."""
+    capi, ti = _c()
+    F, P, nb = 2049, 1 << 20, 10
+    fr = torch.ones((F, P), dtype=torch.uint16, device=dev)
+    fr[F - 1, 1000:1010] = 5          # pixels 1000..1009: one of each bin, +4 each
+    fr[F - 2, 7] = 0                  # a zero pixel in bin 7 of frame F-2
+    bm = (torch.arange(P, device=dev, dtype=torch.int32) % nb).contiguous()
+    counts = torch.empty((F, nb), dtype=torch.int64, device=dev)
+    sums = torch.empty((F, nb), dtype=torch.float64, device=dev)
+    capi.call("nca_hip_bin", ctypes.byref(ti.view(fr)), ctypes.byref(ti.view(bm)), nb, capi.COUNT_NONZERO,
+              ctypes.byref(ti.view(counts)), ctypes.byref(ti.view(sums)), None)
+    c, s = counts.cpu().numpy(), sums.cpu().numpy()
+    per_bin = np.bincount(np.arange(P) % nb, minlength=nb)
+    assert np.array_equal(c[0], per_bin) and np.array_equal(s[0], per_bin.astype(np.float64))
+    assert np.array_equal(c[F - 1], per_bin) and np.array_equal(s[F - 1], per_bin + 4.0)
+    want = per_bin.copy()
+    want[7] -= 1
+    assert np.array_equal(c[F - 2], want) and np.array_equal(s[F - 2], want.astype(np.float64))
+    assert np.array_equal(c[1:F - 2], np.broadcast_to(per_bin, (F - 3, nb)))

@@ -229,3 +229,34 @@ def test_fused_compensated_sums_of_infinities(dev):
     c = col.cpu().numpy()
     assert c[3, 5] == np.inf and np.isfinite(np.delete(c.ravel(), 3 * 64 + 5)).all()
     assert tot.item() == float("inf")
+
+
+def test_beyond_2_31_elements(dev):
+    """64-bit element indexing (the reference's own element index is a 32-bit unsigned,
+    array_impl.hh:2101-2179): on 2^31 + 1000 uint8 elements the full reductions, argmax/argmin,
+    a fused expression and its fused sum all reach the elements past 2^31."""
+    from ncarray_amd import tapes  # noqa: F401
+    capi, ti = _c()
+    n = (1 << 31) + 1000
+    hi = (1 << 31) + 3
+    x = torch.zeros(n, dtype=torch.uint8, device=dev)
+    x[5] = 3
+    x[hi] = 9
+    x[-1] = 7
+    assert full(x, capi.REDUCE_SUM).item() == 19
+    assert full(x, capi.REDUCE_MAX).item() == 9
+    assert full(x, capi.REDUCE_ARGMAX).item() == hi
+    y = torch.ones(n, dtype=torch.uint8, device=dev)
+    y[(1 << 31) + 500] = 0
+    assert full(y, capi.REDUCE_ARGMIN).item() == (1 << 31) + 500
+    del y
+    P = capi.pack
+    tape = capi.Tape([P(capi.OP_LOAD_NCARR, 0), P(capi.OP_LOAD_NCARR, 1), P(capi.OP_ADD)], [capi.UINT8] * 2, [],
+                     capi.UINT8, capi.UINT8)
+    out = torch.empty_like(x)
+    s = torch.zeros((), dtype=torch.int64, device=dev)  # uint64 sum, read as int64 (small value)
+    tape.launch_reduce([ti.view(x), ti.view(x)], ti.view(out), capi.FUSE_SUM, capi.ACCUM_PLAIN, s.data_ptr(), None,
+                       None)
+    torch.cuda.synchronize()
+    assert s.item() == 38
+    assert out[5].item() == 6 and out[hi].item() == 18 and out[-1].item() == 14 and out[hi + 1].item() == 0
