@@ -91,6 +91,11 @@ enum { NCA_FUSE_SUM = 1, NCA_FUSE_MAX = 2, NCA_FUSE_MIN = 4 };
  * ZERO: zero-Dirichlet: a source read outside the array reads 0. */
 enum { NCA_BOUNDARY_NONE = 0, NCA_BOUNDARY_ZERO = 1 };
 
+/* ---- view layouts --------------------------------------------------------------
+ * NC: strided, with an optional NC pointer axis (NCOffsetsPolicy).
+ * SO: PEP 3118 strides and suboffsets (SOArrayPolicy). */
+enum { NCA_LAYOUT_NC = 0, NCA_LAYOUT_SO = 1 };
+
 /* ---- bin-count predicates ------------------------------------------------------ */
 enum { NCA_COUNT_ALL = 0, NCA_COUNT_NONZERO = 1 };
 
@@ -104,8 +109,14 @@ enum { NCA_COUNT_ALL = 0, NCA_COUNT_NONZERO = 1 };
  * i.e. data is a device table of row pointers, strides[P] counts table entries (1 for the
  * reference's layouts, whose advance ignores it; 0 broadcasts one row; k > 1 is a stepped
  * slice), ptr_offset is the reference's m_offsets[P] and row_offset the sum of its byte
- * offsets of the axes after P.  Only the expression entry points (expr_launch,
- * expr_launch_reduce) accept pointer-axis views. */
+ * offsets of the axes after P.
+ * layout == NCA_LAYOUT_SO (pointer_axis must be -1) reads the PEP 3118 form of
+ * SOArrayPolicy::advance (layout.hh:677-700): starting from data, for d = 0 .. ndim-1
+ *     p += i_d * strides[d];  if (suboffsets[d] >= 0) p = *(char**)p + suboffsets[d];
+ * so any number of axes may be indirect.  suboffsets is read only for this layout; an SO
+ * view with no suboffset >= 0 is a plain strided view and is accepted everywhere.
+ * Only the expression entry points (expr_launch, expr_launch_reduce) accept views with an
+ * NC pointer axis or an SO suboffset >= 0. */
 typedef struct nca_view {
   void* data;
   int32_t dtype;
@@ -113,9 +124,10 @@ typedef struct nca_view {
   int64_t shape[NCA_MAX_NDIM];
   int64_t strides[NCA_MAX_NDIM];
   int32_t pointer_axis;
-  int32_t reserved;
+  int32_t layout; /* NCA_LAYOUT_NC (0) or NCA_LAYOUT_SO */
   int64_t ptr_offset;
   int64_t row_offset;
+  int64_t suboffsets[NCA_MAX_NDIM]; /* NCA_LAYOUT_SO only; < 0: a direct axis */
 } nca_view;
 
 /* A scalar constant.  The expression's dtype tables say which member is meant. */

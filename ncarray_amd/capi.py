@@ -32,6 +32,7 @@ ACCUM_PLAIN, ACCUM_COMPENSATED = 0, 1
 FUSE_SUM, FUSE_MAX, FUSE_MIN = 1, 2, 4
 BOUNDARY_NONE, BOUNDARY_ZERO = 0, 1
 COUNT_ALL, COUNT_NONZERO = 0, 1
+LAYOUT_NC, LAYOUT_SO = 0, 1
 # stream argument: the legacy default stream without synchronising (NULL/None synchronises)
 STREAM_DEFAULT_ASYNC = 1
 
@@ -48,8 +49,9 @@ def pack(op, idx=0):
 class View(ctypes.Structure):
     _fields_ = [("data", ctypes.c_void_p), ("dtype", ctypes.c_int32), ("ndim", ctypes.c_int32),
                 ("shape", ctypes.c_int64 * MAX_NDIM), ("strides", ctypes.c_int64 * MAX_NDIM),
-                ("pointer_axis", ctypes.c_int32), ("reserved", ctypes.c_int32),
-                ("ptr_offset", ctypes.c_int64), ("row_offset", ctypes.c_int64)]
+                ("pointer_axis", ctypes.c_int32), ("layout", ctypes.c_int32),
+                ("ptr_offset", ctypes.c_int64), ("row_offset", ctypes.c_int64),
+                ("suboffsets", ctypes.c_int64 * MAX_NDIM)]
 
 
 class Scalar(ctypes.Union):
@@ -200,6 +202,19 @@ def make_nc_view(table_ptr, dtype, shape, strides, pointer_axis, ptr_offset=0, r
     v.pointer_axis = pointer_axis
     v.ptr_offset = ptr_offset
     v.row_offset = row_offset
+    return v
+
+
+def make_so_view(ptr, dtype, shape, strides, suboffsets):
+    """A View in the PEP 3118 form of SOArrayPolicy (layout.hh:677-700): strides in bytes, and
+    for each axis whose suboffset is >= 0 the address after its stride step is replaced by the
+    pointer stored there plus that many bytes."""
+    v = make_view(ptr, dtype, shape, strides)
+    v.layout = LAYOUT_SO
+    for d, so in enumerate(suboffsets):
+        v.suboffsets[d] = int(so)
+    for d in range(len(suboffsets), MAX_NDIM):
+        v.suboffsets[d] = -1
     return v
 
 

@@ -83,12 +83,43 @@ ScalarSlots make_scalar_slots(const Tape& t, const nca_scalar* s, int n) {
 }
 
 // ---- operand analysis ----------------------------------------------------------------------
+// The dereference chain of a view: start address, byte strides, indirect axes and the byte
+// offset added after each pointer read.  An NC pointer axis P (NCOffsetsPolicy::advance,
+// layout.hh:468-480) is one step: the start moves by ptr_offset table entries, strides[P]
+// (table entries) becomes bytes and row_offset is the offset after the read.  An SO view
+// (SOArrayPolicy::advance, layout.hh:677-700) has a step at each axis with suboffset >= 0.
+struct Chain {
+  uintptr_t base = 0;
+  std::array<int64_t, NCA_MAX_NDIM> strides{};
+  std::vector<int> axes;
+  std::vector<int64_t> subs;
+};
+static Chain chain_of(const nca_view& v) {
+  Chain c;
+  c.base = reinterpret_cast<uintptr_t>(v.data);
+  for (int d = 0; d < v.ndim; ++d) c.strides[d] = v.strides[d];
+  if (v.pointer_axis >= 0) {
+    const int P = v.pointer_axis;
+    c.base += static_cast<uintptr_t>(v.ptr_offset * static_cast<int64_t>(sizeof(void*)));
+    c.strides[P] = v.strides[P] * static_cast<int64_t>(sizeof(void*));
+    c.axes.push_back(P);
+    c.subs.push_back(v.row_offset);
+  } else if (v.layout == NCA_LAYOUT_SO) {
+    for (int d = 0; d < v.ndim; ++d)
+      if (v.suboffsets[d] >= 0) {
+        c.axes.push_back(d);
+        c.subs.push_back(v.suboffsets[d]);
+      }
+  }
+  return c;
+}
+
 static bool views_alias(const nca_view& a, const nca_view& b) {
-  if (a.data != b.data || a.dtype != b.dtype || a.ndim != b.ndim) return false;
-  if (a.pointer_axis != b.pointer_axis) return false;
-  if (a.pointer_axis >= 0 && (a.ptr_offset != b.ptr_offset || a.row_offset != b.row_offset)) return false;
+  if (a.dtype != b.dtype || a.ndim != b.ndim) return false;
+  const Chain ca = chain_of(a), cb = chain_of(b);
+  if (ca.base != cb.base || ca.axes != cb.axes || ca.subs != cb.subs) return false;
   for (int d = 0; d < a.ndim; ++d)
-    if (a.shape[d] != b.shape[d] || (a.shape[d] != 1 && a.strides[d] != b.strides[d])) return false;
+    if (a.shape[d] != b.shape[d] || (a.shape[d] != 1 && ca.strides[d] != cb.strides[d])) return false;
   return true;
 }
 
@@ -102,14 +133,18 @@ Operands analyse_operands(const Tape& t, const nca_view* ins, int n_in, const nc
   ops.n = view_size(out);
   ops.alias.resize(n_in);
   ops.bcast.assign(n_in, 0);
-  ops.pax.assign(n_in, -1);
+  ops.base.assign(n_in, nullptr);
+  ops.ind.assign(n_in, {});
+  ops.isub.assign(n_in, {});
+  ops.baseo = out.data;
   for (int k = 0; k < n_in; ++k) {
     const nca_view& v = ins[k];
     validate_view_nc(v, "operand");
     NCA_REQUIRE(v.dtype == t.arr_dtypes[k], "operand " + std::to_string(k) + " dtype " +
                                                 dtype_info(v.dtype).name + " does not match the tape's " +
                                                 dtype_info(t.arr_dtypes[k]).name);
-    if (v.pointer_axis < 0 && (v.ndim == 0 || (is_broadcast_scalar(v) && view_size(v) <= 1) ||
+    ops.base[k] = v.data;
+    if (!view_indirect(v) && (v.ndim == 0 || (is_broadcast_scalar(v) && view_size(v) <= 1) ||
                                (is_broadcast_scalar(v) && v.ndim == out.ndim && same_shape(v, out) && ops.n > 1))) {
       ops.bcast[k] = 1;
     } else {
@@ -142,11 +177,11 @@ Operands analyse_operands(const Tape& t, const nca_view* ins, int n_in, const nc
   for (int k : ops.unique)
     if (!ops.bcast[k] && !aligned(ins[k].data, itemsize(ins[k].dtype))) vec = 1;
   ops.vec = contig ? vec : 1;
-  // NC pointer-axis views: strided kind over the output's own axes (no coalescing across the
-  // pointer axis)
-  bool nc = out.data && out.pointer_axis >= 0;
+  // Indirect views (NC pointer axis, SO suboffsets): strided kind over the output's own axes
+  // (no coalescing across an indirect axis)
+  bool nc = out.data && view_indirect(out);
   for (int k : ops.unique)
-    if (!ops.bcast[k] && ins[k].pointer_axis >= 0) nc = true;
+    if (!ops.bcast[k] && view_indirect(ins[k])) nc = true;
   if (nc) {
     ops.nc = true;
     ops.contiguous = false;
@@ -155,16 +190,22 @@ Operands analyse_operands(const Tape& t, const nca_view* ins, int n_in, const nc
     ops.sndim = nd;
     ops.sshape.assign(nd, 1);
     ops.sout.assign(nd, 0);
-    for (int d = 0; d < out.ndim; ++d) {
-      ops.sshape[d] = out.shape[d];
-      ops.sout[d] = out.data ? out.strides[d] : 0;
+    for (int d = 0; d < out.ndim; ++d) ops.sshape[d] = out.shape[d];
+    if (out.data) {
+      const Chain c = chain_of(out);
+      for (int d = 0; d < out.ndim; ++d) ops.sout[d] = c.strides[d];
+      ops.baseo = reinterpret_cast<const void*>(c.base);
+      ops.indo = c.axes;
+      ops.isubo = c.subs;
     }
-    ops.paxo = out.data ? out.pointer_axis : -1;
     ops.sstrides.assign(n_in, std::vector<int64_t>(nd, 0));
     for (int k : ops.unique) {
       if (ops.bcast[k]) continue;
-      for (int d = 0; d < ins[k].ndim; ++d) ops.sstrides[k][d] = ins[k].strides[d];
-      ops.pax[k] = ins[k].pointer_axis;
+      const Chain c = chain_of(ins[k]);
+      for (int d = 0; d < ins[k].ndim; ++d) ops.sstrides[k][d] = c.strides[d];
+      ops.base[k] = reinterpret_cast<const void*>(c.base);
+      ops.ind[k] = c.axes;
+      ops.isub[k] = c.subs;
     }
     return ops;
   }
@@ -219,9 +260,15 @@ static std::string sig_operands(const Tape& t, const Operands& ops) {
   std::ostringstream os;
   for (size_t k = 0; k < ops.alias.size(); ++k) os << ops.alias[k] << (ops.bcast[k] ? "b" : "") << ",";
   if (ops.nc) {
+    auto axes = [&](const std::vector<int>& a) {
+      os << "[";
+      for (int x : a) os << x << ".";
+      os << "]";
+    };
     os << "nc";
-    for (int k : ops.unique) os << ops.pax[k] << ",";
-    os << "o" << ops.paxo;
+    for (int k : ops.unique) axes(ops.ind[k]);
+    os << "o";
+    axes(ops.indo);
   }
   return os.str();
 }
@@ -248,10 +295,12 @@ static std::string param_list(const Tape& t, const Operands& ops, bool strided, 
     os << "nca_dims shp";
     for (int k : ops.unique) if (!ops.bcast[k]) os << ", nca_dims st" << k;
     if (with_out) os << ", nca_dims sto";
-    // NC views: row-table index offset and row byte offset
+    // indirect views: the byte offset added after each pointer read
     for (int k : ops.unique)
-      if (!ops.bcast[k] && ops.nc && ops.pax[k] >= 0) os << ", nca_i64 po" << k << ", nca_i64 ro" << k;
-    if (with_out && ops.nc && ops.paxo >= 0) os << ", nca_i64 poo, nca_i64 roo";
+      if (!ops.bcast[k] && ops.nc)
+        for (size_t j = 0; j < ops.ind[k].size(); ++j) os << ", nca_i64 q" << k << "_" << j;
+    if (with_out && ops.nc)
+      for (size_t j = 0; j < ops.indo.size(); ++j) os << ", nca_i64 qo_" << j;
   }
   sep();
   os << "nca_i64 n";
@@ -286,55 +335,67 @@ static std::string strided_prelude(int nd) {
   return os.str();
 }
 
-// Body that computes offsets of element `idx` for the strided variant.  For an NC view with
-// pointer axis P (NCOffsetsPolicy::advance, layout.hh:468-480), o accumulates the axes before
-// P (bytes on the row-pointer table), i the table entry along P (index times strides[P], which
-// counts entries) and r the axes after P (bytes on the row).
+// Body that computes offsets of element `idx` for the strided variant.  A plain view
+// accumulates one byte offset (o<k>, oo).  An indirect view with steps at axes a_0 < ... < a_{m-1}
+// accumulates m+1 segment offsets: o<k>_0 over axes 0..a_0, o<k>_j over a_{j-1}+1..a_j and
+// o<k>_m over the axes after a_{m-1}; the address then alternates segment additions and pointer
+// reads (operand_address).
+static void segment_update(std::ostringstream& os, const std::vector<int>& a, const std::string& o,
+                           const std::string& st) {
+  os << "        ";
+  for (size_t j = 0; j < a.size(); ++j)
+    os << "if (d <= " << a[j] << ") " << o << "_" << j << " += c * " << st << ".v[d];\n        else ";
+  os << o << "_" << a.size() << " += c * " << st << ".v[d];\n";
+}
+
 static std::string strided_offsets(const Tape& t, const Operands& ops, bool with_out) {
   std::ostringstream os;
-  auto ncp = [&](int k) { return ops.nc && ops.pax[k] >= 0; };
-  const bool nco = with_out && ops.nc && ops.paxo >= 0;
+  auto indirect = [&](int k) { return ops.nc && !ops.ind[k].empty(); };
+  const bool indo = with_out && ops.nc && !ops.indo.empty();
+  auto decl = [&](const std::string& o, size_t m) {
+    os << "      nca_i64 " << o << "_0 = 0";
+    for (size_t j = 1; j <= m; ++j) os << ", " << o << "_" << j << " = 0";
+    os << ";\n";
+  };
   os << "      nca_i64 rem = idx;\n";
   for (int k : ops.unique)
     if (!ops.bcast[k]) {
-      os << "      nca_i64 o" << k << " = 0;\n";
-      if (ncp(k)) os << "      nca_i64 i" << k << " = 0, r" << k << " = 0;\n";
+      if (indirect(k)) decl("o" + std::to_string(k), ops.ind[k].size());
+      else os << "      nca_i64 o" << k << " = 0;\n";
     }
   if (with_out) {
-    os << "      nca_i64 oo = 0;\n";
-    if (nco) os << "      nca_i64 io = 0, rpo = 0;\n";
+    if (indo) decl("oo", ops.indo.size());
+    else os << "      nca_i64 oo = 0;\n";
   }
   os << "#pragma unroll\n      for (int d = NCA_ND - 1; d >= 0; --d) {\n"
      << "        const nca_i64 c = rem % shp.v[d];\n        rem /= shp.v[d];\n";
   for (int k : ops.unique) {
     if (ops.bcast[k]) continue;
-    if (ncp(k))
-      os << "        if (d < " << ops.pax[k] << ") o" << k << " += c * st" << k << ".v[d];\n"
-         << "        else if (d == " << ops.pax[k] << ") i" << k << " = c * st" << k << ".v[d];\n"
-         << "        else r" << k << " += c * st" << k << ".v[d];\n";
-    else
-      os << "        o" << k << " += c * st" << k << ".v[d];\n";
+    if (indirect(k)) segment_update(os, ops.ind[k], "o" + std::to_string(k), "st" + std::to_string(k));
+    else os << "        o" << k << " += c * st" << k << ".v[d];\n";
   }
   if (with_out) {
-    if (nco)
-      os << "        if (d < " << ops.paxo << ") oo += c * sto.v[d];\n"
-         << "        else if (d == " << ops.paxo << ") io = c * sto.v[d];\n"
-         << "        else rpo += c * sto.v[d];\n";
-    else
-      os << "        oo += c * sto.v[d];\n";
+    if (indo) segment_update(os, ops.indo, "oo", "sto");
+    else os << "        oo += c * sto.v[d];\n";
   }
   os << "      }\n";
   return os.str();
 }
 
-// Address expression of operand k (strided kind): plain views add the offset, NC views read
-// the row pointer first.
+// Address of an element: start + segment 0, then per step a pointer read plus its byte offset
+// and the next segment.
+static std::string chain_address(const std::string& p, const std::string& o, const std::string& q, size_t m) {
+  std::string a = "(" + p + " + " + o + "_0)";
+  for (size_t j = 0; j < m; ++j)
+    a = "(nca_chase(" + a + ", " + q + std::to_string(j) + ") + " + o + "_" + std::to_string(j + 1) + ")";
+  return a;
+}
+
+// Address expression of operand k (strided kind).
 static std::string operand_address(const Operands& ops, int k) {
-  if (ops.nc && ops.pax[k] >= 0)
-    return "(*reinterpret_cast<const char* const*>(p" + std::to_string(k) + " + o" + std::to_string(k) + " + (i" +
-           std::to_string(k) + " + po" + std::to_string(k) + ") * (nca_i64)sizeof(void*)) + ro" + std::to_string(k) +
-           " + r" + std::to_string(k) + ")";
-  return "(p" + std::to_string(k) + " + o" + std::to_string(k) + ")";
+  const std::string ks = std::to_string(k);
+  if (ops.nc && !ops.ind[k].empty()) return chain_address("p" + ks, "o" + ks, "q" + ks + "_", ops.ind[k].size());
+  return "(p" + ks + " + o" + ks + ")";
 }
 
 struct ReduceSpec {
@@ -458,9 +519,9 @@ std::string gen_elementwise(const Tape& t, const Operands& ops, bool strided, in
     if (t.uses_idx) os << "      const nca_i64 lin = idx;\n";
     os << body;
     os << "      const " << D << " st = nca_store_cvt<" << D << ", " << C << ">::to(" << roots[0] << ");\n";
-    if (ops.nc && ops.paxo >= 0)
-      os << "      if (outp) *reinterpret_cast<" << D << "*>(*reinterpret_cast<char* const*>(outp + oo + (io + poo) * "
-         << "(nca_i64)sizeof(void*)) + roo + rpo) = st;\n";
+    if (ops.nc && !ops.indo.empty())
+      os << "      if (outp) *reinterpret_cast<" << D << "*>" << chain_address("outp", "oo", "qo_", ops.indo.size())
+         << " = st;\n";
     else
       os << "      if (outp) *reinterpret_cast<" << D << "*>(outp + oo) = st;\n";
     os << rc.update;
@@ -482,7 +543,7 @@ int grid_blocks(int64_t work_items, int threads, int max_blocks) {
 struct LaunchArgs {
   std::deque<const void*> ptrs;             // storage for pointer params (stable addresses)
   std::deque<std::array<int64_t, NCA_MAX_NDIM>> dims;
-  std::deque<int64_t> scal;                 // NC row-table and row offsets
+  std::deque<int64_t> scal;                 // byte offsets after the pointer reads of indirect views
   int64_t n = 0;
   std::vector<void*> params;
 };
@@ -491,11 +552,11 @@ static void push_common(LaunchArgs& la, const Tape& t, const Operands& ops, cons
                         const nca_view& out, bool strided, bool with_out, ScalarSlots& sc) {
   la.params.clear();
   for (int k : ops.unique) {
-    la.ptrs.push_back(ins[k].data);
+    la.ptrs.push_back(ops.base[k]);
     la.params.push_back(&la.ptrs.back());
   }
   if (with_out) {
-    la.ptrs.push_back(out.data);
+    la.ptrs.push_back(ops.baseo);
     la.params.push_back(&la.ptrs.back());
   }
   if (strided) {
@@ -518,18 +579,16 @@ static void push_common(LaunchArgs& la, const Tape& t, const Operands& ops, cons
     }
     if (ops.nc) {
       for (int k : ops.unique)
-        if (!ops.bcast[k] && ops.pax[k] >= 0) {
-          la.scal.push_back(ins[k].ptr_offset);
-          la.params.push_back(&la.scal.back());
-          la.scal.push_back(ins[k].row_offset);
+        if (!ops.bcast[k])
+          for (int64_t q : ops.isub[k]) {
+            la.scal.push_back(q);
+            la.params.push_back(&la.scal.back());
+          }
+      if (with_out)
+        for (int64_t q : ops.isubo) {
+          la.scal.push_back(q);
           la.params.push_back(&la.scal.back());
         }
-      if (with_out && ops.paxo >= 0) {
-        la.scal.push_back(out.ptr_offset);
-        la.params.push_back(&la.scal.back());
-        la.scal.push_back(out.row_offset);
-        la.params.push_back(&la.scal.back());
-      }
     }
   }
   la.n = ops.n;

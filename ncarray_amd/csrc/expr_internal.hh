@@ -39,11 +39,19 @@ struct Operands {
   std::vector<int64_t> sshape;
   std::vector<int64_t> sout;
   std::vector<std::vector<int64_t>> sstrides;
-  // NC pointer-axis views (strided kind only, axes not coalesced): the pointer axis of each
-  // operand and of the output, -1 for plain strided views
+  // Start addresses passed to the kernel: the view's data, or for an indirect view the start
+  // of its dereference chain.
+  std::vector<const void*> base;
+  const void* baseo = nullptr;
+  // Indirect views (strided kind only, axes not coalesced) in the form of SOArrayPolicy::advance
+  // (layout.hh:677-700): after the stride step of axis ind[k][j] the address becomes the
+  // pointer stored there plus isub[k][j] bytes.  NC pointer-axis views enter rewritten to that
+  // form (one step at the pointer axis); indo/isubo describe the output.
   bool nc = false;
-  std::vector<int> pax;
-  int paxo = -1;
+  std::vector<std::vector<int>> ind;
+  std::vector<std::vector<int64_t>> isub;
+  std::vector<int> indo;
+  std::vector<int64_t> isubo;
 };
 Operands analyse_operands(const Tape& t, const nca_view* ins, int n_in, const nca_view& out,
                           bool out_may_be_null);

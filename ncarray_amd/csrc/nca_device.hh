@@ -67,6 +67,15 @@ template <typename T, typename W> __device__ __forceinline__ void nca_st(T* p, W
   *p = nca_store_cvt<T, W>::to(v);
 }
 
+// One dereference step of an indirect view (SOArrayPolicy::advance, layout.hh:677-700): the
+// pointer stored at p, plus a byte offset.
+__device__ __forceinline__ const char* nca_chase(const char* p, nca_i64 sub) {
+  return *reinterpret_cast<const char* const*>(p) + sub;
+}
+__device__ __forceinline__ char* nca_chase(char* p, nca_i64 sub) {
+  return *reinterpret_cast<char* const*>(p) + sub;
+}
+
 // ---- numeric limits ------------------------------------------------------------------
 template <typename T> struct nca_lim;
 template <> struct nca_lim<bool> { __device__ static bool lowest() { return false; } __device__ static bool max() { return true; } };

@@ -104,8 +104,11 @@ struct ViewDesc {
 };
 int64_t view_size(const nca_view& v);
 void validate_view(const nca_view& v, const char* what, bool allow_null_data = false);
-// As validate_view, but also accepts NC pointer-axis views (expression operands and outputs).
+// As validate_view, but also accepts indirect views (expression operands and outputs).
 void validate_view_nc(const nca_view& v, const char* what, bool allow_null_data = false);
+// True when reaching an element reads a stored pointer: an NC pointer axis, or an SO axis
+// whose suboffset is >= 0.
+bool view_indirect(const nca_view& v);
 bool is_c_contiguous(const nca_view& v);
 bool same_shape(const nca_view& a, const nca_view& b);
 // True when every stride is zero (a broadcast scalar).

@@ -151,14 +151,27 @@ int64_t view_size(const nca_view& v) {
   return n;
 }
 
+bool view_indirect(const nca_view& v) {
+  if (v.pointer_axis >= 0) return true;
+  if (v.layout != NCA_LAYOUT_SO) return false;
+  for (int d = 0; d < v.ndim && d < NCA_MAX_NDIM; ++d)
+    if (v.suboffsets[d] >= 0) return true;
+  return false;
+}
+
 void validate_view_nc(const nca_view& v, const char* what, bool allow_null_data) {
-  if (v.pointer_axis < 0) {
+  if (!view_indirect(v)) {
     validate_view(v, what, allow_null_data);
     return;
   }
   nca_view plain = v;
   plain.pointer_axis = -1;
+  plain.layout = NCA_LAYOUT_NC;
   validate_view(plain, what, allow_null_data);
+  if (v.layout == NCA_LAYOUT_SO) {
+    NCA_REQUIRE(v.pointer_axis < 0, std::string(what) + ": an SO view has no NC pointer axis (pointer_axis must be -1)");
+    return;
+  }
   NCA_REQUIRE(v.pointer_axis < v.ndim, std::string(what) + ": pointer axis out of range");
   NCA_REQUIRE(v.ptr_offset >= 0, std::string(what) + ": negative pointer offset");
   const int64_t n = v.shape[v.pointer_axis];
@@ -168,6 +181,8 @@ void validate_view_nc(const nca_view& v, const char* what, bool allow_null_data)
 
 void validate_view(const nca_view& v, const char* what, bool allow_null_data) {
   NCA_REQUIRE(v.ndim >= 0 && v.ndim <= NCA_MAX_NDIM, std::string(what) + ": ndim out of range");
+  NCA_REQUIRE(v.layout == NCA_LAYOUT_NC || v.layout == NCA_LAYOUT_SO,
+              std::string(what) + ": unknown layout " + std::to_string(v.layout));
   if (!is_supported(v.dtype))
     NCA_THROW(NCA_ERR_UNSUPPORTED,
               std::string(what) + ": dtype " +
@@ -179,12 +194,16 @@ void validate_view(const nca_view& v, const char* what, bool allow_null_data) {
     NCA_THROW(NCA_ERR_UNSUPPORTED, std::string(what) +
                                        ": this entry point does not take NC pointer-axis views (the expression "
                                        "entry points do)");
+  if (view_indirect(v))
+    NCA_THROW(NCA_ERR_UNSUPPORTED, std::string(what) +
+                                       ": this entry point does not take SO views with suboffsets (the expression "
+                                       "entry points do)");
   if (!allow_null_data && view_size(v) > 0)
     NCA_REQUIRE(v.data != nullptr, std::string(what) + ": null data pointer");
 }
 
 bool is_c_contiguous(const nca_view& v) {
-  if (v.pointer_axis >= 0) return false;
+  if (view_indirect(v)) return false;
   int64_t expect = itemsize(v.dtype);
   for (int d = v.ndim - 1; d >= 0; --d) {
     if (v.shape[d] == 1) continue;

@@ -162,6 +162,8 @@ nca_view broadcast_view(const nca_view& v, const std::vector<int64_t>& shape) {
   // stride 0) repeats its one row
   if (v.pointer_axis >= 0) r.pointer_axis = v.pointer_axis + lead;
   for (int d = n - 1; d >= 0; --d) {
+    // SO suboffsets move with their axes; added axes are direct
+    r.suboffsets[d] = d < lead ? -1 : v.suboffsets[d - lead];
     if (d < lead) {
       r.shape[d] = shape[d];
       r.strides[d] = 0;

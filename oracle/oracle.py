@@ -219,3 +219,75 @@ def ulp_diff_f32(a, b):
     ia = np.where(ia < 0, -(ia & 0x7FFFFFFF), ia)
     ib = np.where(ib < 0, -(ib & 0x7FFFFFFF), ib)
     return np.abs(ia - ib)
+
+
+def so_offsets(image, image_addr, base_addr, shape, strides, suboffsets):
+    """Byte offsets into `image` of every element of an SO view, in C order.
+
+    Restates SOArrayPolicy::advance (layout.hh:677-700) over a byte image of device memory:
+    starting from base_addr, for each axis d the address moves by i_d * strides[d]; when
+    suboffsets[d] >= 0 it is then replaced by the 8-byte pointer stored there plus
+    suboffsets[d].  image_addr is the device address of image[0]; every address the view
+    reaches must fall inside the image.
+    """
+    img = np.frombuffer(np.ascontiguousarray(image).tobytes(), dtype=np.uint8)
+    off = np.array(int(base_addr) - int(image_addr), dtype=np.int64)
+    for d, n in enumerate(shape):
+        off = off[..., None] + np.arange(n, dtype=np.int64) * int(strides[d])
+        so = int(suboffsets[d]) if d < len(suboffsets) else -1
+        if so >= 0:
+            assert off.size == 0 or (off.min() >= 0 and off.max() + 8 <= img.size), "pointer read outside the image"
+            ptr = img[off[..., None] + np.arange(8)].copy().view("<u8")[..., 0].astype(np.int64)
+            off = ptr - int(image_addr) + so
+    return off.reshape(tuple(shape))
+
+
+def so_gather(image, image_addr, base_addr, dtype, shape, strides, suboffsets):
+    """The elements of an SO view (so_offsets) as a C-ordered numpy array."""
+    dt = np.dtype(dtype)
+    img = np.frombuffer(np.ascontiguousarray(image).tobytes(), dtype=np.uint8)
+    off = so_offsets(img, image_addr, base_addr, shape, strides, suboffsets)
+    assert off.size == 0 or (off.min() >= 0 and off.max() + dt.itemsize <= img.size), "element outside the image"
+    raw = img[off[..., None] + np.arange(dt.itemsize)].copy()
+    return raw.view(dt)[..., 0].reshape(tuple(shape))
+
+
+def so_scatter(image, image_addr, base_addr, values, strides, suboffsets):
+    """A copy of `image` with `values` stored at the elements of the SO view of values.shape
+    (so_offsets), in C order (a later element overwrites an earlier one at the same address)."""
+    vals = np.ascontiguousarray(values)
+    img = np.frombuffer(np.ascontiguousarray(image).tobytes(), dtype=np.uint8).copy()
+    off = so_offsets(img, image_addr, base_addr, vals.shape, strides, suboffsets).ravel()
+    raw = vals.reshape(-1, 1).view(np.uint8).reshape(off.size, vals.dtype.itemsize)
+    for i in range(off.size):
+        img[off[i]:off[i] + vals.dtype.itemsize] = raw[i]
+    return img
+
+
+class ByteImage:
+    """A byte image of one device allocation at address `addr`, for building indirect-view test
+    inputs: chunks are placed at aligned offsets and pointers are written as addr + offset, so
+    the image copied to that allocation is valid device memory for the views built over it."""
+
+    def __init__(self, size, addr):
+        self.buf = np.zeros(int(size), np.uint8)
+        self.addr = int(addr)
+        self.top = 0
+
+    def alloc(self, nbytes, align=16):
+        off = (self.top + align - 1) // align * align
+        assert off + nbytes <= self.buf.size, "image too small"
+        self.top = off + int(nbytes)
+        return off
+
+    def put(self, arr, align=16):
+        """Copy `arr` (C order) into the image; returns its offset."""
+        raw = np.ascontiguousarray(arr).reshape(-1).view(np.uint8)
+        off = self.alloc(raw.size, align)
+        self.buf[off:off + raw.size] = raw
+        return off
+
+    def put_ptrs(self, offsets, deltas=0):
+        """A table of 8-byte pointers to image offsets (plus `deltas` bytes each); returns its offset."""
+        vals = np.asarray(offsets, np.int64) + np.asarray(deltas, np.int64) + self.addr
+        return self.put(vals.astype("<u8"), align=8)

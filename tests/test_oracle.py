@@ -186,3 +186,69 @@ def test_bf16_rounding_matches_torch():
     x[:4] = [np.inf, -np.inf, 0.0, -0.0]
     ref = torch.from_numpy(x).to(torch.bfloat16).float().numpy()
     assert np.array_equal(O.bf16_round(x), ref)
+
+
+# ---- SO views (SOArrayPolicy::advance, layout.hh:677-700) ------------------------------------------
+def test_so_offsets_without_suboffsets_is_as_strided():
+    rng = np.random.default_rng(30)
+    x = rng.standard_normal((5, 6, 7)).astype(np.float32)
+    im = O.ByteImage(4096, 0x7F00_0000_0000)
+    off = im.put(x)
+    base = im.addr + off
+    # a transposed, stepped view: element (i, j) = x[2*j, 1, i]
+    st = (4, 2 * 6 * 7 * 4)
+    got = O.so_gather(im.buf, im.addr, base + 1 * 7 * 4, np.float32, (7, 3), st, (-1, -1))
+    assert np.array_equal(got, x[0::2, 1, :].T)
+    want = np.lib.stride_tricks.as_strided(x[0, 1], shape=(7, 3), strides=st)
+    assert np.array_equal(got, want)
+
+
+def test_so_one_indirect_axis_is_a_row_table():
+    """Suboffset at axis 0: rows reached through a pointer table, each read `so` bytes in
+    (the NC pointer-axis form with row_offset = so)."""
+    rng = np.random.default_rng(31)
+    rows = [rng.standard_normal(9).astype(np.float32) for _ in range(4)]
+    im = O.ByteImage(4096, 0x10000)
+    row_off = [im.put(r) for r in reversed(rows)][::-1]
+    perm = [2, 0, 3, 3, 1]
+    tbl = im.put_ptrs([row_off[p] for p in perm])
+    got = O.so_gather(im.buf, im.addr, im.addr + tbl, np.float32, (5, 7), (8, 4), (8, -1))
+    assert np.array_equal(got, np.stack([rows[p][2:] for p in perm]))
+
+
+def test_so_two_level_indirection_known_answer():
+    """A table of tables of rows; element (i, j, k) = 100 i + 10 j + k by construction, with the
+    chunks placed in a shuffled order so that a mixed-up index or offset reads another value."""
+    R, H, W = 3, 4, 6
+    im = O.ByteImage(8192, 0x4000)
+    order = np.random.default_rng(32).permutation(R * H)
+    row_at = {}
+    for r in order:
+        i, j = divmod(int(r), H)
+        row = np.concatenate([[-1.0, -2.0], 100 * i + 10 * j + np.arange(W)]).astype(np.float64)
+        row_at[(i, j)] = im.put(row)
+    sub = [im.put_ptrs([row_at[(i, j)] for j in range(H)], deltas=-24) for i in range(R)]
+    top = im.put_ptrs(sub, deltas=-40)
+    got = O.so_gather(im.buf, im.addr, im.addr + top, np.float64, (R, H, W), (8, 8, 8), (40, 24 + 16, -1))
+    i, j, k = np.meshgrid(np.arange(R), np.arange(H), np.arange(W), indexing="ij")
+    assert np.array_equal(got, 100 * i + 10 * j + k)
+    # the innermost axis indirect as well: each element through its own pointer
+    flat = im.put(np.arange(10, dtype=np.int32) * 7)
+    ptrs = im.put_ptrs([flat + 4 * (9 - t) for t in range(10)], deltas=-4)
+    got = O.so_gather(im.buf, im.addr, im.addr + ptrs, np.int32, (10,), (8,), (4,))
+    assert np.array_equal(got, np.arange(9, -1, -1) * 7)
+
+
+def test_so_scatter_round_trip_touches_only_the_view():
+    R, W = 3, 5
+    im = O.ByteImage(2048, 0x9000)
+    rows = [im.put(np.full(W + 2, -9.0, np.float32)) for _ in range(R)]
+    tbl = im.put_ptrs(rows)
+    vals = np.arange(R * W, dtype=np.float32).reshape(R, W)
+    out = O.so_scatter(im.buf, im.addr, im.addr + tbl, vals, (8, 4), (4, -1))
+    assert np.array_equal(O.so_gather(out, im.addr, im.addr + tbl, np.float32, (R, W), (8, 4), (4, -1)), vals)
+    changed = np.flatnonzero(out != im.buf)
+    assert changed.size <= R * W * 4
+    for r in range(R):
+        assert np.array_equal(out[rows[r]:rows[r] + 4].view(np.float32), [-9.0])
+        assert np.array_equal(out[rows[r] + 4 * (W + 1):rows[r] + 4 * (W + 2)].view(np.float32), [-9.0])
