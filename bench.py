@@ -406,6 +406,18 @@ def bench_c3(torch, capi, tapes, ti, dev, sptr, time_loop):
         recs = capi.profile_read()
         capi.profile_enable(False)
         kms = float(sum(ms for _, ms in recs)) / 5
+        # the same for one step of laplacian_c0_value_and_grad (loss and gradients in one march
+        # pass, then the transposed stencil)
+        xd, yd = x.detach(), y.detach()
+        A.laplacian_c0_value_and_grad(xd, yd)
+        torch.cuda.synchronize()
+        capi.profile_enable(True)
+        for _ in range(5):
+            vg = A.laplacian_c0_value_and_grad(xd, yd)
+        torch.cuda.synchronize()
+        vrecs = capi.profile_read()
+        capi.profile_enable(False)
+        vms = float(sum(ms for _, ms in vrecs)) / 5
         with torch.no_grad():
             loss_eager = A.laplacian_c0_loss(x, y).clone()
         gfm, _ = graph_loop(torch, fwd, 10, 10)  # forward only (launched from this thread)
@@ -426,13 +438,18 @@ def bench_c3(torch, capi, tapes, ti, dev, sptr, time_loop):
                     "device_fwd_bwd_ms": round(kms, 4), "device_fwd_bwd_GBps": gbps(fb_bytes, kms),
                     "device_fwd_bwd_frac_of_8TBps": frac(fb_bytes, kms),
                     "device_fwd_bwd_kernels": sorted({n for n, _ in recs}),
-                    "loss": float(loss_eager)}
+                    "device_value_and_grad_ms": round(vms, 4), "device_value_and_grad_GBps": gbps(fb_bytes, vms),
+                    "device_value_and_grad_frac_of_8TBps": frac(fb_bytes, vms),
+                    "device_value_and_grad_kernels": sorted({n for n, _ in vrecs}),
+                    "loss": float(loss_eager), "loss_value_and_grad": float(vg[0])}
     res["note"] = ("bytes are algorithmic (each input read once, each output written once); the backward is "
                    "one march pass (the Laplacian recomputed, both gradients of the C0 expression written) and "
                    "the transposed stencil; fwd_bwd_ms is host-bound (torch autograd and Python per step); "
                    "graph_fwd replays a hipGraph of 10 forwards (no Python launch overhead), graph_fwd_bwd one of 10 "
-                   "forward+backward steps (laplacian_c0_value_and_grad); device_fwd_bwd "
-                   "sums the step's kernel durations (HIP events per launch); 134 MB working set: "
+                   "forward+backward steps (laplacian_c0_value_and_grad: one march pass computes the loss and both "
+                   "gradients of the C0 expression, then the transposed stencil); device_fwd_bwd sums the kernel "
+                   "durations of a torch-autograd step (forward kernel, then the backward), device_value_and_grad "
+                   "those of a laplacian_c0_value_and_grad step (HIP events per launch); 134 MB working set: "
                    "cache-resident")
     return res
 

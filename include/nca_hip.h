@@ -320,6 +320,20 @@ int nca_hip_stencil_expr_vjp(uint64_t stencil_handle, uint64_t expr_handle,
                              const nca_scalar* expr_scalars, int32_t n_expr_scalars,
                              const void* grad_scalar, const nca_view* w,
                              const nca_view* grad_extras, int32_t boundary, void* stream);
+/* nca_hip_stencil_expr_vjp that in the same pass also reduces the expression's value
+ * expr(stencil(src), extras...) (the loss itself; no counterpart in the reference): fuse
+ * (NCA_FUSE_*, not 0) and accum_mode as in nca_hip_stencil_apply_reduce, results in the
+ * device scalars sum_out / max_out / min_out typed as a reduction of the expression's
+ * dest dtype.  One pass then yields the loss and the gradients of a training step. */
+int nca_hip_stencil_expr_vjp_reduce(uint64_t stencil_handle, uint64_t expr_handle,
+                                    const int32_t* expr_operand, int32_t n_operand,
+                                    const nca_view* src, const nca_view* extras, int32_t n_extra,
+                                    const nca_scalar* stencil_scalars, int32_t n_stencil_scalars,
+                                    const nca_scalar* expr_scalars, int32_t n_expr_scalars,
+                                    const void* grad_scalar, const nca_view* w,
+                                    const nca_view* grad_extras, int32_t fuse, int32_t accum_mode,
+                                    void* sum_out, void* max_out, void* min_out, int32_t boundary,
+                                    void* stream);
 int nca_hip_stencil_source(uint64_t handle, int32_t variant, char* buf, size_t n);
 
 /* ================================================================================ */

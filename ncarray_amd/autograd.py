@@ -304,16 +304,34 @@ def laplacian_c0_loss(x, y, compensated=True):
 
 
 def laplacian_c0_value_and_grad(x, y, compensated=True, grad_scale=None):
-    """(loss, dx, dy) of loss = sum(C0(Laplacian(x), y)) without torch's autograd: the forward
-    kernel and the two-kernel backward, all launched from the calling thread on its current
-    stream, so a whole training step can be captured in one hipGraph.  grad_scale: optional 0-d
-    float32 device tensor scaling the gradients (the upstream gradient; default 1)."""
+    """(loss, dx, dy) of loss = sum(C0(Laplacian(x), y)) without torch's autograd, all launched
+    from the calling thread on its current stream, so a whole training step can be captured in
+    one hipGraph.  Where the march kernel applies, one pass computes the loss (the forward's
+    accumulation: per-thread float32, Kahan when compensated, float64 across blocks) together
+    with w = dC0/da * g and dy, and the transposed Laplacian of w gives dx; otherwise the fused
+    forward kernel and laplacian_c0_backward run.  grad_scale: optional 0-d float32 device
+    tensor scaling the gradients (the upstream gradient; default 1)."""
     _init(x)
     xc, yc = x.detach().contiguous(), y.detach().contiguous()
-    st = fused_loss_stencil(_code(xc), _code(yc))
     loss = torch.empty((), dtype=torch.float32, device=x.device)
+    mode = capi.ACCUM_COMPENSATED if compensated else capi.ACCUM_PLAIN
+    if _march_vjp_ok(xc, yc):
+        s = _stream()
+        lap, adj = laplacian_stencil(_code(xc), capi.FLOAT32)
+        tape = c0_tape(capi.FLOAT32, _code(yc), capi.FLOAT32)
+        gf = None if grad_scale is None else (grad_scale if grad_scale.dtype == torch.float32 else grad_scale.float())
+        w = torch.empty(xc.shape, dtype=torch.float32, device=x.device)
+        dy = torch.empty_like(yc)
+        vw = ti.view(w)
+        lap.expr_vjp(tape, [-1, 0, -1, -1, 0], ti.view(xc), vw, extras=[ti.view(yc)], grad_extras=[ti.view(dy)],
+                     scalars=(6.0,), expr_scalars=(0.5,), grad_scalar=None if gf is None else gf.data_ptr(),
+                     stream=s, fuse=capi.FUSE_SUM, mode=mode, sum_out=loss.data_ptr())
+        dx = torch.empty(xc.shape, dtype=xc.dtype, device=x.device)
+        adj.apply(vw, ti.view(dx), capi.BOUNDARY_ZERO, scalars=(6.0,), stream=s)
+        return loss, dx, dy
+    st = fused_loss_stencil(_code(xc), _code(yc))
     st.apply_reduce(ti.view(xc), ti.shape_view(list(xc.shape), capi.FLOAT32), capi.BOUNDARY_ZERO,
-                    fuse=capi.FUSE_SUM, mode=capi.ACCUM_COMPENSATED if compensated else capi.ACCUM_PLAIN,
-                    sum_out=loss.data_ptr(), extras=[ti.view(yc)], scalars=(6.0, 0.5), stream=_stream())
+                    fuse=capi.FUSE_SUM, mode=mode, sum_out=loss.data_ptr(), extras=[ti.view(yc)],
+                    scalars=(6.0, 0.5), stream=_stream())
     dx, dy = laplacian_c0_backward(xc, yc, grad_scale)
     return loss, dx, dy

@@ -128,6 +128,8 @@ _SIGS = {
     "nca_hip_stencil_adjoint_create": [_U64, ctypes.POINTER(_U64)],
     "nca_hip_stencil_expr_vjp": [_U64, _U64, ctypes.POINTER(_I32), _I32, _VP, _VP, _I32, _SP, _I32, _SP, _I32, _P, _VP,
                                  _VP, _I32, _P],
+    "nca_hip_stencil_expr_vjp_reduce": [_U64, _U64, ctypes.POINTER(_I32), _I32, _VP, _VP, _I32, _SP, _I32, _SP, _I32,
+                                        _P, _VP, _VP, _I32, _I32, _P, _P, _P, _I32, _P],
     "nca_hip_stencil_source": [_U64, _I32, ctypes.c_char_p, _SZ],
     "nca_hip_scatter_add": [_VP, _VP, _VP, _P],
     "nca_hip_bin": [_VP, _VP, _I32, _I32, _VP, _VP, _P],
@@ -347,19 +349,27 @@ class Stencil:
                        self.scalar_dtypes, handle=h.value)
 
     def expr_vjp(self, tape, operands, src, w, extras=(), grad_extras=(), scalars=(), expr_scalars=(),
-                 grad_scalar=None, boundary=BOUNDARY_ZERO, stream=None):
+                 grad_scalar=None, boundary=BOUNDARY_ZERO, stream=None, fuse=0, mode=ACCUM_PLAIN, sum_out=None,
+                 max_out=None, min_out=None):
         """Backward of sum(tape(stencil(src), extras)): writes w = d tape / d a * g and the extras'
         gradients (grad_extras[j] None: not computed) in one pass; `operands[k]` is -1 when tape
         operand k reads the stencil value, j when it reads extras[j]; g = *grad_scalar (device
-        pointer, None: 1)."""
+        pointer, None: 1).  fuse != 0 also reduces the tape's value in the same pass into the
+        device scalars sum_out / max_out / min_out."""
         ops = (_I32 * max(1, len(operands)))(*operands)
         ea = view_array(extras)
         ge = [v if v is not None else View() for v in grad_extras] + [View()] * (len(extras) - len(grad_extras))
         ga = view_array(ge)
         sa = scalar_array(scalars, self.scalar_dtypes)
         xa = scalar_array(expr_scalars, tape.scalar_dtypes)
-        call("nca_hip_stencil_expr_vjp", self.handle, tape.handle, ops, len(operands), ctypes.byref(src), ea,
-             len(extras), sa, len(scalars), xa, len(expr_scalars), grad_scalar, ctypes.byref(w), ga, boundary, stream)
+        if fuse:
+            call("nca_hip_stencil_expr_vjp_reduce", self.handle, tape.handle, ops, len(operands), ctypes.byref(src),
+                 ea, len(extras), sa, len(scalars), xa, len(expr_scalars), grad_scalar, ctypes.byref(w), ga, fuse, mode,
+                 sum_out, max_out, min_out, boundary, stream)
+        else:
+            call("nca_hip_stencil_expr_vjp", self.handle, tape.handle, ops, len(operands), ctypes.byref(src), ea,
+                 len(extras), sa, len(scalars), xa, len(expr_scalars), grad_scalar, ctypes.byref(w), ga, boundary,
+                 stream)
 
     def source(self, variant=0):
         n = lib.nca_hip_stencil_source(self.handle, variant, None, 0)

@@ -438,6 +438,8 @@ struct MarchVjp {
   std::vector<int> roots;           // derivative roots, one per output (-1: exactly zero)
   std::vector<int32_t> out_dtypes;  // output m's dtype (output 0: d/da; then the extras asked for)
   std::vector<int> alias;           // expression operand k -> first operand reading the same value
+  int32_t value_dtype = NCA_FLOAT32; // the expression's dest dtype: the type of the values that fused
+                                     // reductions (StencilFuse::fuse) sum, in the forward's way
 };
 
 static std::string gen_march_q(const StencilProgram& p, const StencilFuse& fz, int32_t dst_dtype, const MarchPlan& mp,
@@ -453,12 +455,16 @@ static std::string gen_march_q(const StencilProgram& p, const StencilFuse& fz, i
   std::vector<std::string> roots;
   std::string body = emit_nodes(t, {t.root}, alias, names_u(n_arr), names_s(static_cast<int>(t.scalar_dtypes.size())),
                                 "lin", "t", roots);
-  const int32_t rcd = compute_dtype(dst_dtype);
+  // values that fused reductions read: the stencil's output (dst dtype), or in vjp mode the
+  // expression's own value (its dest dtype), held in res / resp
+  const int32_t vdt = vj ? vj->value_dtype : dst_dtype;
+  const std::string VD = ctype_of(vdt);
+  const int32_t rcd = compute_dtype(vdt);
   const std::string RC = ctype_of(rcd);
   const bool comp = fz.mode == NCA_ACCUM_COMPENSATED;
   // vjp mode: the expression's derivative code, its loads renamed to the stencil value / extras
   const std::vector<int32_t>& xdt = vj ? vj->extra_dtypes : p.extra_dtypes;
-  std::string vbody;
+  std::string vbody, pname;
   std::vector<std::string> vnames;
   if (vj) {
     std::vector<std::string> en(vj->operand.size()), rn(vj->expr.scalar_dtypes.size());
@@ -468,11 +474,14 @@ static std::string gen_march_q(const StencilProgram& p, const StencilFuse& fz, i
     std::vector<int> live;
     for (int r : vj->roots)
       if (r >= 0) live.push_back(r);
+    if (fz.fuse) live.push_back(vj->expr.root);
     std::vector<std::string> ln;
     vbody = emit_nodes(vj->expr, live, vj->alias, en, rn, "lin", "q", ln);
     size_t li = 0;
     for (int r : vj->roots) vnames.push_back(r >= 0 ? ln[li++] : "(" + C + ")0");
+    if (fz.fuse) pname = ln[li];
   }
+  const std::string rsrc = vj ? "resp" : "res";
   const int VX = mp.vx, RPT = mp.rpt, RZ = g.rz, RY = g.ry, RX = g.rx;
   const int Q = 2 * RZ + 2;
   const int TXT = mp.txt, TYT = 256 / mp.txt;
@@ -481,12 +490,14 @@ static std::string gen_march_q(const StencilProgram& p, const StencilFuse& fz, i
   std::ostringstream os;
   os << "#define VX " << VX << "\n#define TX " << TX << "\n#define TY " << TY << "\n#define LH " << LH << "\n#define LW " << LW
      << "\n#define PADL " << PADL << "\n#define RY " << RY << "\n#define RX " << RX << "\n";
-  const std::string kname = vj ? std::string("nca_stencil_march_vjp_") + dtype_info(p.src_dtype).name : march_name(p, fz, dst_dtype);
+  const std::string kname = vj ? std::string("nca_stencil_march_vjp_") + dtype_info(p.src_dtype).name + (fz.fuse ? "_sum" : "")
+                               : march_name(p, fz, dst_dtype);
   os << "extern \"C\" __global__ __launch_bounds__(256) void " << kname << "(const " << S << "* __restrict__ src";
   for (size_t j = 0; j < xdt.size(); ++j) os << ", const " << ctype_of(xdt[j]) << "* __restrict__ e" << j;
   if (vj) {
     for (size_t m = 0; m < vj->out_dtypes.size(); ++m) os << ", " << ctype_of(vj->out_dtypes[m]) << "* __restrict__ o" << m;
     os << ", nca_i64 NZ, nca_i64 NY, nca_i64 NX, int ntx, int nty, int zlen";
+    if (fz.fuse) os << ", void* __restrict__ psum, void* __restrict__ pmax, void* __restrict__ pmin";
   } else {
     os << ", " << D << "* __restrict__ dst, nca_i64 NZ, nca_i64 NY, nca_i64 NX, int ntx, int nty, int zlen"
        << ", double* __restrict__ colpart, void* __restrict__ colout, void* __restrict__ psum, void* __restrict__ pmax"
@@ -644,10 +655,11 @@ static std::string gen_march_q(const StencilProgram& p, const StencilFuse& fz, i
            << "      if (valid" << r << ") X" << jx << " = *reinterpret_cast<const nca_vec<" << E << ", VX>*>(e" << jx
            << " + soff + ioff" << r << ");\n";
       }
-      if (vj)
+      if (vj) {
         for (size_t m = 0; m < vj->out_dtypes.size(); ++m)
           os << "      nca_vec<" << ctype_of(vj->out_dtypes[m]) << ", VX> res" << m << ";\n";
-      else
+        if (fz.fuse) os << "      nca_vec<" << VD << ", VX> resp;\n";
+      } else
         os << "      nca_vec<" << D << ", VX> res;\n";
       for (int j = 0; j < VX; ++j) {
         os << "      {\n";
@@ -676,6 +688,7 @@ static std::string gen_march_q(const StencilProgram& p, const StencilFuse& fz, i
           for (size_t m = 0; m < vj->out_dtypes.size(); ++m)
             os << "        res" << m << ".v[" << j << "] = nca_store_cvt<" << ctype_of(vj->out_dtypes[m]) << ", " << C
                << ">::to((" << C << ")(gsc * " << vnames[m] << "));\n";
+          if (fz.fuse) os << "        resp.v[" << j << "] = nca_store_cvt<" << VD << ", " << C << ">::to(" << pname << ");\n";
         } else {
           os << "        res.v[" << j << "] = nca_store_cvt<" << D << ", " << C << ">::to(" << roots[0] << ");\n";
         }
@@ -688,7 +701,8 @@ static std::string gen_march_q(const StencilProgram& p, const StencilFuse& fz, i
       if (fz.axis0 || fz.fuse) {
         os << "      if (valid" << r << ") {\n";
         for (int j = 0; j < VX; ++j)
-          os << "        const " << RC << " rv" << j << " = nca_cvt<" << RC << ", " << D << ">::from(res.v[" << j << "]);\n";
+          os << "        const " << RC << " rv" << j << " = nca_cvt<" << RC << ", " << VD << ">::from(" << rsrc << ".v[" << j
+             << "]);\n";
         if (fz.axis0)
           for (int j = 0; j < VX; ++j) {
             if (comp) os << "        nca_kahan_add(cs[" << r << "][" << j << "], cc[" << r << "][" << j << "], (float)rv" << j << ");\n";
@@ -973,7 +987,22 @@ static void stencil_apply_impl(uint64_t handle, const nca_view* src, const nca_v
                 "axis0_sum must be C-contiguous with shape dst.shape[1:]");
     NCA_REQUIRE(axis0->dtype == NCA_FLOAT32 || axis0->dtype == NCA_FLOAT64, "axis0_sum dtype must be float32 or float64");
   }
-  if (n == 0) return;
+  if (n == 0) {
+    // empty: the reductions' identities (zero sums and column sums, the extrema's identities)
+    if (fuse || axis0) {
+      FinalizeArgs fa;
+      fa.K = axis0 ? view_size(*axis0) : 0;
+      fa.colout = axis0 ? axis0->data : nullptr;
+      fa.col_dtype = axis0 ? axis0->dtype : NCA_FLOAT64;
+      fa.value_dtype = dst->dtype;
+      fa.fuse = fuse;
+      fa.sum_out = sum_out;
+      fa.max_out = max_out;
+      fa.min_out = min_out;
+      launch_finalize_fused(fa, s);
+    }
+    return;
+  }
   if (march) {
     const MarchPlan& mp = mplan;
     const int64_t NZ = dst->shape[0], NY = dst->shape[1], NX = dst->shape[2];
@@ -1187,12 +1216,18 @@ int nca_hip_stencil_apply_reduce(uint64_t handle, const nca_view* src, const nca
   });
 }
 
-int nca_hip_stencil_expr_vjp(uint64_t stencil_handle, uint64_t expr_handle, const int32_t* expr_operand,
-                             int32_t n_operand, const nca_view* src, const nca_view* extras, int32_t n_extra,
-                             const nca_scalar* stencil_scalars, int32_t n_stencil_scalars, const nca_scalar* expr_scalars,
-                             int32_t n_expr_scalars, const void* grad_scalar, const nca_view* w,
-                             const nca_view* grad_extras, int32_t boundary, void* stream) {
-  return api_call([&] {
+}  // extern "C"
+
+namespace nca {
+// One march pass of the stencil-expression vjp; with fuse != 0 the pass also reduces the
+// expression's value (sum / max / min as the fused forward reductions do).
+static void stencil_expr_vjp_impl(uint64_t stencil_handle, uint64_t expr_handle, const int32_t* expr_operand,
+                                  int32_t n_operand, const nca_view* src, const nca_view* extras, int32_t n_extra,
+                                  const nca_scalar* stencil_scalars, int32_t n_stencil_scalars,
+                                  const nca_scalar* expr_scalars, int32_t n_expr_scalars, const void* grad_scalar,
+                                  const nca_view* w, const nca_view* grad_extras, int32_t fuse, int32_t mode,
+                                  void* sum_out, void* max_out, void* min_out, int32_t boundary, void* stream) {
+  {
     StencilProgram& p = stencil_program(stencil_handle);
     const Tape& et = expr_program(expr_handle).tape;
     NCA_REQUIRE(src && w && (n_operand == 0 || expr_operand), "null argument");
@@ -1208,8 +1243,14 @@ int nca_hip_stencil_expr_vjp(uint64_t stencil_handle, uint64_t expr_handle, cons
     NCA_REQUIRE(src->ndim == 3 && same_shape(*src, *w) && is_c_contiguous(*src) && is_c_contiguous(*w),
                 "src and w must be C-contiguous 3-D arrays of one shape");
     NCA_REQUIRE(is_float(w->dtype), "w must be floating point");
+    NCA_REQUIRE((fuse & ~(NCA_FUSE_SUM | NCA_FUSE_MAX | NCA_FUSE_MIN)) == 0, "unknown fuse flags");
+    NCA_REQUIRE(mode == NCA_ACCUM_PLAIN || mode == NCA_ACCUM_COMPENSATED, "unknown accumulation mode");
+    NCA_REQUIRE((!(fuse & NCA_FUSE_SUM) || sum_out) && (!(fuse & NCA_FUSE_MAX) || max_out) &&
+                    (!(fuse & NCA_FUSE_MIN) || min_out),
+                "a fused reduction needs its output pointer");
     MarchVjp vj;
     vj.expr = et;
+    vj.value_dtype = et.dest_dtype;
     vj.operand.assign(expr_operand, expr_operand + n_operand);
     int first_a = -1;
     for (int k = 0; k < n_operand; ++k) {
@@ -1261,8 +1302,20 @@ int nca_hip_stencil_expr_vjp(uint64_t stencil_handle, uint64_t expr_handle, cons
     NCA_REQUIRE(ok, "the stencil-expression vjp needs the last extent a multiple of " + std::to_string(vx) +
                         " elements and arrays aligned to whole vectors");
     const int64_t n = view_size(*w);
-    if (n == 0) return;
     hipStream_t s = as_stream(stream);
+    if (n == 0) {
+      if (fuse) {
+        FinalizeArgs fa;
+        fa.value_dtype = et.dest_dtype;
+        fa.fuse = fuse;
+        fa.sum_out = sum_out;
+        fa.max_out = max_out;
+        fa.min_out = min_out;
+        launch_finalize_fused(fa, s);
+      }
+      finish_call(stream);
+      return;
+    }
     DeviceState& dev = current_device();
     std::string key = "vjp|" + et.signature + "|";
     for (int o : vj.operand) key += std::to_string(o) + ",";
@@ -1271,15 +1324,14 @@ int nca_hip_stencil_expr_vjp(uint64_t stencil_handle, uint64_t expr_handle, cons
     key += "|";
     for (size_t m = 0; m < vj.roots.size(); ++m) key += std::to_string(vj.out_dtypes[m]) + (vj.roots[m] < 0 ? "z," : ",");
     key += "|" + std::to_string(mp.vx) + "," + std::to_string(mp.rpt) + "," + std::to_string(mp.txt);
-    const std::string kname = std::string("nca_stencil_march_vjp_") + dtype_info(p.src_dtype).name;
-    hipFunction_t fn = stencil_kernel(
-        p, key,
-        [&]() {
-          StencilFuse fz;
-          fz.with_dst = false;
-          return gen_march_q(p, fz, w->dtype, mp, &vj);
-        },
-        kname);
+    key += "|f" + std::to_string(fuse) + "," + std::to_string(fuse ? mode : 0);
+    const std::string kname =
+        std::string("nca_stencil_march_vjp_") + dtype_info(p.src_dtype).name + (fuse ? "_sum" : "");
+    StencilFuse fz;
+    fz.with_dst = false;
+    fz.fuse = fuse;
+    fz.mode = mode;
+    hipFunction_t fn = stencil_kernel(p, key, [&]() { return gen_march_q(p, fz, w->dtype, mp, &vj); }, kname);
     const int txw = mp.txt * vx, tyw = (256 / mp.txt) * mp.rpt;
     const int ntx = static_cast<int>((NX + txw - 1) / txw), nty = static_cast<int>((NY + tyw - 1) / tyw);
     const int64_t tiles = static_cast<int64_t>(ntx) * nty;
@@ -1302,6 +1354,18 @@ int nca_hip_stencil_expr_vjp(uint64_t stencil_handle, uint64_t expr_handle, cons
     for (auto& d : dims) params.push_back(&d);
     int ints[3] = {ntx, nty, zlen};
     for (auto& i : ints) params.push_back(&i);
+    void* psum = nullptr;
+    void* pmax = nullptr;
+    void* pmin = nullptr;
+    if (fuse) {
+      Workspace& ws = workspace(s, static_cast<size_t>(blocks) * 24);
+      psum = ws.partials;
+      pmax = static_cast<char*>(ws.partials) + static_cast<size_t>(blocks) * 8;
+      pmin = static_cast<char*>(ws.partials) + static_cast<size_t>(blocks) * 16;
+      params.push_back(&psum);
+      params.push_back(&pmax);
+      params.push_back(&pmin);
+    }
     for (auto& x : ss.slots) params.push_back(&x);
     for (auto& x : es.slots) params.push_back(&x);
     ptrs.push_back(grad_scalar);
@@ -1311,7 +1375,49 @@ int nca_hip_stencil_expr_vjp(uint64_t stencil_handle, uint64_t expr_handle, cons
       jit_launch(fn, dim3(static_cast<unsigned>(blocks)), dim3(256), 0, s, params);
     }
     check_launch("stencil_expr_vjp");
+    if (fuse) {
+      FinalizeArgs fa;
+      fa.value_dtype = et.dest_dtype;
+      fa.fuse = fuse;
+      fa.psum = psum;
+      fa.pmax = pmax;
+      fa.pmin = pmin;
+      fa.nblocks = static_cast<int>(blocks);
+      fa.sum_out = sum_out;
+      fa.max_out = max_out;
+      fa.min_out = min_out;
+      launch_finalize_fused(fa, s);
+    }
     finish_call(stream);
+  }
+}
+}  // namespace nca
+
+extern "C" {
+
+int nca_hip_stencil_expr_vjp(uint64_t stencil_handle, uint64_t expr_handle, const int32_t* expr_operand,
+                             int32_t n_operand, const nca_view* src, const nca_view* extras, int32_t n_extra,
+                             const nca_scalar* stencil_scalars, int32_t n_stencil_scalars, const nca_scalar* expr_scalars,
+                             int32_t n_expr_scalars, const void* grad_scalar, const nca_view* w,
+                             const nca_view* grad_extras, int32_t boundary, void* stream) {
+  return api_call([&] {
+    stencil_expr_vjp_impl(stencil_handle, expr_handle, expr_operand, n_operand, src, extras, n_extra, stencil_scalars,
+                          n_stencil_scalars, expr_scalars, n_expr_scalars, grad_scalar, w, grad_extras, 0,
+                          NCA_ACCUM_PLAIN, nullptr, nullptr, nullptr, boundary, stream);
+  });
+}
+
+int nca_hip_stencil_expr_vjp_reduce(uint64_t stencil_handle, uint64_t expr_handle, const int32_t* expr_operand,
+                                    int32_t n_operand, const nca_view* src, const nca_view* extras, int32_t n_extra,
+                                    const nca_scalar* stencil_scalars, int32_t n_stencil_scalars,
+                                    const nca_scalar* expr_scalars, int32_t n_expr_scalars, const void* grad_scalar,
+                                    const nca_view* w, const nca_view* grad_extras, int32_t fuse, int32_t mode,
+                                    void* sum_out, void* max_out, void* min_out, int32_t boundary, void* stream) {
+  return api_call([&] {
+    NCA_REQUIRE(fuse != 0, "no reduction requested (nca_hip_stencil_expr_vjp computes the gradients alone)");
+    stencil_expr_vjp_impl(stencil_handle, expr_handle, expr_operand, n_operand, src, extras, n_extra, stencil_scalars,
+                          n_stencil_scalars, expr_scalars, n_expr_scalars, grad_scalar, w, grad_extras, fuse, mode,
+                          sum_out, max_out, min_out, boundary, stream);
   });
 }
 

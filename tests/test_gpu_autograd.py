@@ -1,5 +1,6 @@
 """Hand-written adjoints behind torch.autograd.Function: checked with gradcheck (float64),
 central finite differences of the FP64 loss (h = 1e-3) and an FP64 reference gradient."""
+import ctypes
 import numpy as np
 import pytest
 import torch
@@ -177,6 +178,98 @@ def test_stencil_expr_vjp_matches_unfused(dev, dtype):
                  scalars=(6.0,), expr_scalars=(0.5,))
     torch.cuda.synchronize()
     assert torch.equal(w1 * 2.5, ga)
+
+
+@pytest.mark.parametrize("dtype", [torch.float32, torch.bfloat16])
+@pytest.mark.parametrize("mode", [0, 1])
+def test_stencil_expr_vjp_reduce(dev, dtype, mode):
+    """nca_hip_stencil_expr_vjp_reduce: the gradients are the bits of nca_hip_stencil_expr_vjp;
+    the fused max and min of the expression's value equal those of the forward kernel's stored
+    values (exact); the fused sum, like the forward kernel's fused sum, is within the
+    accumulation's bound of the FP64 loss."""
+    A, capi, tapes = _mods()
+    from ncarray_amd import torch_interop as ti
+    g = torch.Generator(device=dev).manual_seed(12)
+    x = torch.randn((24, 20, 48), generator=g, device=dev).to(dtype)
+    y = torch.randn((24, 20, 48), generator=g, device=dev).to(dtype)
+    gs = torch.tensor(0.75, dtype=torch.float32, device=dev)
+    xc = ti.TORCH_TO_CODE[dtype]
+    lap, _ = A.laplacian_stencil(xc, capi.FLOAT32)
+    tape = A.c0_tape(capi.FLOAT32, xc, capi.FLOAT32)
+    args = dict(extras=[ti.view(y)], scalars=(6.0,), expr_scalars=(0.5,), grad_scalar=gs.data_ptr())
+    w, gy = torch.empty(x.shape, dtype=torch.float32, device=dev), torch.empty_like(y)
+    sm, mx, mn = (torch.full((), np.nan, dtype=torch.float32, device=dev) for _ in range(3))
+    lap.expr_vjp(tape, [-1, 0, -1, -1, 0], ti.view(x), ti.view(w), grad_extras=[ti.view(gy)],
+                 fuse=capi.FUSE_SUM | capi.FUSE_MAX | capi.FUSE_MIN, mode=mode, sum_out=sm.data_ptr(),
+                 max_out=mx.data_ptr(), min_out=mn.data_ptr(), **args)
+    w2, gy2 = torch.empty_like(w), torch.empty_like(gy)
+    lap.expr_vjp(tape, [-1, 0, -1, -1, 0], ti.view(x), ti.view(w2), grad_extras=[ti.view(gy2)], **args)
+    # the forward: the fused Laplacian-C0 stencil storing its values, and its own fused sum
+    fused = A.fused_loss_stencil(xc, xc)
+    val = torch.empty(x.shape, dtype=torch.float32, device=dev)
+    fsum = torch.empty((), dtype=torch.float32, device=dev)
+    fused.apply_reduce(ti.view(x), ti.view(val), capi.BOUNDARY_ZERO, fuse=capi.FUSE_SUM, mode=mode,
+                       sum_out=fsum.data_ptr(), extras=[ti.view(y)], scalars=(6.0, 0.5))
+    torch.cuda.synchronize()
+    assert torch.equal(w, w2) and torch.equal(gy, gy2)
+    assert mx.item() == val.max().item() and mn.item() == val.min().item()
+    xn, yn = x.float().cpu().numpy(), y.float().cpu().numpy()
+    ref = loss64(xn, yn)
+    a = lap64(xn)
+    mass = float(np.sum(np.abs(a * yn) + np.exp(-a * a) + 0.5 * np.abs(yn)))
+    # per-element float32 evaluation (a few ulp) plus the accumulation: Kahan per thread or a
+    # float32 running sum over at most NZ values per thread, float64 beyond
+    bound = (8 if mode else 8 + x.shape[0]) * np.finfo(np.float32).eps * mass
+    assert abs(sm.item() - ref) <= bound
+    assert abs(fsum.item() - ref) <= bound
+
+
+def test_stencil_expr_vjp_reduce_empty_and_rejects(dev):
+    """An empty array yields the reductions' identities (a zero sum) from both fused stencil
+    entry points; the fused vjp needs a reduction and its output pointers."""
+    A, capi, tapes = _mods()
+    from ncarray_amd import torch_interop as ti
+    x = torch.empty((0, 8, 16), device=dev)
+    lap, _ = A.laplacian_stencil(capi.FLOAT32, capi.FLOAT32)
+    tape = A.c0_tape(capi.FLOAT32, capi.FLOAT32, capi.FLOAT32)
+    w = torch.empty_like(x)
+    sm = torch.full((), np.nan, dtype=torch.float32, device=dev)
+    args = dict(extras=[ti.view(x)], scalars=(6.0,), expr_scalars=(0.5,))
+    lap.expr_vjp(tape, [-1, 0, -1, -1, 0], ti.view(x), ti.view(w), fuse=capi.FUSE_SUM, sum_out=sm.data_ptr(), **args)
+    fsum = torch.full((), np.nan, dtype=torch.float32, device=dev)
+    A.fused_loss_stencil(capi.FLOAT32, capi.FLOAT32).apply_reduce(
+        ti.view(x), ti.shape_view(list(x.shape), capi.FLOAT32), capi.BOUNDARY_ZERO, fuse=capi.FUSE_SUM,
+        sum_out=fsum.data_ptr(), extras=[ti.view(x)], scalars=(6.0, 0.5))
+    torch.cuda.synchronize()
+    assert sm.item() == 0.0 and fsum.item() == 0.0
+    x = torch.randn((8, 8, 16), device=dev)
+    w = torch.empty_like(x)
+    args = dict(extras=[ti.view(x)], scalars=(6.0,), expr_scalars=(0.5,))
+    with pytest.raises(capi.NcaError, match="output pointer"):
+        lap.expr_vjp(tape, [-1, 0, -1, -1, 0], ti.view(x), ti.view(w), fuse=capi.FUSE_MAX, **args)
+    with pytest.raises(capi.NcaError, match="no reduction"):
+        capi.call("nca_hip_stencil_expr_vjp_reduce", lap.handle, tape.handle, (ctypes.c_int32 * 5)(-1, 0, -1, -1, 0), 5,
+                  ctypes.byref(ti.view(x)), capi.view_array([ti.view(x)]), 1,
+                  capi.scalar_array((6.0,), lap.scalar_dtypes), 1, capi.scalar_array((0.5,), tape.scalar_dtypes), 1,
+                  None, ctypes.byref(ti.view(w)), capi.view_array([capi.View()]), 0, 0, None, None, None,
+                  capi.BOUNDARY_ZERO, None)
+
+
+def test_value_and_grad_fallback_shape(dev):
+    """laplacian_c0_value_and_grad on a shape the march kernel does not take (the fused forward
+    plus the three-kernel backward) agrees with FP64."""
+    A, capi, tapes = _mods()
+    rng = np.random.default_rng(6)
+    x = rng.standard_normal((12, 14, 18)).astype(np.float32)
+    y = rng.standard_normal((12, 14, 18)).astype(np.float32)
+    xt, yt = torch.from_numpy(x).to(dev), torch.from_numpy(y).to(dev)
+    assert not A._march_vjp_ok(xt, yt)
+    loss, dx, dy = A.laplacian_c0_value_and_grad(xt, yt)
+    torch.cuda.synchronize()
+    gx64, gy64 = grad64(x, y)
+    assert abs(loss.item() - loss64(x, y)) <= 1e-5 * abs(loss64(x, y)) + 1e-3
+    assert np.allclose(dx.cpu().numpy(), gx64, rtol=1e-4, atol=1e-4)
+    assert np.allclose(dy.cpu().numpy(), gy64, rtol=1e-5, atol=1e-5)
 
 
 def test_fused_loss_backward_fallback_shape(dev):
