@@ -236,3 +236,69 @@ def test_validation():
     with pytest.raises(capi.NcaError) as e:
         tape.launch([bad], ti.view(out))
     assert e.value.code == capi.ERR_INVALID
+
+
+# ---- the reference's SOArray known answers, on device SO views -------------------------------------
+def test_reference_soarray_ops():
+    """test_operations.py:89-97: SOArrayRef([2, 4, 6]) + [1, 2, 3] == [3, 6, 9].  The reference
+    binding puts a pointer axis in front (suboffset 0 on axis 0 of shape (1, 3))."""
+    t, im = _image(256)
+    row = im.put(np.array([2, 4, 6], np.int32))
+    tbl = im.put_ptrs([row])
+    _upload(t, im)
+    so = capi.make_so_view(im.addr + tbl, capi.INT32, (1, 3), (8, 4), (0, -1))
+    b = torch.tensor([[1, 2, 3]], dtype=torch.int32, device="cuda")
+    out = torch.empty((1, 3), dtype=torch.int32, device="cuda")
+    tape = capi.Tape([tapes.P(capi.OP_LOAD_SOARR, 0), tapes.P(capi.OP_LOAD_NCARR, 1), tapes.P(capi.OP_ADD)],
+                     [capi.INT32, capi.INT32], [], capi.INT32, capi.INT32)
+    tape.launch([so, ti.view(b)], ti.view(out))
+    torch.cuda.synchronize()
+    assert out.squeeze(0).tolist() == [3, 6, 9]
+
+
+def test_reference_soarray_slice():
+    """test_indexing.py:124-137: soarr[0, 1:3, 0:2] of arange(12).reshape(3, 4) behind a pointer
+    axis; slicing the axes after the indirect one moves its suboffset (1 row of 16 bytes)."""
+    arr = np.arange(12, dtype=np.int32).reshape(3, 4)
+    t, im = _image(512)
+    row = im.put(arr)
+    tbl = im.put_ptrs([row])
+    _upload(t, im)
+    v = capi.make_so_view(im.addr + tbl, capi.INT32, (1, 2, 2), (8, 16, 4), (16, -1, -1))
+    assert np.array_equal(O.so_gather(im.buf, im.addr, im.addr + tbl, np.int32, (1, 2, 2), (8, 16, 4), (16, -1, -1))[0],
+                          arr[1:3, 0:2])
+    out = torch.empty((1, 2, 2), dtype=torch.int32, device="cuda")
+    tape = capi.Tape([tapes.P(capi.OP_LOAD_SOARR, 0)], [capi.INT32], [], capi.INT32, capi.INT32)
+    tape.launch([v], ti.view(out))
+    torch.cuda.synchronize()
+    assert np.array_equal(out[0].cpu().numpy(), arr[1:3, 0:2])
+    assert out[0, 0, 1].item() == 5 and out[0, 1, 0].item() == 8   # soarr[0, 1, 2] == 6 in the unsliced view
+
+
+def test_reference_soarray_reductions():
+    """test_reductions.cc:90-152 on SO views: five 1.5f values sum to 7.5 with max = min = 1.5
+    (through one pointer per element, fused into the expression pass), and iota + 1 on 2 x 3
+    (an SO view without suboffsets, i.e. the reference's SOArray owner) gives axis-0 sums
+    [5, 7, 9], maxima [4, 5, 6], minima [1, 2, 3] and axis-1 sums [6, 15], maxima [3, 6], minima
+    [1, 4]."""
+    t, im = _image(512)
+    vals = im.put(np.full(5, 1.5, np.float32))
+    ptrs = im.put_ptrs([vals + 4 * k for k in range(5)])
+    _upload(t, im)
+    v = capi.make_so_view(im.addr + ptrs, capi.FLOAT32, (5,), (8,), (0,))
+    tape = capi.Tape([tapes.P(capi.OP_LOAD_SOARR, 0)], [capi.FLOAT32], [], capi.FLOAT32, capi.FLOAT32)
+    s, mx, mn = (torch.empty((), dtype=torch.float32, device="cuda") for _ in range(3))
+    tape.launch_reduce([v], capi.make_view(None, capi.FLOAT32, (5,)), capi.FUSE_SUM | capi.FUSE_MAX | capi.FUSE_MIN,
+                       capi.ACCUM_PLAIN, s.data_ptr(), mx.data_ptr(), mn.data_ptr())
+    torch.cuda.synchronize()
+    assert (s.item(), mx.item(), mn.item()) == (7.5, 1.5, 1.5)
+    x = (torch.arange(6, dtype=torch.int32, device="cuda") + 1).reshape(2, 3)
+    sv = capi.make_so_view(x.data_ptr(), capi.INT32, (2, 3), (12, 4), (-1, -1))
+    want = {(0, capi.REDUCE_SUM): [5, 7, 9], (0, capi.REDUCE_MAX): [4, 5, 6], (0, capi.REDUCE_MIN): [1, 2, 3],
+            (1, capi.REDUCE_SUM): [6, 15], (1, capi.REDUCE_MAX): [3, 6], (1, capi.REDUCE_MIN): [1, 4]}
+    for (axis, op), w in want.items():
+        out = torch.empty(len(w), dtype=torch.int32, device="cuda")  # int32 sums stay int32 (sum_type = T)
+        ax = (ctypes.c_int64 * 1)(axis)
+        capi.call("nca_hip_reduce_axes", ctypes.byref(sv), ax, 1, op, capi.ACCUM_PLAIN, ctypes.byref(ti.view(out)), None)
+        torch.cuda.synchronize()
+        assert out.tolist() == w, (axis, op)
