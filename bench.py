@@ -318,13 +318,42 @@ def bench_c0(torch, capi, tapes, ti, dev, sptr, time_loop):
     eager_sum = float(s.item())
     gfm, _ = graph_loop(torch, fused, 20, 10)
     gom, _ = graph_loop(torch, only, 20, 10)
+    # the same expression through the Python array API (the reference's call path: NCDevArray
+    # operators, eager = one kernel and one allocation per operator, lazy = one fused kernel)
+    import ncarray_amd as nca
+    A, B = nca.from_torch(a), nca.from_torch(b)
+    was_eager = nca.is_eager()
+
+    def py_eager():
+        nca.set_eager(True)
+        return A * B + nca.exp(-(A * A)) - B * 0.5
+
+    def py_lazy():
+        with nca.lazy_mode():
+            e = A * B + nca.exp(-(A * A)) - B * 0.5
+        return nca.materialize(e)
+
+    def py_lazy_sum():
+        with nca.lazy_mode():
+            e = A * B + nca.exp(-(A * A)) - B * 0.5
+        return e.sum()  # a host scalar: waits for the result
+
+    pe, _ = time_loop(py_eager, 30, 5)
+    pl, _ = time_loop(py_lazy, 30, 5)
+    ps, _ = time_loop(py_lazy_sum, 30, 5)
+    nca.set_eager(was_eager)
     return {"n": n, "fused_ms": round(float(np.mean(fm)), 4), "fused_GBps": gbps(12 * n, np.mean(fm)),
+            "python_eager_ms": round(float(np.median(pe)), 4), "python_lazy_ms": round(float(np.median(pl)), 4),
+            "python_lazy_sum_ms": round(float(np.median(ps)), 4),
             "expr_only_ms": round(float(np.mean(om)), 4), "expr_only_GBps": gbps(12 * n, np.mean(om)),
             "graph_fused_ms": round(gfm, 4), "graph_fused_GBps": gbps(12 * n, gfm),
             "graph_expr_only_ms": round(gom, 4), "graph_expr_only_GBps": gbps(12 * n, gom),
             "graph_sum_equals_eager": bool(float(s.item()) == eager_sum),
             "note": "cache-resident (50 MB < 256 MB MALL); bytes = read a, b + write out; graph_* = "
-                    "20 steps per hipGraph replay (no host launch overhead)"}
+                    "20 steps per hipGraph replay (no host launch overhead); python_* = the expression "
+                    "written with NCDevArray operators (eager: one kernel and one output allocation per "
+                    "operator; lazy: materialize() runs one fused kernel; lazy_sum: the sum fused into "
+                    "the evaluation, returned to the host), median per call"}
 
 
 def bench_c2(torch, capi, tapes, ti, dev, sptr, time_loop):

@@ -190,7 +190,13 @@ const char* nca_hip_version(void);
 int nca_hip_device_info(int32_t* n_cus, int32_t* lds_per_block, int64_t* total_mem,
                         char* arch_name, size_t arch_len);
 
-/* Device memory for DevOwnerPolicy (storage.hh:176-186, 242-252). */
+/* Device memory for DevOwnerPolicy (storage.hh:176-186, 242-252).  Like the reference's
+ * cudaMallocAsync on one stream, blocks come from the device's default memory pool,
+ * stream-ordered on the legacy default stream: work on that stream (and on streams that
+ * synchronise with it) may use the memory at once; a caller that uses it on a non-blocking
+ * stream orders that stream after the allocation, and finishes that stream's work before
+ * nca_hip_free, which returns the block to the pool in the default stream's order (no device
+ * synchronisation).  Without a pool these are hipMalloc / hipFree. */
 int nca_hip_malloc(void** ptr, size_t bytes);
 int nca_hip_free(void* ptr);
 int nca_hip_memcpy_h2d(void* dst, const void* src, size_t bytes, void* stream);

@@ -80,6 +80,9 @@ struct DeviceState {
   std::string arch = "gfx950";
   std::mutex mu;
   std::map<hipStream_t, std::unique_ptr<Workspace>> workspaces;
+  // the device's default stream-ordered memory pool (nca_hip_malloc / nca_hip_free); null when
+  // the runtime has none, then they fall back to hipMalloc / hipFree
+  hipMemPool_t pool = nullptr;
 };
 
 DeviceState& current_device();           // throws NCA_ERR_NOT_INIT when not initialised

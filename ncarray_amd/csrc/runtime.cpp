@@ -57,6 +57,16 @@ static void init_device(int device) {
     std::string arch = p.gcnArchName;
     auto colon = arch.find(':');
     st->arch = colon == std::string::npos ? arch : arch.substr(0, colon);
+    // Owner allocations come from the device's default memory pool, stream-ordered on the
+    // legacy default stream (the reference allocates with cudaMallocAsync on one stream,
+    // storage.hh:176-186).  Freed blocks stay in the pool (no release threshold) so the next
+    // allocation reuses them without a device synchronisation.
+    hipMemPool_t pool = nullptr;
+    if (hipDeviceGetDefaultMemPool(&pool, device) == hipSuccess && pool) {
+      uint64_t keep = UINT64_MAX;
+      if (hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep) == hipSuccess) st->pool = pool;
+    }
+    (void)hipGetLastError();
     g_devices[device] = std::move(st);
   }
   t_device = device;
@@ -265,6 +275,7 @@ int nca_hip_shutdown(void) {
         if (w.second->big) (void)hipFree(w.second->big);
       }
       kv.second->workspaces.clear();
+      if (kv.second->pool) (void)hipMemPoolTrimTo(kv.second->pool, 0);
     }
   });
 }
@@ -296,16 +307,20 @@ int nca_hip_device_info(int32_t* n_cus, int32_t* lds_per_block, int64_t* total_m
 int nca_hip_malloc(void** ptr, size_t bytes) {
   return api_call([&] {
     NCA_REQUIRE(ptr != nullptr, "nca_hip_malloc: null out pointer");
-    current_device();
+    DeviceState& d = current_device();
     *ptr = nullptr;
     if (bytes == 0) bytes = 256;
-    HIP_CHECK(hipMalloc(ptr, bytes));
+    if (d.pool) HIP_CHECK(hipMallocFromPoolAsync(ptr, bytes, d.pool, nullptr));
+    else HIP_CHECK(hipMalloc(ptr, bytes));
   });
 }
 
 int nca_hip_free(void* ptr) {
   return api_call([&] {
-    if (ptr) HIP_CHECK(hipFree(ptr));
+    if (!ptr) return;
+    DeviceState& d = current_device();
+    if (d.pool) HIP_CHECK(hipFreeAsync(ptr, nullptr));
+    else HIP_CHECK(hipFree(ptr));
   });
 }
 
