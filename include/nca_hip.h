@@ -340,6 +340,10 @@ int nca_hip_stencil_expr_vjp_reduce(uint64_t stencil_handle, uint64_t expr_handl
                                     const nca_view* grad_extras, int32_t fuse, int32_t accum_mode,
                                     void* sum_out, void* max_out, void* min_out, int32_t boundary,
                                     void* stream);
+/* The generated source of a stencil kernel (for inspection and nca_hip_rtc_compile):
+ * variant 0 the march kernel, 1 the march with the fused axis-0 sum and max, 4 the same
+ * with compensated sums, 2 / 3 the direct kernel without / with the zero boundary.
+ * Returns the text length (negative status on error); writes at most n bytes. */
 int nca_hip_stencil_source(uint64_t handle, int32_t variant, char* buf, size_t n);
 
 /* ================================================================================ */

@@ -705,8 +705,9 @@ static std::string gen_march_q(const StencilProgram& p, const StencilFuse& fz, i
              << "]);\n";
         if (fz.axis0)
           for (int j = 0; j < VX; ++j) {
-            if (comp) os << "        nca_kahan_add(cs[" << r << "][" << j << "], cc[" << r << "][" << j << "], (float)rv" << j << ");\n";
-            else os << "        cs[" << r << "][" << j << "] += (float)rv" << j << ";\n";
+            const std::string rj = "[" + I(r) + "][" + I(j) + "]";
+            if (comp) os << "        nca_kahan_add(cs" << rj << ", cc" << rj << ", (float)rv" << j << ");\n";
+            else os << "        cs" << rj << " += (float)rv" << j << ";\n";
           }
         auto tree = [&](const std::string& f) {
           std::vector<std::string> v;
@@ -1473,13 +1474,14 @@ int nca_hip_stencil_source(uint64_t handle, int32_t variant, char* buf, size_t n
   std::string src;
   int rc = api_call([&] {
     StencilProgram& p = stencil_program(handle);
-    if (variant == 0 || variant == 1) {
+    if (variant == 0 || variant == 1 || variant == 4) {
       NCA_REQUIRE(p.ndim == 3, "the march kernel is 3-D");
       StencilFuse fz;
-      if (variant == 1) {
+      if (variant != 0) {
         fz.axis0 = true;
         fz.fuse = NCA_FUSE_MAX;
       }
+      if (variant == 4) fz.mode = NCA_ACCUM_COMPENSATED;
       const MarchPlan mp = march_plan(p);
       src = mp.queue ? gen_march_q(p, fz, p.tape.dest_dtype, mp) : gen_march(p, fz, p.tape.dest_dtype);
     } else {

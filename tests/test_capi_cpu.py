@@ -110,8 +110,10 @@ def test_linearizable_vs_vm_evaluation_dtype():
 @pytest.mark.parametrize("dt", [capi.FLOAT32, capi.BFLOAT16])
 def test_stencil_kernels_compile(dt):
     st = capi.Stencil(tapes.LAPLACIAN_OFFSETS, tapes.laplacian(), dt, capi.FLOAT32, dt, [capi.FLOAT32])
-    for variant in (0, 1, 2, 3):
+    for variant in (0, 1, 2, 3, 4):
         _compiles(st.source(variant))
+    # compensated column sums: one Kahan step per plane and column
+    assert "nca_kahan_add(cs[0][0], cc[0][0]" in st.source(4) and "cc[" not in st.source(1)
     adj = st.adjoint()
     for variant in (0, 2):
         _compiles(adj.source(variant))
