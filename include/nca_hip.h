@@ -340,6 +340,32 @@ int nca_hip_stencil_expr_vjp_reduce(uint64_t stencil_handle, uint64_t expr_handl
                                     const nca_view* grad_extras, int32_t fuse, int32_t accum_mode,
                                     void* sum_out, void* max_out, void* min_out, int32_t boundary,
                                     void* stream);
+/* A whole training step of loss = sum(expr(stencil(src), extras)) in one pass over src and
+ * the extras (no counterpart in the reference): grad_src = stencil^T(d expr / d a * g), the
+ * extras' gradients (grad_extras[j] with data) and, with fuse != 0, the reductions of the
+ * expression's value (as nca_hip_stencil_expr_vjp_reduce).  The pass stages src with a
+ * two-point halo in LDS, evaluates d expr / d a on each tile and a one-point ring around it,
+ * and applies the stencil's adjoint from the last three planes of it kept in LDS, so
+ * nothing but the inputs and the gradients reaches memory.  Requires a linear 3-D stencil
+ * without extras or hooks, of radius <= 1 on every axis, boundary ZERO, and C-contiguous
+ * arrays of one shape (any extents, no alignment requirement). */
+int nca_hip_stencil_expr_train(uint64_t stencil_handle, uint64_t expr_handle,
+                               const int32_t* expr_operand, int32_t n_operand,
+                               const nca_view* src, const nca_view* extras, int32_t n_extra,
+                               const nca_scalar* stencil_scalars, int32_t n_stencil_scalars,
+                               const nca_scalar* expr_scalars, int32_t n_expr_scalars,
+                               const void* grad_scalar, const nca_view* grad_src,
+                               const nca_view* grad_extras, int32_t fuse, int32_t accum_mode,
+                               void* sum_out, void* max_out, void* min_out, int32_t boundary,
+                               void* stream);
+/* The generated source of that pass for the given operand codes, extra dtypes, grad_src
+ * dtype and reductions (every extra's gradient written), for inspection and
+ * nca_hip_rtc_compile.  Returns the text length (negative status on error). */
+int nca_hip_stencil_expr_train_source(uint64_t stencil_handle, uint64_t expr_handle,
+                                      const int32_t* expr_operand, int32_t n_operand,
+                                      const int32_t* extra_dtypes, int32_t n_extra,
+                                      int32_t grad_src_dtype, int32_t fuse, int32_t accum_mode,
+                                      char* buf, size_t n);
 /* The generated source of a stencil kernel (for inspection and nca_hip_rtc_compile):
  * variant 0 the march kernel, 1 the march with the fused axis-0 sum and max, 4 the same
  * with compensated sums, 2 / 3 the direct kernel without / with the zero boundary.

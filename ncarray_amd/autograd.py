@@ -259,13 +259,23 @@ def _march_vjp_ok(x, y):
             and x.data_ptr() % 16 == 0 and y.data_ptr() % (vx * y.element_size()) == 0)
 
 
-def laplacian_c0_backward(x, y, g=None):
+def _train_ok(x, y):
+    """What the one-pass training kernel needs: 3-D C-contiguous x and y of one shape."""
+    return x.dim() == 3 and x.is_contiguous() and y.is_contiguous() and tuple(x.shape) == tuple(y.shape)
+
+
+def laplacian_c0_backward(x, y, g=None, method=None):
     """(dx, dy) of sum(C0(Laplacian(x), y)) times the 0-d upstream gradient g (None: 1).
 
-    One march pass computes a = Laplacian(x) on the fly and writes w = dC0/da * g and
-    dy = dC0/db * g (nca_hip_stencil_expr_vjp); the transposed Laplacian of w is dx.  Shapes the
-    march kernel does not take go through the three-kernel composition (Laplacian, derivative
-    expression, transposed stencil)."""
+    method "vjp" (the default where the vectorised march kernels apply: 3-D, last extent a whole
+    number of 16-byte vectors, aligned arrays): nca_hip_stencil_expr_vjp writes w = dC0/da * g
+    and dy = dC0/db * g in one march pass and the transposed stencil turns w into dx.  "train"
+    (the default for other 3-D C-contiguous arrays): one pass (nca_hip_stencil_expr_train)
+    stages x with a two-point halo in LDS, evaluates w on each tile and a one-point ring, writes
+    dy and applies the transposed Laplacian to w from LDS; it moves fewer bytes but loads one
+    element per lane, and measured no faster at 256^3 (f32 72 vs 72.5 us, BF16 97 vs 58 us), so
+    it serves the shapes the march kernels do not take.  "compose": the Laplacian, the
+    derivative expression and the transposed stencil as three kernels (any shape)."""
     s = _stream()
     xcode, ycode = _code(x), _code(y)
     lap, adj = laplacian_stencil(xcode, capi.FLOAT32)
@@ -274,15 +284,26 @@ def laplacian_c0_backward(x, y, g=None):
         gf = None
     else:
         gf = g if g.dtype == torch.float32 else g.float()
-    if _march_vjp_ok(x, y):
+    gptr = None if gf is None else gf.data_ptr()
+    if method is None:
+        method = "vjp" if _march_vjp_ok(x, y) else ("train" if _train_ok(x, y) else "compose")
+    if method == "train":
+        dx = torch.empty(x.shape, dtype=x.dtype, device=x.device)
+        gy = torch.empty_like(y)
+        lap.expr_train(tape, [-1, 0, -1, -1, 0], ti.view(x), ti.view(dx), extras=[ti.view(y)],
+                       grad_extras=[ti.view(gy)], scalars=(6.0,), expr_scalars=(0.5,), grad_scalar=gptr, stream=s)
+        return dx, gy
+    if method == "vjp":
         w = torch.empty(x.shape, dtype=torch.float32, device=x.device)
         gy = torch.empty_like(y)
         vw = ti.view(w)
         lap.expr_vjp(tape, [-1, 0, -1, -1, 0], ti.view(x), vw, extras=[ti.view(y)], grad_extras=[ti.view(gy)],
-                     scalars=(6.0,), expr_scalars=(0.5,), grad_scalar=None if gf is None else gf.data_ptr(), stream=s)
+                     scalars=(6.0,), expr_scalars=(0.5,), grad_scalar=gptr, stream=s)
         dx = torch.empty(x.shape, dtype=x.dtype, device=x.device)
         adj.apply(vw, ti.view(dx), capi.BOUNDARY_ZERO, scalars=(6.0,), stream=s)
         return dx, gy
+    if method != "compose":
+        raise ValueError(f"unknown method {method!r}")
     if gf is None:
         gf = torch.ones((), dtype=torch.float32, device=x.device)
     a = torch.empty(x.shape, dtype=torch.float32, device=x.device)
@@ -306,20 +327,21 @@ def laplacian_c0_loss(x, y, compensated=True):
 def laplacian_c0_value_and_grad(x, y, compensated=True, grad_scale=None):
     """(loss, dx, dy) of loss = sum(C0(Laplacian(x), y)) without torch's autograd, all launched
     from the calling thread on its current stream, so a whole training step can be captured in
-    one hipGraph.  Where the march kernel applies, one pass computes the loss (the forward's
-    accumulation: per-thread float32, Kahan when compensated, float64 across blocks) together
-    with w = dC0/da * g and dy, and the transposed Laplacian of w gives dx; otherwise the fused
-    forward kernel and laplacian_c0_backward run.  grad_scale: optional 0-d float32 device
-    tensor scaling the gradients (the upstream gradient; default 1)."""
+    one hipGraph.  The loss is fused into the gradient pass (per-thread float32, Kahan when
+    compensated, float64 across blocks): where the vectorised march kernels apply,
+    nca_hip_stencil_expr_vjp_reduce (loss, dloss/da, dy) and the transposed stencil (dx); for
+    other 3-D arrays the one-pass nca_hip_stencil_expr_train; otherwise the fused forward kernel
+    and laplacian_c0_backward.  grad_scale: optional 0-d float32 device tensor scaling the
+    gradients (the upstream gradient; default 1)."""
     _init(x)
     xc, yc = x.detach().contiguous(), y.detach().contiguous()
     loss = torch.empty((), dtype=torch.float32, device=x.device)
     mode = capi.ACCUM_COMPENSATED if compensated else capi.ACCUM_PLAIN
+    gf = None if grad_scale is None else (grad_scale if grad_scale.dtype == torch.float32 else grad_scale.float())
     if _march_vjp_ok(xc, yc):
         s = _stream()
         lap, adj = laplacian_stencil(_code(xc), capi.FLOAT32)
         tape = c0_tape(capi.FLOAT32, _code(yc), capi.FLOAT32)
-        gf = None if grad_scale is None else (grad_scale if grad_scale.dtype == torch.float32 else grad_scale.float())
         w = torch.empty(xc.shape, dtype=torch.float32, device=x.device)
         dy = torch.empty_like(yc)
         vw = ti.view(w)
@@ -328,6 +350,16 @@ def laplacian_c0_value_and_grad(x, y, compensated=True, grad_scale=None):
                      stream=s, fuse=capi.FUSE_SUM, mode=mode, sum_out=loss.data_ptr())
         dx = torch.empty(xc.shape, dtype=xc.dtype, device=x.device)
         adj.apply(vw, ti.view(dx), capi.BOUNDARY_ZERO, scalars=(6.0,), stream=s)
+        return loss, dx, dy
+    if _train_ok(xc, yc):
+        lap, _ = laplacian_stencil(_code(xc), capi.FLOAT32)
+        tape = c0_tape(capi.FLOAT32, _code(yc), capi.FLOAT32)
+        dx = torch.empty(xc.shape, dtype=xc.dtype, device=x.device)
+        dy = torch.empty_like(yc)
+        lap.expr_train(tape, [-1, 0, -1, -1, 0], ti.view(xc), ti.view(dx), extras=[ti.view(yc)],
+                       grad_extras=[ti.view(dy)], scalars=(6.0,), expr_scalars=(0.5,),
+                       grad_scalar=None if gf is None else gf.data_ptr(), stream=_stream(), fuse=capi.FUSE_SUM,
+                       mode=mode, sum_out=loss.data_ptr())
         return loss, dx, dy
     st = fused_loss_stencil(_code(xc), _code(yc))
     st.apply_reduce(ti.view(xc), ti.shape_view(list(xc.shape), capi.FLOAT32), capi.BOUNDARY_ZERO,

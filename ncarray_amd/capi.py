@@ -130,6 +130,10 @@ _SIGS = {
                                  _VP, _I32, _P],
     "nca_hip_stencil_expr_vjp_reduce": [_U64, _U64, ctypes.POINTER(_I32), _I32, _VP, _VP, _I32, _SP, _I32, _SP, _I32,
                                         _P, _VP, _VP, _I32, _I32, _P, _P, _P, _I32, _P],
+    "nca_hip_stencil_expr_train": [_U64, _U64, ctypes.POINTER(_I32), _I32, _VP, _VP, _I32, _SP, _I32, _SP, _I32,
+                                   _P, _VP, _VP, _I32, _I32, _P, _P, _P, _I32, _P],
+    "nca_hip_stencil_expr_train_source": [_U64, _U64, ctypes.POINTER(_I32), _I32, ctypes.POINTER(_I32), _I32, _I32,
+                                          _I32, _I32, ctypes.c_char_p, _SZ],
     "nca_hip_stencil_source": [_U64, _I32, ctypes.c_char_p, _SZ],
     "nca_hip_scatter_add": [_VP, _VP, _VP, _P],
     "nca_hip_bin": [_VP, _VP, _I32, _I32, _VP, _VP, _P],
@@ -370,6 +374,34 @@ class Stencil:
             call("nca_hip_stencil_expr_vjp", self.handle, tape.handle, ops, len(operands), ctypes.byref(src), ea,
                  len(extras), sa, len(scalars), xa, len(expr_scalars), grad_scalar, ctypes.byref(w), ga, boundary,
                  stream)
+
+    def expr_train(self, tape, operands, src, grad_src, extras=(), grad_extras=(), scalars=(), expr_scalars=(),
+                   grad_scalar=None, boundary=BOUNDARY_ZERO, stream=None, fuse=0, mode=ACCUM_PLAIN, sum_out=None,
+                   max_out=None, min_out=None):
+        """One pass of a training step of sum(tape(stencil(src), extras)): grad_src = the stencil's
+        adjoint of d tape / d a * g, the extras' gradients (grad_extras[j] None: not computed) and
+        with fuse != 0 the reductions of the tape's value into the device scalars sum_out /
+        max_out / min_out; `operands` and g as in expr_vjp."""
+        ops = (_I32 * max(1, len(operands)))(*operands)
+        ea = view_array(extras)
+        ge = [v if v is not None else View() for v in grad_extras] + [View()] * (len(extras) - len(grad_extras))
+        ga = view_array(ge)
+        sa = scalar_array(scalars, self.scalar_dtypes)
+        xa = scalar_array(expr_scalars, tape.scalar_dtypes)
+        call("nca_hip_stencil_expr_train", self.handle, tape.handle, ops, len(operands), ctypes.byref(src), ea,
+             len(extras), sa, len(scalars), xa, len(expr_scalars), grad_scalar, ctypes.byref(grad_src), ga, fuse, mode,
+             sum_out, max_out, min_out, boundary, stream)
+
+    def train_source(self, tape, operands, extra_dtypes, grad_src_dtype, fuse=0, mode=ACCUM_PLAIN):
+        ops = (_I32 * max(1, len(operands)))(*operands)
+        xd = (_I32 * max(1, len(extra_dtypes)))(*extra_dtypes)
+        args = (self.handle, tape.handle, ops, len(operands), xd, len(extra_dtypes), grad_src_dtype, fuse, mode)
+        n = lib.nca_hip_stencil_expr_train_source(*args, None, 0)
+        if n < 0:
+            check(-n)
+        buf = ctypes.create_string_buffer(n + 1)
+        lib.nca_hip_stencil_expr_train_source(*args, buf, n + 1)
+        return buf.value.decode()
 
     def source(self, variant=0):
         n = lib.nca_hip_stencil_source(self.handle, variant, None, 0)

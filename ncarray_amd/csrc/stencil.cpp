@@ -793,6 +793,202 @@ static std::string gen_march_q(const StencilProgram& p, const StencilFuse& fz, i
   return os.str();
 }
 
+// ---- training-step march -----------------------------------------------------------------------
+// One pass of a whole training step for loss = sum(expr(S(x), extras)) with S linear and of
+// radius <= 1 on every axis (zero boundary).  A block owns a TX x TY tile over a run of z
+// planes.  Per plane z it stages x with a two-point halo in LDS (three planes, the next one
+// prefetched into registers), evaluates w = g * d expr / d a on the tile and a one-point ring
+// around it (zero outside the array), and from the last three w planes in LDS applies the
+// adjoint of S to give dx at plane z - 1.  The extras' gradients and the reductions of the
+// expression's value are taken at the tile's own points of its own planes.  x and the extras
+// are read once and dx and the extras' gradients written once: nothing else reaches HBM.
+static std::string gen_march_train(const StencilProgram& p, const Tape& adj, const MarchVjp& vj, const StencilFuse& fz,
+                                   int32_t dx_dtype, int TXs, int TYs) {
+  const Tape& t = p.tape;
+  const std::string C = ctype_of(t.compute_dtype);
+  const std::string S = ctype_of(p.src_dtype);
+  const std::string DX = ctype_of(dx_dtype);
+  const std::string VD = ctype_of(vj.value_dtype);
+  const int32_t rcd = compute_dtype(vj.value_dtype);
+  const std::string RC = ctype_of(rcd);
+  const bool comp = fz.mode == NCA_ACCUM_COMPENSATED;
+  const int n_off = p.n_off;
+  const int n_x = static_cast<int>(vj.extra_dtypes.size());
+  auto I = [](long long v) { return std::to_string(v); };
+  // stencil value a at a ring point, the derivative roots and (with reductions) the value
+  std::vector<int> alias(t.arr_dtypes.size());
+  for (size_t k = 0; k < alias.size(); ++k) alias[k] = static_cast<int>(k);
+  std::vector<std::string> sroots;
+  const std::string sbody = emit_nodes(t, {t.root}, alias, names_u(static_cast<int>(t.arr_dtypes.size())),
+                                       names_s(static_cast<int>(t.scalar_dtypes.size())), "lin", "t", sroots);
+  std::vector<std::string> en(vj.operand.size()), rn(vj.expr.scalar_dtypes.size());
+  for (size_t k = 0; k < en.size(); ++k) en[k] = vj.operand[k] < 0 ? sroots[0] : "u" + I(n_off + vj.operand[k]);
+  for (size_t i = 0; i < rn.size(); ++i) rn[i] = "r" + I(i);
+  std::vector<int> live;
+  for (int r : vj.roots)
+    if (r >= 0) live.push_back(r);
+  if (fz.fuse) live.push_back(vj.expr.root);
+  std::vector<std::string> ln;
+  const std::string vbody = emit_nodes(vj.expr, live, vj.alias, en, rn, "lin", "q", ln);
+  std::vector<std::string> vnames;
+  size_t li = 0;
+  for (int r : vj.roots) vnames.push_back(r >= 0 ? ln[li++] : "(" + C + ")0");
+  const std::string pname = fz.fuse ? ln[li] : "";
+  // the adjoint: LOAD k reads w at the negated offset k
+  std::vector<std::string> vn;
+  for (int k = 0; k < n_off; ++k) vn.push_back("v" + I(k));
+  std::vector<int> aalias(adj.arr_dtypes.size());
+  for (size_t k = 0; k < aalias.size(); ++k) aalias[k] = static_cast<int>(k);
+  std::vector<std::string> aroots;
+  const std::string abody = emit_nodes(adj, {adj.root}, aalias, vn, names_s(static_cast<int>(adj.scalar_dtypes.size())),
+                                       "lin", "a", aroots);
+  const std::string kname = std::string("nca_stencil_train_") + dtype_info(p.src_dtype).name + (fz.fuse ? "_sum" : "");
+  std::ostringstream os;
+  os << "#define TX " << TXs << "\n#define TY " << TYs << "\n#define XW (TX + 4)\n#define XH (TY + 4)\n"
+     << "#define RW (TX + 2)\n#define RH (TY + 2)\n#define NXL ((XW * XH + 255) / 256)\n#define NRL ((RW * RH + 255) / 256)\n"
+     << "#define NTL ((TX * TY + 255) / 256)\n";
+  os << "__device__ __forceinline__ int nca_mod3(int z) { const int r = z % 3; return r < 0 ? r + 3 : r; }\n";
+  os << "extern \"C\" __global__ __launch_bounds__(256) void " << kname << "(const " << S << "* __restrict__ src";
+  for (int j = 0; j < n_x; ++j) os << ", const " << ctype_of(vj.extra_dtypes[j]) << "* __restrict__ e" << j;
+  os << ", " << DX << "* __restrict__ dxo";
+  for (size_t m = 1; m < vj.out_dtypes.size(); ++m) os << ", " << ctype_of(vj.out_dtypes[m]) << "* __restrict__ o" << m;
+  os << ", nca_i64 NZ, nca_i64 NY, nca_i64 NX, int ntx, int nty, int zlen";
+  if (fz.fuse) os << ", void* __restrict__ psum, void* __restrict__ pmax, void* __restrict__ pmin";
+  for (size_t i = 0; i < t.scalar_dtypes.size(); ++i) os << ", " << C << " s" << i;
+  for (size_t i = 0; i < vj.expr.scalar_dtypes.size(); ++i) os << ", " << C << " r" << i;
+  os << ", const " << C << "* __restrict__ gin) {\n";
+  os << "  const " << C << " gsc = gin ? *gin : (" << C << ")1;\n";
+  // LDS planes, flat: x with a two-point halo, w and the extras on the ring region
+  os << "  __shared__ " << C << " xs[3][XH * XW];\n  __shared__ " << C << " ws[3][RH * RW];\n";
+  for (int j = 0; j < n_x; ++j) os << "  __shared__ " << C << " es" << j << "[RH * RW];\n";
+  os << "  const unsigned nb = gridDim.x, b = blockIdx.x;\n"
+     << "  const unsigned tix = (nb % 8u == 0u) ? (b % 8u) * (nb / 8u) + b / 8u : b;\n"
+     << "  const int txi = tix % ntx, tyi = (tix / ntx) % nty, tzi = tix / (ntx * nty);\n"
+     << "  const int x0 = txi * TX, y0 = tyi * TY, NYi = (int)NY, NXi = (int)NX;\n"
+     << "  const int zb = (int)((nca_i64)tzi * zlen), ze = (int)((zb + (nca_i64)zlen < NZ) ? zb + zlen : NZ);\n"
+     << "  const size_t plane = (size_t)NY * (size_t)NX;\n  const int NZi = (int)NZ;\n  const int tid = threadIdx.x;\n";
+  // per-thread positions, the same on every plane (element offsets within a plane fit in 32
+  // bits; -1: outside the array)
+  os << "  int xo[NXL];\n#pragma unroll\n  for (int m = 0; m < NXL; ++m) {\n"
+     << "    const int i = tid + 256 * m, ly = i / XW, lx = i - ly * XW, gy = y0 - 2 + ly, gx = x0 - 2 + lx;\n"
+     << "    xo[m] = (i < XW * XH && gy >= 0 && gy < NYi && gx >= 0 && gx < NXi) ? gy * NXi + gx : -1;\n  }\n";
+  os << "  int go[NRL], xb[NRL];\n  unsigned own = 0u;\n#pragma unroll\n  for (int m = 0; m < NRL; ++m) {\n"
+     << "    const int i = tid + 256 * m, ly = i / RW, lx = i - ly * RW, gy = y0 - 1 + ly, gx = x0 - 1 + lx;\n"
+     << "    const bool in = i < RW * RH && gy >= 0 && gy < NYi && gx >= 0 && gx < NXi;\n"
+     << "    go[m] = in ? gy * NXi + gx : -1;\n    xb[m] = (ly + 1) * XW + lx + 1;\n"
+     << "    if (in && ly >= 1 && ly <= TY && lx >= 1 && lx <= TX) own |= 1u << m;\n  }\n";
+  os << "  int to[NTL], tb[NTL];\n#pragma unroll\n  for (int m = 0; m < NTL; ++m) {\n"
+     << "    const int i = tid + 256 * m, ty = i / TX, tx = i - ty * TX, gy = y0 + ty, gx = x0 + tx;\n"
+     << "    to[m] = (i < TX * TY && gy < NYi && gx < NXi) ? gy * NXi + gx : -1;\n    tb[m] = (ty + 1) * RW + tx + 1;\n  }\n";
+  os << "  " << C << " xr[NXL];\n";
+  for (int j = 0; j < n_x; ++j) os << "  " << C << " er" << j << "[NRL];\n";
+  os << "  auto load_x = [&](int z) {\n    const bool zin = (unsigned)z < (unsigned)NZi;\n"
+     << "    const " << S << "* __restrict__ pl = src + (zin ? (size_t)z * plane : (size_t)0);\n"
+     << "#pragma unroll\n    for (int m = 0; m < NXL; ++m)\n"
+     << "      xr[m] = (zin && xo[m] >= 0) ? nca_cvt<" << C << ", " << S << ">::from(pl[xo[m]]) : (" << C << ")0;\n  };\n";
+  os << "  auto store_x = [&](int slot) {\n#pragma unroll\n    for (int m = 0; m < NXL; ++m) {\n"
+     << "      const int i = tid + 256 * m;\n      if (i < XW * XH) xs[slot][i] = xr[m];\n    }\n  };\n";
+  os << "  auto load_e = [&](int z) {\n    const bool zin = (unsigned)z < (unsigned)NZi;\n";
+  for (int j = 0; j < n_x; ++j) {
+    const std::string E = ctype_of(vj.extra_dtypes[j]);
+    os << "    { const " << E << "* __restrict__ pl = e" << j << " + (zin ? (size_t)z * plane : (size_t)0);\n"
+       << "#pragma unroll\n      for (int m = 0; m < NRL; ++m)\n"
+       << "        er" << j << "[m] = (zin && go[m] >= 0) ? nca_cvt<" << C << ", " << E << ">::from(pl[go[m]]) : (" << C
+       << ")0; }\n";
+  }
+  os << "  };\n";
+  os << "  auto store_e = [&]() {\n#pragma unroll\n    for (int m = 0; m < NRL; ++m) {\n"
+     << "      const int i = tid + 256 * m;\n      if (i < RW * RH) {\n";
+  for (int j = 0; j < n_x; ++j) os << "        es" << j << "[i] = er" << j << "[m];\n";
+  os << "      }\n    }\n  };\n";
+  if (fz.fuse & NCA_FUSE_SUM) {
+    os << "  " << RC << " acc_s = 0;\n";
+    if (comp) os << "  " << RC << " acc_c = 0;\n";
+  }
+  if (fz.fuse & NCA_FUSE_MAX) os << "  " << RC << " acc_mx = nca_lim<" << RC << ">::lowest();\n";
+  if (fz.fuse & NCA_FUSE_MIN) os << "  " << RC << " acc_mn = nca_lim<" << RC << ">::max();\n";
+  // prologue: x planes zb-2, zb-1 in LDS, x(zb) and the extras' plane zb-1 in registers
+  os << "  load_x(zb - 2);\n  store_x(nca_mod3(zb - 2));\n  load_x(zb - 1);\n  store_x(nca_mod3(zb - 1));\n"
+     << "  load_x(zb);\n  load_e(zb - 1);\n";
+  os << "  for (int zw = zb - 1; zw <= ze; ++zw) {\n"
+     << "    store_x(nca_mod3(zw + 1));\n    store_e();\n"
+     << "    if (zw + 2 <= ze + 1) load_x(zw + 2);\n    if (zw + 1 <= ze) load_e(zw + 1);\n"
+     << "    __syncthreads();\n";
+  // w on the ring region
+  os << "    const bool own_z = zw >= zb && zw < ze, zin = (unsigned)zw < (unsigned)NZi;\n"
+     << "    const int sw = nca_mod3(zw), sm = nca_mod3(zw - 1), sp = nca_mod3(zw + 1);\n"
+     << "    const size_t zoff = (size_t)(zw > 0 ? zw : 0) * plane;\n"
+     << "#pragma unroll\n    for (int m = 0; m < NRL; ++m) {\n"
+     << "      const int i = tid + 256 * m;\n"
+     << "      if (i < RW * RH) {\n"
+     << "        " << C << " wv = (" << C << ")0;\n"
+     << "        if (zin && go[m] >= 0) {\n";
+  for (int k = 0; k < n_off; ++k) {
+    const auto& o = p.offsets[k];
+    const std::string sl = o[0] < 0 ? "sm" : (o[0] > 0 ? "sp" : "sw");
+    os << "          const " << C << " u" << k << " = xs[" << sl << "][xb[m] + (" << o[1] * (TXs + 4) + o[2] << ")];\n";
+  }
+  for (int j = 0; j < n_x; ++j) os << "          const " << C << " u" << (n_off + j) << " = es" << j << "[i];\n";
+  os << sbody << vbody;
+  os << "          wv = (" << C << ")(gsc * " << vnames[0] << ");\n";
+  const bool own_work = vj.out_dtypes.size() > 1 || fz.fuse;
+  if (own_work) {
+    os << "          if (own_z && ((own >> m) & 1u)) {\n"
+       << "            const size_t off = zoff + (size_t)go[m];\n";
+    for (size_t m = 1; m < vj.out_dtypes.size(); ++m)
+      os << "            o" << m << "[off] = nca_store_cvt<" << ctype_of(vj.out_dtypes[m]) << ", " << C << ">::to((" << C
+         << ")(gsc * " << vnames[m] << "));\n";
+    if (fz.fuse) {
+      os << "            const " << RC << " rv = nca_cvt<" << RC << ", " << VD << ">::from(nca_store_cvt<" << VD << ", " << C
+         << ">::to(" << pname << "));\n";
+      if (fz.fuse & NCA_FUSE_SUM) {
+        if (comp) os << "            nca_kahan_add(acc_s, acc_c, rv);\n";
+        else os << "            acc_s += rv;\n";
+      }
+      if (fz.fuse & NCA_FUSE_MAX) os << "            acc_mx = nca_maxnum(acc_mx, rv);\n";
+      if (fz.fuse & NCA_FUSE_MIN) os << "            acc_mn = nca_minnum(acc_mn, rv);\n";
+    }
+    os << "          }\n";
+  }
+  os << "        }\n        ws[sw][i] = wv;\n      }\n    }\n    __syncthreads();\n";
+  // dx at plane zw - 1 from w planes zw - 2 .. zw
+  os << "    if (zw - 1 >= zb) {\n"
+     << "      const int a0 = nca_mod3(zw - 2), a1 = sm, a2 = sw;\n"
+     << "      const size_t doff = (size_t)(zw - 1) * plane;\n"
+     << "#pragma unroll\n      for (int m = 0; m < NTL; ++m) {\n"
+     << "        if (to[m] >= 0) {\n";
+  for (int k = 0; k < n_off; ++k) {
+    const auto& o = p.offsets[k];  // the adjoint reads w at -o
+    const int dz = static_cast<int>(-o[0]);
+    const std::string sl = dz < 0 ? "a0" : (dz > 0 ? "a2" : "a1");
+    os << "          const " << C << " v" << k << " = ws[" << sl << "][tb[m] + (" << -o[1] * (TXs + 2) - o[2] << ")];\n";
+  }
+  os << abody;
+  os << "          dxo[doff + (size_t)to[m]] = nca_store_cvt<" << DX << ", " << C << ">::to(" << aroots[0] << ");\n";
+  os << "        }\n      }\n    }\n  }\n";
+  if (fz.fuse) {
+    os << "  __shared__ double red_sd[4];\n  __shared__ " << RC << " red_mx[4];\n  __shared__ " << RC << " red_mn[4];\n"
+       << "  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;\n";
+    if (fz.fuse & NCA_FUSE_SUM) {
+      if (comp) os << "  nca_wave_comp(acc_s, acc_c);\n  if (lane == 0) red_sd[wid] = nca_comp_value(acc_s, acc_c);\n";
+      else os << "  acc_s = nca_wave_sum(acc_s);\n  if (lane == 0) red_sd[wid] = (double)acc_s;\n";
+    }
+    if (fz.fuse & NCA_FUSE_MAX) os << "  acc_mx = nca_wave_max(acc_mx);\n  if (lane == 0) red_mx[wid] = acc_mx;\n";
+    if (fz.fuse & NCA_FUSE_MIN) os << "  acc_mn = nca_wave_min(acc_mn);\n  if (lane == 0) red_mn[wid] = acc_mn;\n";
+    os << "  __syncthreads();\n  if (threadIdx.x == 0) {\n";
+    if (fz.fuse & NCA_FUSE_SUM) os << "    ((double*)psum)[b] = red_sd[0] + red_sd[1] + red_sd[2] + red_sd[3];\n";
+    if (fz.fuse & NCA_FUSE_MAX)
+      os << "    " << RC << " m = red_mx[0];\n    for (int w = 1; w < 4; ++w) m = nca_maxop(m, red_mx[w]);\n    *(" << RC
+         << "*)((char*)pmax + 8 * (size_t)b) = m;\n";
+    if (fz.fuse & NCA_FUSE_MIN)
+      os << "    " << RC << " m2 = red_mn[0];\n    for (int w = 1; w < 4; ++w) m2 = nca_minop(m2, red_mn[w]);\n    *(" << RC
+         << "*)((char*)pmin + 8 * (size_t)b) = m2;\n";
+    os << "  }\n";
+  }
+  os << "}\n";
+  return os.str();
+}
+
 // ---- direct kernel ----------------------------------------------------------------------------------------
 static std::string gen_direct(const StencilProgram& p, int32_t dst_dtype, bool zero_boundary, int fuse, int mode) {
   const Tape& t = p.tape;
@@ -1220,6 +1416,43 @@ int nca_hip_stencil_apply_reduce(uint64_t handle, const nca_view* src, const nca
 }  // extern "C"
 
 namespace nca {
+// The adjoint (transpose) tape of a stencil linear in its source loads: sum_k c_k * LOAD k with
+// c_k = d f / d v_k derived symbolically (constants or functions of the scalars, whose slots it
+// shares with the forward tape); LOAD k reads the gradient at the negated offset k.
+static Tape adjoint_tape(const StencilProgram& f) {
+  NCA_REQUIRE(!f.has_ext, "cannot differentiate a stencil with extension hooks");
+  Tape base = f.tape;
+  std::vector<int> alias(base.arr_dtypes.size());
+  for (size_t k = 0; k < alias.size(); ++k) alias[k] = static_cast<int>(k);
+  // c_k = d f / d v_k must not depend on any load (linear stencil)
+  std::vector<int> coef;
+  for (int k = 0; k < f.n_off; ++k) {
+    int dk = differentiate(base, base.root, k, alias);
+    NCA_REQUIRE(!depends_on_loads(base, dk),
+                "the stencil is not linear in its source loads; its adjoint is not a stencil");
+    coef.push_back(dk);
+  }
+  Tape t = base;
+  t.arr_dtypes.assign(f.n_off, f.tape.dest_dtype);
+  int acc = -1;
+  for (int k = 0; k < f.n_off; ++k) {
+    if (coef[k] < 0) continue;
+    Node ld;
+    ld.op = NCA_OP_LOAD_NCARR;
+    ld.index = k;
+    int term = t.bin(NCA_OP_MUL, coef[k], t.add(ld));
+    acc = acc < 0 ? term : t.bin(NCA_OP_ADD, acc, term);
+  }
+  NCA_REQUIRE(acc >= 0, "the stencil does not depend on its source");
+  t.root = acc;
+  t.uses_idx = false;
+  t.dest_dtype = f.src_dtype;
+  t.signature = "adjoint(" + f.tape.signature + ")";
+  return t;
+}
+}  // namespace nca
+
+namespace nca {
 // One march pass of the stencil-expression vjp; with fuse != 0 the pass also reduces the
 // expression's value (sum / max / min as the fused forward reductions do).
 static void stencil_expr_vjp_impl(uint64_t stencil_handle, uint64_t expr_handle, const int32_t* expr_operand,
@@ -1392,6 +1625,190 @@ static void stencil_expr_vjp_impl(uint64_t stencil_handle, uint64_t expr_handle,
     finish_call(stream);
   }
 }
+
+// Tile of the training-step march: TX x TY points per block, 64 x 16 (64 x 8 for 2-byte
+// sources: measured fastest at 256^3); NCA_HIP_TRAIN_TILE = "TXxTY" overrides it, for tuning.
+static void train_tile(int32_t src_dtype, int& tx, int& ty) {
+  tx = 64;
+  ty = itemsize(src_dtype) == 2 ? 8 : 16;
+  if (const char* e = std::getenv("NCA_HIP_TRAIN_TILE")) {
+    int a = 0, b = 0;
+    if (std::sscanf(e, "%dx%d", &a, &b) == 2 && (a == 32 || a == 64 || a == 128 || a == 256) &&
+        (b == 2 || b == 4 || b == 8 || b == 16 || b == 32)) {
+      tx = a;
+      ty = b;
+    }
+  }
+}
+
+// The whole training step of loss = sum(expr(stencil(src), extras)) in one pass
+// (gen_march_train): dx = stencil^T(d expr / d a * g), the extras' gradients and optionally the
+// reductions of the expression's value.
+static void stencil_expr_train_impl(uint64_t stencil_handle, uint64_t expr_handle, const int32_t* expr_operand,
+                                    int32_t n_operand, const nca_view* src, const nca_view* extras, int32_t n_extra,
+                                    const nca_scalar* stencil_scalars, int32_t n_stencil_scalars,
+                                    const nca_scalar* expr_scalars, int32_t n_expr_scalars, const void* grad_scalar,
+                                    const nca_view* grad_src, const nca_view* grad_extras, int32_t fuse, int32_t mode,
+                                    void* sum_out, void* max_out, void* min_out, int32_t boundary, void* stream) {
+  StencilProgram& p = stencil_program(stencil_handle);
+  const Tape& et = expr_program(expr_handle).tape;
+  NCA_REQUIRE(src && grad_src && (n_operand == 0 || expr_operand), "null argument");
+  NCA_REQUIRE(p.extra_dtypes.empty() && !p.has_ext, "the stencil of a stencil-expression step reads only its source");
+  NCA_REQUIRE(p.ndim == 3, "the stencil-expression step is 3-D");
+  NCA_REQUIRE(boundary == NCA_BOUNDARY_ZERO, "the stencil-expression step needs the zero boundary");
+  for (auto& o : p.offsets)
+    NCA_REQUIRE(o[0] >= -1 && o[0] <= 1 && o[1] >= -1 && o[1] <= 1 && o[2] >= -1 && o[2] <= 1,
+                "the stencil-expression step needs a stencil of radius <= 1 on every axis");
+  NCA_REQUIRE(!p.tape.uses_idx && !et.uses_idx, "the stencil-expression step does not evaluate IDX");
+  NCA_REQUIRE(n_operand == static_cast<int32_t>(et.arr_dtypes.size()), "one operand code per expression operand");
+  NCA_REQUIRE(n_extra >= 0 && n_extra <= 8 && (n_extra == 0 || extras), "0..8 extra arrays");
+  NCA_REQUIRE(et.compute_dtype == p.tape.compute_dtype, "the expression and the stencil must compute in one dtype");
+  NCA_REQUIRE((fuse & ~(NCA_FUSE_SUM | NCA_FUSE_MAX | NCA_FUSE_MIN)) == 0, "unknown fuse flags");
+  NCA_REQUIRE(mode == NCA_ACCUM_PLAIN || mode == NCA_ACCUM_COMPENSATED, "unknown accumulation mode");
+  NCA_REQUIRE((!(fuse & NCA_FUSE_SUM) || sum_out) && (!(fuse & NCA_FUSE_MAX) || max_out) &&
+                  (!(fuse & NCA_FUSE_MIN) || min_out),
+              "a fused reduction needs its output pointer");
+  validate_view(*src, "src");
+  validate_view(*grad_src, "grad_src");
+  NCA_REQUIRE(src->dtype == p.src_dtype, std::string("src dtype must be ") + dtype_info(p.src_dtype).name);
+  NCA_REQUIRE(src->ndim == 3 && same_shape(*src, *grad_src) && is_c_contiguous(*src) && is_c_contiguous(*grad_src),
+              "src and grad_src must be C-contiguous 3-D arrays of one shape");
+  NCA_REQUIRE(is_float(grad_src->dtype), "grad_src must be floating point");
+  const Tape adj = adjoint_tape(p);
+  MarchVjp vj;
+  vj.expr = et;
+  vj.value_dtype = et.dest_dtype;
+  vj.operand.assign(expr_operand, expr_operand + n_operand);
+  int first_a = -1;
+  for (int k = 0; k < n_operand; ++k) {
+    NCA_REQUIRE(vj.operand[k] >= -1 && vj.operand[k] < n_extra, "expression operand code out of range");
+    if (vj.operand[k] < 0 && first_a < 0) first_a = k;
+  }
+  NCA_REQUIRE(first_a >= 0, "the expression must read the stencil value");
+  for (int j = 0; j < n_extra; ++j) {
+    validate_view(extras[j], "extra");
+    NCA_REQUIRE(same_shape(extras[j], *src) && is_c_contiguous(extras[j]), "extras must be C-contiguous with src's shape");
+    vj.extra_dtypes.push_back(extras[j].dtype);
+  }
+  for (int k = 0; k < n_operand; ++k)
+    if (vj.operand[k] >= 0)
+      NCA_REQUIRE(et.arr_dtypes[k] == extras[vj.operand[k]].dtype, "extra dtype differs from the expression operand's");
+  vj.alias.resize(n_operand);
+  for (int k = 0; k < n_operand; ++k) {
+    vj.alias[k] = k;
+    for (int q = 0; q < k; ++q)
+      if (vj.operand[q] == vj.operand[k]) { vj.alias[k] = q; break; }
+  }
+  // root 0: d expr / d a (stays on chip); then d expr / d extra_j for every grad_extras[j] with data
+  std::vector<void*> outs;
+  vj.roots.push_back(differentiate(vj.expr, vj.expr.root, vj.alias[first_a], vj.alias));
+  vj.out_dtypes.push_back(p.tape.compute_dtype);
+  for (int j = 0; j < n_extra && grad_extras; ++j) {
+    if (!grad_extras[j].data) continue;
+    validate_view(grad_extras[j], "grad_extra");
+    NCA_REQUIRE(same_shape(grad_extras[j], *src) && is_c_contiguous(grad_extras[j]) && is_float(grad_extras[j].dtype),
+                "extra gradients must be C-contiguous floating-point arrays of src's shape");
+    int var = -1;
+    for (int k = 0; k < n_operand && var < 0; ++k)
+      if (vj.operand[k] == j) var = vj.alias[k];
+    vj.roots.push_back(var >= 0 ? differentiate(vj.expr, vj.expr.root, var, vj.alias) : -1);
+    vj.out_dtypes.push_back(grad_extras[j].dtype);
+    outs.push_back(grad_extras[j].data);
+  }
+  const int64_t NZ = src->shape[0], NY = src->shape[1], NX = src->shape[2];
+  NCA_REQUIRE(NZ < (int64_t(1) << 30), "too many planes");
+  NCA_REQUIRE(NY * NX < (int64_t(1) << 31), "plane too large");
+  const int64_t n = view_size(*src);
+  hipStream_t s = as_stream(stream);
+  if (n == 0) {
+    if (fuse) {
+      FinalizeArgs fa;
+      fa.value_dtype = et.dest_dtype;
+      fa.fuse = fuse;
+      fa.sum_out = sum_out;
+      fa.max_out = max_out;
+      fa.min_out = min_out;
+      launch_finalize_fused(fa, s);
+    }
+    finish_call(stream);
+    return;
+  }
+  int TXs = 64, TYs = 16;
+  train_tile(p.src_dtype, TXs, TYs);
+  DeviceState& dev = current_device();
+  std::string key = "train|" + et.signature + "|";
+  for (int o : vj.operand) key += std::to_string(o) + ",";
+  key += "|";
+  for (auto d : vj.extra_dtypes) key += std::to_string(d) + ",";
+  key += "|";
+  for (size_t m = 0; m < vj.roots.size(); ++m) key += std::to_string(vj.out_dtypes[m]) + (vj.roots[m] < 0 ? "z," : ",");
+  key += "|dx" + std::to_string(grad_src->dtype) + "|f" + std::to_string(fuse) + "," + std::to_string(fuse ? mode : 0) +
+         "|t" + std::to_string(TXs) + "x" + std::to_string(TYs);
+  const std::string kname = std::string("nca_stencil_train_") + dtype_info(p.src_dtype).name + (fuse ? "_sum" : "");
+  StencilFuse fz;
+  fz.with_dst = false;
+  fz.fuse = fuse;
+  fz.mode = mode;
+  hipFunction_t fn =
+      stencil_kernel(p, key, [&]() { return gen_march_train(p, adj, vj, fz, grad_src->dtype, TXs, TYs); }, kname);
+  const int ntx = static_cast<int>((NX + TXs - 1) / TXs), nty = static_cast<int>((NY + TYs - 1) / TYs);
+  const int64_t tiles = static_cast<int64_t>(ntx) * nty;
+  const int64_t resident = static_cast<int64_t>(dev.n_cus) * blocks_per_cu(fn);
+  int64_t nzc = std::max<int64_t>(1, std::min<int64_t>(resident / std::max<int64_t>(tiles, 1), NZ));
+  const int zlen = static_cast<int>((NZ + nzc - 1) / nzc);
+  nzc = (NZ + zlen - 1) / zlen;
+  const int64_t blocks = tiles * nzc;
+  NCA_REQUIRE(blocks < (int64_t(1) << 31), "grid too large");
+  ScalarSlots ss = make_scalar_slots(p.tape, stencil_scalars, n_stencil_scalars);
+  ScalarSlots es = make_scalar_slots(et, expr_scalars, n_expr_scalars);
+  std::deque<const void*> ptrs;
+  std::vector<void*> params;
+  ptrs.push_back(src->data);
+  params.push_back(&ptrs.back());
+  for (int j = 0; j < n_extra; ++j) { ptrs.push_back(extras[j].data); params.push_back(&ptrs.back()); }
+  ptrs.push_back(grad_src->data);
+  params.push_back(&ptrs.back());
+  for (void* o : outs) { ptrs.push_back(o); params.push_back(&ptrs.back()); }
+  int64_t dims[3] = {NZ, NY, NX};
+  for (auto& d : dims) params.push_back(&d);
+  int ints[3] = {ntx, nty, zlen};
+  for (auto& i : ints) params.push_back(&i);
+  void* psum = nullptr;
+  void* pmax = nullptr;
+  void* pmin = nullptr;
+  if (fuse) {
+    Workspace& ws = workspace(s, static_cast<size_t>(blocks) * 24);
+    psum = ws.partials;
+    pmax = static_cast<char*>(ws.partials) + static_cast<size_t>(blocks) * 8;
+    pmin = static_cast<char*>(ws.partials) + static_cast<size_t>(blocks) * 16;
+    params.push_back(&psum);
+    params.push_back(&pmax);
+    params.push_back(&pmin);
+  }
+  for (auto& x : ss.slots) params.push_back(&x);
+  for (auto& x : es.slots) params.push_back(&x);
+  ptrs.push_back(grad_scalar);
+  params.push_back(&ptrs.back());
+  {
+    ProfScope ps(kname.c_str(), s);
+    jit_launch(fn, dim3(static_cast<unsigned>(blocks)), dim3(256), 0, s, params);
+  }
+  check_launch("stencil_expr_train");
+  if (fuse) {
+    FinalizeArgs fa;
+    fa.value_dtype = et.dest_dtype;
+    fa.fuse = fuse;
+    fa.psum = psum;
+    fa.pmax = pmax;
+    fa.pmin = pmin;
+    fa.nblocks = static_cast<int>(blocks);
+    fa.sum_out = sum_out;
+    fa.max_out = max_out;
+    fa.min_out = min_out;
+    launch_finalize_fused(fa, s);
+  }
+  finish_call(stream);
+}
 }  // namespace nca
 
 extern "C" {
@@ -1406,6 +1823,71 @@ int nca_hip_stencil_expr_vjp(uint64_t stencil_handle, uint64_t expr_handle, cons
                           n_stencil_scalars, expr_scalars, n_expr_scalars, grad_scalar, w, grad_extras, 0,
                           NCA_ACCUM_PLAIN, nullptr, nullptr, nullptr, boundary, stream);
   });
+}
+
+int nca_hip_stencil_expr_train(uint64_t stencil_handle, uint64_t expr_handle, const int32_t* expr_operand,
+                               int32_t n_operand, const nca_view* src, const nca_view* extras, int32_t n_extra,
+                               const nca_scalar* stencil_scalars, int32_t n_stencil_scalars,
+                               const nca_scalar* expr_scalars, int32_t n_expr_scalars, const void* grad_scalar,
+                               const nca_view* grad_src, const nca_view* grad_extras, int32_t fuse, int32_t mode,
+                               void* sum_out, void* max_out, void* min_out, int32_t boundary, void* stream) {
+  return api_call([&] {
+    stencil_expr_train_impl(stencil_handle, expr_handle, expr_operand, n_operand, src, extras, n_extra,
+                            stencil_scalars, n_stencil_scalars, expr_scalars, n_expr_scalars, grad_scalar, grad_src,
+                            grad_extras, fuse, mode, sum_out, max_out, min_out, boundary, stream);
+  });
+}
+
+int nca_hip_stencil_expr_train_source(uint64_t stencil_handle, uint64_t expr_handle, const int32_t* expr_operand,
+                                      int32_t n_operand, const int32_t* extra_dtypes, int32_t n_extra,
+                                      int32_t grad_src_dtype, int32_t fuse, int32_t accum_mode, char* buf, size_t n) {
+  std::string src;
+  int rc = api_call([&] {
+    StencilProgram& p = stencil_program(stencil_handle);
+    const Tape& et = expr_program(expr_handle).tape;
+    NCA_REQUIRE(p.ndim == 3 && p.extra_dtypes.empty() && !p.has_ext, "a 3-D stencil that reads only its source");
+    NCA_REQUIRE(n_operand == static_cast<int32_t>(et.arr_dtypes.size()) && (n_operand == 0 || expr_operand),
+                "one operand code per expression operand");
+    NCA_REQUIRE(n_extra >= 0 && n_extra <= 8 && (n_extra == 0 || extra_dtypes), "0..8 extra dtypes");
+    MarchVjp vj;
+    vj.expr = et;
+    vj.value_dtype = et.dest_dtype;
+    vj.operand.assign(expr_operand, expr_operand + n_operand);
+    vj.extra_dtypes.assign(extra_dtypes, extra_dtypes + n_extra);
+    vj.alias.resize(n_operand);
+    int first_a = -1;
+    for (int k = 0; k < n_operand; ++k) {
+      NCA_REQUIRE(vj.operand[k] >= -1 && vj.operand[k] < n_extra, "expression operand code out of range");
+      if (vj.operand[k] < 0 && first_a < 0) first_a = k;
+      vj.alias[k] = k;
+      for (int q = 0; q < k; ++q)
+        if (vj.operand[q] == vj.operand[k]) { vj.alias[k] = q; break; }
+    }
+    NCA_REQUIRE(first_a >= 0, "the expression must read the stencil value");
+    vj.roots.push_back(differentiate(vj.expr, vj.expr.root, vj.alias[first_a], vj.alias));
+    vj.out_dtypes.push_back(p.tape.compute_dtype);
+    for (int j = 0; j < n_extra; ++j) {
+      int var = -1;
+      for (int k = 0; k < n_operand && var < 0; ++k)
+        if (vj.operand[k] == j) var = vj.alias[k];
+      vj.roots.push_back(var >= 0 ? differentiate(vj.expr, vj.expr.root, var, vj.alias) : -1);
+      vj.out_dtypes.push_back(extra_dtypes[j]);
+    }
+    StencilFuse fz;
+    fz.with_dst = false;
+    fz.fuse = fuse;
+    fz.mode = accum_mode;
+    int TXs = 64, TYs = 16;
+    train_tile(p.src_dtype, TXs, TYs);
+    src = device_prelude() + "\n" + gen_march_train(p, adjoint_tape(p), vj, fz, grad_src_dtype, TXs, TYs);
+  });
+  if (rc != NCA_OK) return -rc;
+  if (buf && n > 0) {
+    size_t k = src.size() < n - 1 ? src.size() : n - 1;
+    std::memcpy(buf, src.data(), k);
+    buf[k] = '\0';
+  }
+  return static_cast<int>(src.size());
 }
 
 int nca_hip_stencil_expr_vjp_reduce(uint64_t stencil_handle, uint64_t expr_handle, const int32_t* expr_operand,
@@ -1426,41 +1908,12 @@ int nca_hip_stencil_adjoint_create(uint64_t stencil_handle, uint64_t* handle) {
   return api_call([&] {
     NCA_REQUIRE(handle != nullptr, "null handle");
     StencilProgram& f = stencil_program(stencil_handle);
-    NCA_REQUIRE(!f.has_ext, "cannot differentiate a stencil with extension hooks");
     auto p = std::make_unique<StencilProgram>();
     p->ndim = f.ndim;
     p->n_off = f.n_off;
     p->src_dtype = f.tape.dest_dtype;  // the adjoint reads gradients of the destination
     p->forward = stencil_handle;
-    Tape base = f.tape;
-    std::vector<int> alias(base.arr_dtypes.size());
-    for (size_t k = 0; k < alias.size(); ++k) alias[k] = static_cast<int>(k);
-    // c_k = d f / d v_k must not depend on any load (linear stencil)
-    std::vector<int> coef;
-    for (int k = 0; k < f.n_off; ++k) {
-      int dk = differentiate(base, base.root, k, alias);
-      NCA_REQUIRE(!depends_on_loads(base, dk),
-                  "the stencil is not linear in its source loads; its adjoint is not a stencil");
-      coef.push_back(dk);
-    }
-    // adjoint tape: sum_k c_k * LOAD k  (offsets negated)
-    Tape t = base;
-    t.arr_dtypes.assign(f.n_off, p->src_dtype);
-    int acc = -1;
-    for (int k = 0; k < f.n_off; ++k) {
-      if (coef[k] < 0) continue;
-      Node ld;
-      ld.op = NCA_OP_LOAD_NCARR;
-      ld.index = k;
-      int term = t.bin(NCA_OP_MUL, coef[k], t.add(ld));
-      acc = acc < 0 ? term : t.bin(NCA_OP_ADD, acc, term);
-    }
-    NCA_REQUIRE(acc >= 0, "the stencil does not depend on its source");
-    t.root = acc;
-    t.uses_idx = false;
-    t.dest_dtype = f.src_dtype;
-    t.signature = "adjoint(" + f.tape.signature + ")";
-    p->tape = std::move(t);
+    p->tape = adjoint_tape(f);
     for (auto& o : f.offsets) {
       std::array<int64_t, NCA_MAX_NDIM> no{};
       for (int d = 0; d < f.ndim; ++d) no[d] = -o[d];

@@ -167,3 +167,21 @@ def test_adjoint_source_negates_offsets():
 def test_stencil_rejects_bad_descriptions():
     with pytest.raises(capi.NcaError):
         capi.Stencil(tapes.LAPLACIAN_OFFSETS, tapes.laplacian(), capi.FLOAT32, capi.FLOAT32, capi.FLOAT32, [])
+
+
+@pytest.mark.parametrize("dt", [capi.FLOAT32, capi.BFLOAT16])
+def test_training_step_kernel_compiles(dt):
+    """The one-pass training kernel (loss, dx via the adjoint from LDS, dy) for the Laplacian
+    and the C0 expression: the stencil loads read x's three LDS planes, the adjoint reads w at
+    the negated offsets, and the loss is reduced only with fuse."""
+    from ncarray_amd import autograd as A
+    lap, _ = A.laplacian_stencil(dt, capi.FLOAT32)
+    tape = A.c0_tape(capi.FLOAT32, dt, capi.FLOAT32)
+    plain = lap.train_source(tape, [-1, 0, -1, -1, 0], [dt], dt)
+    fused = lap.train_source(tape, [-1, 0, -1, -1, 0], [dt], dt, fuse=capi.FUSE_SUM | capi.FUSE_MAX,
+                             mode=capi.ACCUM_COMPENSATED)
+    for src in (plain, fused):
+        _compiles(src)
+        assert "xs[sm][xb[m] + (0)]" in src and "xs[sp][xb[m] + (0)]" in src   # the z neighbours
+        assert "ws[a2][tb[m] + (0)]" in src and "ws[a0][tb[m] + (0)]" in src   # w at z + 1, z - 1
+    assert "nca_kahan_add(acc_s" in fused and "acc_s" not in plain and "nca_maxnum(acc_mx" in fused

@@ -255,15 +255,15 @@ def test_stencil_expr_vjp_reduce_empty_and_rejects(dev):
                   capi.BOUNDARY_ZERO, None)
 
 
-def test_value_and_grad_fallback_shape(dev):
-    """laplacian_c0_value_and_grad on a shape the march kernel does not take (the fused forward
-    plus the three-kernel backward) agrees with FP64."""
+def test_value_and_grad_ragged_shape(dev):
+    """laplacian_c0_value_and_grad on a shape the vectorised march kernels do not take (last
+    extent not a whole number of vectors): the one-pass training kernel agrees with FP64."""
     A, capi, tapes = _mods()
     rng = np.random.default_rng(6)
     x = rng.standard_normal((12, 14, 18)).astype(np.float32)
     y = rng.standard_normal((12, 14, 18)).astype(np.float32)
     xt, yt = torch.from_numpy(x).to(dev), torch.from_numpy(y).to(dev)
-    assert not A._march_vjp_ok(xt, yt)
+    assert not A._march_vjp_ok(xt, yt) and A._train_ok(xt, yt)
     loss, dx, dy = A.laplacian_c0_value_and_grad(xt, yt)
     torch.cuda.synchronize()
     gx64, gy64 = grad64(x, y)
@@ -272,20 +272,94 @@ def test_value_and_grad_fallback_shape(dev):
     assert np.allclose(dy.cpu().numpy(), gy64, rtol=1e-5, atol=1e-5)
 
 
-def test_fused_loss_backward_fallback_shape(dev):
-    """A last extent that is not a whole number of vectors takes the three-kernel backward;
-    both backward paths agree with the FP64 gradient."""
+@pytest.mark.parametrize("method", ["train", "compose"])
+def test_fused_loss_backward_ragged_shape(dev, method):
+    """A last extent that is not a whole number of vectors: the one-pass training backward (the
+    default) and the three-kernel composition agree with the FP64 gradient."""
     A, capi, tapes = _mods()
     rng = np.random.default_rng(4)
     x = rng.standard_normal((16, 20, 18)).astype(np.float32)
     y = rng.standard_normal((16, 20, 18)).astype(np.float32)
-    xt = torch.from_numpy(x).to(dev).requires_grad_()
-    yt = torch.from_numpy(y).to(dev).requires_grad_()
-    assert not A._march_vjp_ok(xt.detach(), yt.detach())
-    A.laplacian_c0_loss(xt, yt).backward()
+    xt, yt = torch.from_numpy(x).to(dev), torch.from_numpy(y).to(dev)
+    assert not A._march_vjp_ok(xt, yt)
+    dx, dy = A.laplacian_c0_backward(xt, yt, method=method)
+    torch.cuda.synchronize()
     gx64, gy64 = grad64(x, y)
-    assert np.max(np.abs(xt.grad.cpu().numpy() - gx64)) <= 1e-4 * max(1.0, np.max(np.abs(gx64)))
-    assert np.max(np.abs(yt.grad.cpu().numpy() - gy64)) <= 1e-5 * max(1.0, np.max(np.abs(gy64)))
+    assert np.max(np.abs(dx.cpu().numpy() - gx64)) <= 1e-4 * max(1.0, np.max(np.abs(gx64)))
+    assert np.max(np.abs(dy.cpu().numpy() - gy64)) <= 1e-5 * max(1.0, np.max(np.abs(gy64)))
+    if method == "train":   # the autograd Function runs the same kernel
+        xa, ya = xt.clone().requires_grad_(), yt.clone().requires_grad_()
+        A.laplacian_c0_loss(xa, ya).backward()
+        assert torch.equal(xa.grad, dx) and torch.equal(ya.grad, dy)
+
+
+@pytest.mark.parametrize("dtype", [torch.float32, torch.bfloat16])
+def test_stencil_expr_train_matches_two_pass(dev, dtype):
+    """nca_hip_stencil_expr_train (one pass: w on tiles and rings in LDS, the transposed
+    Laplacian from LDS) gives the bits of the two-kernel backward (w through memory) with an
+    upstream gradient other than 1; its fused loss is within the accumulation's bound of FP64
+    and equals the forward's fused sum to that bound."""
+    A, capi, tapes = _mods()
+    g = torch.Generator(device=dev).manual_seed(13)
+    x = torch.randn((40, 36, 80), generator=g, device=dev).to(dtype)   # several z runs per tile column
+    y = torch.randn((40, 36, 80), generator=g, device=dev).to(dtype)
+    gs = torch.tensor(1.75, dtype=torch.float32, device=dev)
+    dx1, dy1 = A.laplacian_c0_backward(x, y, gs, method="train")
+    dx2, dy2 = A.laplacian_c0_backward(x, y, gs, method="vjp")
+    torch.cuda.synchronize()
+    assert torch.equal(dy1, dy2)
+    assert torch.equal(dx1, dx2)
+    loss, dx3, dy3 = A.laplacian_c0_value_and_grad(x, y)
+    torch.cuda.synchronize()
+    xn, yn = x.float().cpu().numpy(), y.float().cpu().numpy()
+    a = lap64(xn)
+    mass = float(np.sum(np.abs(a * yn) + np.exp(-a * a) + 0.5 * np.abs(yn)))
+    assert abs(loss.item() - loss64(xn, yn)) <= 8 * np.finfo(np.float32).eps * mass
+    gx64, gy64 = grad64(xn, yn)
+    tol = 1e-4 if dtype == torch.float32 else 2e-2
+    assert np.max(np.abs(dx3.float().cpu().numpy() - gx64)) <= tol * max(1.0, np.max(np.abs(gx64)))
+    assert np.max(np.abs(dy3.float().cpu().numpy() - gy64)) <= tol * max(1.0, np.max(np.abs(gy64)))
+
+
+@pytest.mark.parametrize("shape", [(1, 5, 7), (3, 1, 130), (9, 17, 1), (2, 33, 65), (70, 3, 3)])
+def test_stencil_expr_train_edge_shapes(dev, shape):
+    """Single planes, single rows and columns, partial tiles and short z runs: the training
+    pass equals the three-kernel composition (the same arithmetic per element)."""
+    A, capi, tapes = _mods()
+    rng = np.random.default_rng(sum(shape))
+    x = rng.standard_normal(shape).astype(np.float32)
+    y = rng.standard_normal(shape).astype(np.float32)
+    xt, yt = torch.from_numpy(x).to(dev), torch.from_numpy(y).to(dev)
+    dx1, dy1 = A.laplacian_c0_backward(xt, yt, method="train")
+    dx2, dy2 = A.laplacian_c0_backward(xt, yt, method="compose")
+    torch.cuda.synchronize()
+    assert torch.equal(dy1, dy2)
+    assert torch.allclose(dx1, dx2, rtol=0, atol=4 * float(np.finfo(np.float32).eps) * max(1.0, float(dx2.abs().max())))
+    gx64, _ = grad64(x, y)
+    assert np.max(np.abs(dx1.cpu().numpy() - gx64)) <= 1e-4 * max(1.0, np.max(np.abs(gx64)))
+
+
+def test_stencil_expr_train_rejects(dev):
+    A, capi, tapes = _mods()
+    from ncarray_amd import torch_interop as ti
+    x = torch.randn((8, 8, 8), device=dev)
+    y = torch.randn((8, 8, 8), device=dev)
+    dx = torch.empty_like(x)
+    tape = A.c0_tape(capi.FLOAT32, capi.FLOAT32, capi.FLOAT32)
+    args = dict(extras=[ti.view(y)], scalars=(6.0,), expr_scalars=(0.5,))
+    wide = capi.Stencil([[0, 0, -2], [0, 0, 0]], [tapes.P(capi.OP_LOAD_NCARR, 0), tapes.P(capi.OP_LOAD_NCARR, 1),
+                                                   tapes.P(capi.OP_ADD)], capi.FLOAT32, capi.FLOAT32, capi.FLOAT32)
+    with pytest.raises(capi.NcaError, match="radius"):
+        wide.expr_train(tape, [-1, 0, -1, -1, 0], ti.view(x), ti.view(dx), **args)
+    square = capi.Stencil([[0, 0, 0]], [tapes.P(capi.OP_LOAD_NCARR, 0), tapes.P(capi.OP_LOAD_NCARR, 0),
+                                        tapes.P(capi.OP_MUL)], capi.FLOAT32, capi.FLOAT32, capi.FLOAT32)
+    with pytest.raises(capi.NcaError, match="not linear"):
+        square.expr_train(tape, [-1, 0, -1, -1, 0], ti.view(x), ti.view(dx), **args)
+    lap, _ = A.laplacian_stencil(capi.FLOAT32, capi.FLOAT32)
+    with pytest.raises(capi.NcaError, match="zero boundary"):
+        lap.expr_train(tape, [-1, 0, -1, -1, 0], ti.view(x), ti.view(dx), boundary=capi.BOUNDARY_NONE, **args)
+    with pytest.raises(capi.NcaError, match="output pointer"):
+        lap.expr_train(tape, [-1, 0, -1, -1, 0], ti.view(x), ti.view(dx), fuse=capi.FUSE_SUM, **args)
 
 
 def test_stencil_expr_vjp_rejects(dev):
