@@ -360,12 +360,14 @@ int nca_hip_stencil_expr_train(uint64_t stencil_handle, uint64_t expr_handle,
                                void* stream);
 /* The generated source of that pass for the given operand codes, extra dtypes, grad_src
  * dtype and reductions (every extra's gradient written), for inspection and
- * nca_hip_rtc_compile.  Returns the text length (negative status on error). */
+ * nca_hip_rtc_compile; vec != 0 gives the 4-point-vector form the pass uses when the last
+ * extent is a multiple of 4 and every array is aligned to whole vectors.  Returns the text
+ * length (negative status on error). */
 int nca_hip_stencil_expr_train_source(uint64_t stencil_handle, uint64_t expr_handle,
                                       const int32_t* expr_operand, int32_t n_operand,
                                       const int32_t* extra_dtypes, int32_t n_extra,
                                       int32_t grad_src_dtype, int32_t fuse, int32_t accum_mode,
-                                      char* buf, size_t n);
+                                      int32_t vec, char* buf, size_t n);
 /* The generated source of a stencil kernel (for inspection and nca_hip_rtc_compile):
  * variant 0 the march kernel, 1 the march with the fused axis-0 sum and max, 4 the same
  * with compensated sums, 2 / 3 the direct kernel without / with the zero boundary.

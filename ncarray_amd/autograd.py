@@ -264,18 +264,26 @@ def _train_ok(x, y):
     return x.dim() == 3 and x.is_contiguous() and y.is_contiguous() and tuple(x.shape) == tuple(y.shape)
 
 
+def _default_method(x, y):
+    if x.element_size() == 2 and _march_vjp_ok(x, y):
+        return "vjp"
+    return "train" if _train_ok(x, y) else ("vjp" if _march_vjp_ok(x, y) else "compose")
+
+
 def laplacian_c0_backward(x, y, g=None, method=None):
     """(dx, dy) of sum(C0(Laplacian(x), y)) times the 0-d upstream gradient g (None: 1).
 
-    method "vjp" (the default where the vectorised march kernels apply: 3-D, last extent a whole
-    number of 16-byte vectors, aligned arrays): nca_hip_stencil_expr_vjp writes w = dC0/da * g
-    and dy = dC0/db * g in one march pass and the transposed stencil turns w into dx.  "train"
-    (the default for other 3-D C-contiguous arrays): one pass (nca_hip_stencil_expr_train)
-    stages x with a two-point halo in LDS, evaluates w on each tile and a one-point ring, writes
-    dy and applies the transposed Laplacian to w from LDS; it moves fewer bytes but loads one
-    element per lane, and measured no faster at 256^3 (f32 72 vs 72.5 us, BF16 97 vs 58 us), so
-    it serves the shapes the march kernels do not take.  "compose": the Laplacian, the
-    derivative expression and the transposed stencil as three kernels (any shape)."""
+    method "train": one pass (nca_hip_stencil_expr_train) stages x with a two-point halo in LDS,
+    evaluates w = dC0/da * g on each tile and a one-point ring, writes dy = dC0/db * g and
+    applies the transposed Laplacian to w from LDS, so only x, y, dx and dy touch memory (4-point
+    vectors when the last extent is a multiple of 4 and the arrays are aligned).  "vjp": the
+    two-kernel form (nca_hip_stencil_expr_vjp writes w and dy in one march pass, the
+    transposed stencil turns w into dx; the vectorised march kernels' shape and alignment rules
+    apply).  "compose": the Laplacian, the derivative expression and the transposed stencil as
+    three kernels (any shape).  The default is the fastest measured at 256^3: "train" for
+    4-byte types (61.8 us against 45.5 + 31.8 us for "vjp"), "vjp" for 2-byte types where it
+    applies (34.6 + 27.1 us against 64.4 us: the one-pass kernel's arithmetic and LDS traffic
+    are those of float32), else "train" for 3-D C-contiguous arrays, else "compose"."""
     s = _stream()
     xcode, ycode = _code(x), _code(y)
     lap, adj = laplacian_stencil(xcode, capi.FLOAT32)
@@ -286,7 +294,7 @@ def laplacian_c0_backward(x, y, g=None, method=None):
         gf = g if g.dtype == torch.float32 else g.float()
     gptr = None if gf is None else gf.data_ptr()
     if method is None:
-        method = "vjp" if _march_vjp_ok(x, y) else ("train" if _train_ok(x, y) else "compose")
+        method = _default_method(x, y)
     if method == "train":
         dx = torch.empty(x.shape, dtype=x.dtype, device=x.device)
         gy = torch.empty_like(y)
@@ -328,17 +336,18 @@ def laplacian_c0_value_and_grad(x, y, compensated=True, grad_scale=None):
     """(loss, dx, dy) of loss = sum(C0(Laplacian(x), y)) without torch's autograd, all launched
     from the calling thread on its current stream, so a whole training step can be captured in
     one hipGraph.  The loss is fused into the gradient pass (per-thread float32, Kahan when
-    compensated, float64 across blocks): where the vectorised march kernels apply,
-    nca_hip_stencil_expr_vjp_reduce (loss, dloss/da, dy) and the transposed stencil (dx); for
-    other 3-D arrays the one-pass nca_hip_stencil_expr_train; otherwise the fused forward kernel
-    and laplacian_c0_backward.  grad_scale: optional 0-d float32 device tensor scaling the
-    gradients (the upstream gradient; default 1)."""
+    compensated, float64 across blocks), on the path laplacian_c0_backward would take: the
+    one-pass nca_hip_stencil_expr_train, or nca_hip_stencil_expr_vjp_reduce (loss, dloss/da,
+    dy) and the transposed stencil (dx); for arrays neither takes, the fused forward kernel and
+    laplacian_c0_backward.  grad_scale: optional 0-d float32 device tensor scaling the gradients
+    (the upstream gradient; default 1)."""
     _init(x)
     xc, yc = x.detach().contiguous(), y.detach().contiguous()
     loss = torch.empty((), dtype=torch.float32, device=x.device)
     mode = capi.ACCUM_COMPENSATED if compensated else capi.ACCUM_PLAIN
     gf = None if grad_scale is None else (grad_scale if grad_scale.dtype == torch.float32 else grad_scale.float())
-    if _march_vjp_ok(xc, yc):
+    method = _default_method(xc, yc)
+    if method == "vjp":
         s = _stream()
         lap, adj = laplacian_stencil(_code(xc), capi.FLOAT32)
         tape = c0_tape(capi.FLOAT32, _code(yc), capi.FLOAT32)
@@ -351,7 +360,7 @@ def laplacian_c0_value_and_grad(x, y, compensated=True, grad_scale=None):
         dx = torch.empty(xc.shape, dtype=xc.dtype, device=x.device)
         adj.apply(vw, ti.view(dx), capi.BOUNDARY_ZERO, scalars=(6.0,), stream=s)
         return loss, dx, dy
-    if _train_ok(xc, yc):
+    if method == "train":
         lap, _ = laplacian_stencil(_code(xc), capi.FLOAT32)
         tape = c0_tape(capi.FLOAT32, _code(yc), capi.FLOAT32)
         dx = torch.empty(xc.shape, dtype=xc.dtype, device=x.device)

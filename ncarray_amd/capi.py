@@ -133,7 +133,7 @@ _SIGS = {
     "nca_hip_stencil_expr_train": [_U64, _U64, ctypes.POINTER(_I32), _I32, _VP, _VP, _I32, _SP, _I32, _SP, _I32,
                                    _P, _VP, _VP, _I32, _I32, _P, _P, _P, _I32, _P],
     "nca_hip_stencil_expr_train_source": [_U64, _U64, ctypes.POINTER(_I32), _I32, ctypes.POINTER(_I32), _I32, _I32,
-                                          _I32, _I32, ctypes.c_char_p, _SZ],
+                                          _I32, _I32, _I32, ctypes.c_char_p, _SZ],
     "nca_hip_stencil_source": [_U64, _I32, ctypes.c_char_p, _SZ],
     "nca_hip_scatter_add": [_VP, _VP, _VP, _P],
     "nca_hip_bin": [_VP, _VP, _I32, _I32, _VP, _VP, _P],
@@ -392,10 +392,11 @@ class Stencil:
              len(extras), sa, len(scalars), xa, len(expr_scalars), grad_scalar, ctypes.byref(grad_src), ga, fuse, mode,
              sum_out, max_out, min_out, boundary, stream)
 
-    def train_source(self, tape, operands, extra_dtypes, grad_src_dtype, fuse=0, mode=ACCUM_PLAIN):
+    def train_source(self, tape, operands, extra_dtypes, grad_src_dtype, fuse=0, mode=ACCUM_PLAIN, vec=False):
         ops = (_I32 * max(1, len(operands)))(*operands)
         xd = (_I32 * max(1, len(extra_dtypes)))(*extra_dtypes)
-        args = (self.handle, tape.handle, ops, len(operands), xd, len(extra_dtypes), grad_src_dtype, fuse, mode)
+        args = (self.handle, tape.handle, ops, len(operands), xd, len(extra_dtypes), grad_src_dtype, fuse, mode,
+                1 if vec else 0)
         n = lib.nca_hip_stencil_expr_train_source(*args, None, 0)
         if n < 0:
             check(-n)

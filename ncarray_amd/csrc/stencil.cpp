@@ -989,6 +989,261 @@ static std::string gen_march_train(const StencilProgram& p, const Tape& adj, con
   return os.str();
 }
 
+// The training-step march with 4-point vectors (last extent a multiple of 4, arrays aligned to
+// whole vectors): each thread holds aligned 4-point vectors of x, the extras, w, dx and the
+// gradients.  w is evaluated on the columns x0-4 .. x0+TX+3 (aligned; the ring needs x0-1 ..
+// x0+TX) from x staged on x0-8 .. x0+TX+7; neighbours come from one aligned LDS vector per
+// (plane, row) plus the scalars left and right of it, as in the queue march.  The arithmetic
+// per element is gen_march_train's.
+static std::string gen_march_train_vec(const StencilProgram& p, const Tape& adj, const MarchVjp& vj, const StencilFuse& fz,
+                                       int32_t dx_dtype, int TXs, int TYs) {
+  const Tape& t = p.tape;
+  const std::string C = ctype_of(t.compute_dtype);
+  const std::string S = ctype_of(p.src_dtype);
+  const std::string DX = ctype_of(dx_dtype);
+  const std::string VD = ctype_of(vj.value_dtype);
+  const int32_t rcd = compute_dtype(vj.value_dtype);
+  const std::string RC = ctype_of(rcd);
+  const bool comp = fz.mode == NCA_ACCUM_COMPENSATED;
+  const int n_off = p.n_off;
+  const int n_x = static_cast<int>(vj.extra_dtypes.size());
+  auto I = [](long long v) { return std::to_string(v); };
+  std::vector<int> alias(t.arr_dtypes.size());
+  for (size_t k = 0; k < alias.size(); ++k) alias[k] = static_cast<int>(k);
+  std::vector<std::string> sroots;
+  const std::string sbody = emit_nodes(t, {t.root}, alias, names_u(static_cast<int>(t.arr_dtypes.size())),
+                                       names_s(static_cast<int>(t.scalar_dtypes.size())), "lin", "t", sroots);
+  std::vector<std::string> en(vj.operand.size()), rn(vj.expr.scalar_dtypes.size());
+  for (size_t k = 0; k < en.size(); ++k) en[k] = vj.operand[k] < 0 ? sroots[0] : "u" + I(n_off + vj.operand[k]);
+  for (size_t i = 0; i < rn.size(); ++i) rn[i] = "r" + I(i);
+  std::vector<int> live;
+  for (int r : vj.roots)
+    if (r >= 0) live.push_back(r);
+  if (fz.fuse) live.push_back(vj.expr.root);
+  std::vector<std::string> ln;
+  const std::string vbody = emit_nodes(vj.expr, live, vj.alias, en, rn, "lin", "q", ln);
+  std::vector<std::string> vnames;
+  size_t li = 0;
+  for (int r : vj.roots) vnames.push_back(r >= 0 ? ln[li++] : "(" + C + ")0");
+  const std::string pname = fz.fuse ? ln[li] : "";
+  std::vector<std::string> vn;
+  for (int k = 0; k < n_off; ++k) vn.push_back("v" + I(k));
+  std::vector<int> aalias(adj.arr_dtypes.size());
+  for (size_t k = 0; k < aalias.size(); ++k) aalias[k] = static_cast<int>(k);
+  std::vector<std::string> aroots;
+  const std::string abody = emit_nodes(adj, {adj.root}, aalias, vn, names_s(static_cast<int>(adj.scalar_dtypes.size())),
+                                       "lin", "a", aroots);
+  const int XW = TXs + 16, XH = TYs + 4, RWV = TXs + 8, RH = TYs + 2;
+  const int XV = XW / 4, WV = RWV / 4, TV = TXs / 4;
+  const std::string kname =
+      std::string("nca_stencil_train_") + dtype_info(p.src_dtype).name + (fz.fuse ? "_sum" : "") + "_v4";
+  std::ostringstream os;
+  os << "#define TX " << TXs << "\n#define TY " << TYs << "\n#define XW " << XW << "\n#define XH " << XH << "\n#define RWV "
+     << RWV << "\n#define RH " << RH << "\n#define XV " << XV << "\n#define WV " << WV << "\n#define TV " << TV << "\n"
+     << "#define NXL ((XV * XH + 255) / 256)\n#define NWL ((WV * RH + 255) / 256)\n#define NTL ((TV * TY + 255) / 256)\n";
+  os << "__device__ __forceinline__ int nca_mod3(int z) { const int r = z % 3; return r < 0 ? r + 3 : r; }\n";
+  os << "extern \"C\" __global__ __launch_bounds__(256) void " << kname << "(const " << S << "* __restrict__ src";
+  for (int j = 0; j < n_x; ++j) os << ", const " << ctype_of(vj.extra_dtypes[j]) << "* __restrict__ e" << j;
+  os << ", " << DX << "* __restrict__ dxo";
+  for (size_t m = 1; m < vj.out_dtypes.size(); ++m) os << ", " << ctype_of(vj.out_dtypes[m]) << "* __restrict__ o" << m;
+  os << ", nca_i64 NZ, nca_i64 NY, nca_i64 NX, int ntx, int nty, int zlen";
+  if (fz.fuse) os << ", void* __restrict__ psum, void* __restrict__ pmax, void* __restrict__ pmin";
+  for (size_t i = 0; i < t.scalar_dtypes.size(); ++i) os << ", " << C << " s" << i;
+  for (size_t i = 0; i < vj.expr.scalar_dtypes.size(); ++i) os << ", " << C << " r" << i;
+  os << ", const " << C << "* __restrict__ gin) {\n";
+  os << "  const " << C << " gsc = gin ? *gin : (" << C << ")1;\n";
+  os << "  struct __attribute__((aligned(16))) lvec { " << C << " v[4]; };\n";
+  os << "  __shared__ __attribute__((aligned(16))) " << C << " xs[3][XH * XW];\n"
+     << "  __shared__ __attribute__((aligned(16))) " << C << " ws[3][RH * RWV];\n";
+  for (int j = 0; j < n_x; ++j) os << "  __shared__ __attribute__((aligned(16))) " << C << " es" << j << "[RH * RWV];\n";
+  os << "  const unsigned nb = gridDim.x, b = blockIdx.x;\n"
+     << "  const unsigned tix = (nb % 8u == 0u) ? (b % 8u) * (nb / 8u) + b / 8u : b;\n"
+     << "  const int txi = tix % ntx, tyi = (tix / ntx) % nty, tzi = tix / (ntx * nty);\n"
+     << "  const int x0 = txi * TX, y0 = tyi * TY, NYi = (int)NY, NXi = (int)NX;\n"
+     << "  const int zb = (int)((nca_i64)tzi * zlen), ze = (int)((zb + (nca_i64)zlen < NZ) ? zb + zlen : NZ);\n"
+     << "  const size_t plane = (size_t)NY * (size_t)NX;\n  const int NZi = (int)NZ;\n  const int tid = threadIdx.x;\n";
+  // per-thread vector positions (element offsets in a plane; -1: outside the array, where a
+  // whole vector is in or out because NX and the column origins are multiples of 4)
+  os << "  int xo[NXL];\n#pragma unroll\n  for (int m = 0; m < NXL; ++m) {\n"
+     << "    const int i = tid + 256 * m, r = i / XV, c = i - r * XV, gy = y0 - 2 + r, gx = x0 - 8 + 4 * c;\n"
+     << "    xo[m] = (i < XV * XH && gy >= 0 && gy < NYi && gx >= 0 && gx < NXi) ? gy * NXi + gx : -1;\n  }\n";
+  os << "  int go[NWL], xb[NWL], wb[NWL];\n  unsigned own = 0u;\n#pragma unroll\n  for (int m = 0; m < NWL; ++m) {\n"
+     << "    const int i = tid + 256 * m, r = i / WV, c = i - r * WV, gy = y0 - 1 + r, gx = x0 - 4 + 4 * c;\n"
+     << "    const bool in = i < WV * RH && gy >= 0 && gy < NYi && gx >= 0 && gx < NXi;\n"
+     << "    go[m] = in ? gy * NXi + gx : -1;\n    xb[m] = (r + 1) * XW + 4 + 4 * c;\n    wb[m] = r * RWV + 4 * c;\n"
+     << "    if (in && r >= 1 && r <= TY && c >= 1 && c <= TV) own |= 1u << m;\n  }\n";
+  os << "  int to[NTL], tb[NTL];\n#pragma unroll\n  for (int m = 0; m < NTL; ++m) {\n"
+     << "    const int i = tid + 256 * m, r = i / TV, c = i - r * TV, gy = y0 + r, gx = x0 + 4 * c;\n"
+     << "    to[m] = (i < TV * TY && gy < NYi && gx < NXi) ? gy * NXi + gx : -1;\n    tb[m] = (r + 1) * RWV + 4 + 4 * c;\n  }\n";
+  os << "  nca_vec<" << S << ", 4> xr[NXL];\n";
+  for (int j = 0; j < n_x; ++j) os << "  nca_vec<" << ctype_of(vj.extra_dtypes[j]) << ", 4> er" << j << "[NWL];\n";
+  os << "  auto load_x = [&](int z) {\n    const bool zin = (unsigned)z < (unsigned)NZi;\n"
+     << "    const " << S << "* __restrict__ pl = src + (zin ? (size_t)z * plane : (size_t)0);\n"
+     << "#pragma unroll\n    for (int m = 0; m < NXL; ++m) {\n"
+     << "      const bool ok = zin && xo[m] >= 0;\n"
+     << "      xr[m] = *reinterpret_cast<const nca_vec<" << S << ", 4>*>(pl + (ok ? xo[m] : 0));\n"
+     << "      if (!ok) xr[m] = nca_vec<" << S << ", 4>{};\n    }\n  };\n";
+  os << "  auto store_x = [&](int slot) {\n#pragma unroll\n    for (int m = 0; m < NXL; ++m) {\n"
+     << "      const int i = tid + 256 * m;\n      if (i < XV * XH) {\n        const int r = i / XV, c = i - r * XV;\n"
+     << "        lvec w;\n#pragma unroll\n        for (int j = 0; j < 4; ++j) w.v[j] = nca_cvt<" << C << ", " << S
+     << ">::from(xr[m].v[j]);\n"
+     << "        *reinterpret_cast<lvec*>(&xs[slot][r * XW + 4 * c]) = w;\n      }\n    }\n  };\n";
+  os << "  auto load_e = [&](int z) {\n    const bool zin = (unsigned)z < (unsigned)NZi;\n";
+  for (int j = 0; j < n_x; ++j) {
+    const std::string E = ctype_of(vj.extra_dtypes[j]);
+    os << "    { const " << E << "* __restrict__ pl = e" << j << " + (zin ? (size_t)z * plane : (size_t)0);\n"
+       << "#pragma unroll\n      for (int m = 0; m < NWL; ++m) {\n        const bool ok = zin && go[m] >= 0;\n"
+       << "        er" << j << "[m] = *reinterpret_cast<const nca_vec<" << E << ", 4>*>(pl + (ok ? go[m] : 0));\n"
+       << "        if (!ok) er" << j << "[m] = nca_vec<" << E << ", 4>{};\n      } }\n";
+  }
+  os << "  };\n";
+  os << "  auto store_e = [&]() {\n#pragma unroll\n    for (int m = 0; m < NWL; ++m) {\n"
+     << "      const int i = tid + 256 * m;\n      if (i < WV * RH) {\n";
+  for (int j = 0; j < n_x; ++j)
+    os << "        { lvec w;\n#pragma unroll\n          for (int q = 0; q < 4; ++q) w.v[q] = nca_cvt<" << C << ", "
+       << ctype_of(vj.extra_dtypes[j]) << ">::from(er" << j << "[m].v[q]);\n"
+       << "          *reinterpret_cast<lvec*>(&es" << j << "[wb[m]]) = w; }\n";
+  os << "      }\n    }\n  };\n";
+  if (fz.fuse & NCA_FUSE_SUM) {
+    os << "  " << RC << " acc_s = 0;\n";
+    if (comp) os << "  " << RC << " acc_c = 0;\n";
+  }
+  if (fz.fuse & NCA_FUSE_MAX) os << "  " << RC << " acc_mx = nca_lim<" << RC << ">::lowest();\n";
+  if (fz.fuse & NCA_FUSE_MIN) os << "  " << RC << " acc_mn = nca_lim<" << RC << ">::max();\n";
+  // neighbour reads around an aligned 4-vector at LDS index `base` of plane arrays `arr[slot]`
+  // with row pitch `pitch`: per (dz, dy) the aligned vector V and the scalars L / R beside it
+  struct Nb { int dz, dy, left = 0, right = 0; };
+  auto neighbours = [&](const std::vector<std::array<int64_t, NCA_MAX_NDIM>>& offs, int sign) {
+    std::map<std::pair<int, int>, Nb> m;
+    for (auto& o : offs) {
+      const int dz = sign * static_cast<int>(o[0]), dy = sign * static_cast<int>(o[1]), dx = sign * static_cast<int>(o[2]);
+      auto& e = m[{dz, dy}];
+      e.dz = dz;
+      e.dy = dy;
+      if (dx < 0) e.left = std::max(e.left, -dx);
+      if (dx > 0) e.right = std::max(e.right, dx);
+    }
+    return m;
+  };
+  auto nm = [&](int v) { return v < 0 ? "m" + I(-v) : I(v); };
+  auto emit_reads = [&](const std::map<std::pair<int, int>, Nb>& nbs, const std::string& arr,
+                        const std::string& slot_m, const std::string& slot_0, const std::string& slot_p,
+                        const std::string& base, int pitch, const std::string& tag) {
+    for (auto& kv : nbs) {
+      const Nb& e = kv.second;
+      const std::string sl = e.dz < 0 ? slot_m : (e.dz > 0 ? slot_p : slot_0);
+      const std::string at = arr + "[" + sl + "][" + base + " + (" + I(static_cast<long long>(e.dy) * pitch) + ")";
+      const std::string id = tag + nm(e.dz) + "_" + nm(e.dy);
+      os << "          const lvec V" << id << " = *reinterpret_cast<const lvec*>(&" << at << "]);\n";
+      for (int q = 1; q <= e.left; ++q) os << "          const " << C << " L" << id << "_" << q << " = " << at << " - " << q << "];\n";
+      for (int q = 1; q <= e.right; ++q) os << "          const " << C << " R" << id << "_" << q << " = " << at << " + " << 3 + q << "];\n";
+    }
+  };
+  auto operand = [&](const std::string& tag, int dz, int dy, int dx, int j) {
+    const std::string id = tag + nm(dz) + "_" + nm(dy);
+    const int idx = j + dx;
+    if (idx >= 0 && idx < 4) return "V" + id + ".v[" + I(idx) + "]";
+    if (idx < 0) return "L" + id + "_" + I(-idx);
+    return "R" + id + "_" + I(idx - 3);
+  };
+  const auto xnb = neighbours(p.offsets, 1), wnb = neighbours(p.offsets, -1);
+  os << "  load_x(zb - 2);\n  store_x(nca_mod3(zb - 2));\n  load_x(zb - 1);\n  store_x(nca_mod3(zb - 1));\n"
+     << "  load_x(zb);\n  load_e(zb - 1);\n";
+  os << "  for (int zw = zb - 1; zw <= ze; ++zw) {\n"
+     << "    store_x(nca_mod3(zw + 1));\n    store_e();\n"
+     << "    if (zw + 2 <= ze + 1) load_x(zw + 2);\n    if (zw + 1 <= ze) load_e(zw + 1);\n"
+     << "    __syncthreads();\n"
+     << "    const bool own_z = zw >= zb && zw < ze, zin = (unsigned)zw < (unsigned)NZi;\n"
+     << "    const int sw = nca_mod3(zw), sm = nca_mod3(zw - 1), sp = nca_mod3(zw + 1);\n"
+     << "    const size_t zoff = (size_t)(zw > 0 ? zw : 0) * plane;\n"
+     << "#pragma unroll\n    for (int m = 0; m < NWL; ++m) {\n"
+     << "      const int i = tid + 256 * m;\n      if (i < WV * RH) {\n"
+     << "        lvec wv = {};\n"
+     << "        if (zin && go[m] >= 0) {\n";
+  emit_reads(xnb, "xs", "sm", "sw", "sp", "xb[m]", XW, "X");
+  for (int j = 0; j < n_x; ++j) os << "          const lvec E" << j << " = *reinterpret_cast<const lvec*>(&es" << j << "[wb[m]]);\n";
+  for (size_t m = 1; m < vj.out_dtypes.size(); ++m)
+    os << "          nca_vec<" << ctype_of(vj.out_dtypes[m]) << ", 4> g" << m << ";\n";
+  if (fz.fuse) os << "          " << RC << " rv[4];\n";
+  for (int j = 0; j < 4; ++j) {
+    os << "          {\n";
+    for (int k = 0; k < n_off; ++k) {
+      const auto& o = p.offsets[k];
+      os << "            const " << C << " u" << k << " = "
+         << operand("X", static_cast<int>(o[0]), static_cast<int>(o[1]), static_cast<int>(o[2]), j) << ";\n";
+    }
+    for (int jx = 0; jx < n_x; ++jx) os << "            const " << C << " u" << (n_off + jx) << " = E" << jx << ".v[" << j << "];\n";
+    os << sbody << vbody;
+    os << "            wv.v[" << j << "] = (" << C << ")(gsc * " << vnames[0] << ");\n";
+    for (size_t m = 1; m < vj.out_dtypes.size(); ++m)
+      os << "            g" << m << ".v[" << j << "] = nca_store_cvt<" << ctype_of(vj.out_dtypes[m]) << ", " << C << ">::to(("
+         << C << ")(gsc * " << vnames[m] << "));\n";
+    if (fz.fuse)
+      os << "            rv[" << j << "] = nca_cvt<" << RC << ", " << VD << ">::from(nca_store_cvt<" << VD << ", " << C
+         << ">::to(" << pname << "));\n";
+    os << "          }\n";
+  }
+  const bool own_work = vj.out_dtypes.size() > 1 || fz.fuse;
+  if (own_work) {
+    os << "          if (own_z && ((own >> m) & 1u)) {\n            const size_t off = zoff + (size_t)go[m];\n";
+    for (size_t m = 1; m < vj.out_dtypes.size(); ++m)
+      os << "            *reinterpret_cast<nca_vec<" << ctype_of(vj.out_dtypes[m]) << ", 4>*>(o" << m << " + off) = g" << m
+         << ";\n";
+    if (fz.fuse & NCA_FUSE_SUM) {
+      os << "            const " << RC << " qs = nca_add2(nca_add2(rv[0], rv[1]), nca_add2(rv[2], rv[3]));\n";
+      if (comp) os << "            nca_kahan_add(acc_s, acc_c, qs);\n";
+      else os << "            acc_s += qs;\n";
+    }
+    if (fz.fuse & NCA_FUSE_MAX)
+      os << "            acc_mx = nca_maxop(acc_mx, nca_maxnum(nca_maxnum(rv[0], rv[1]), nca_maxnum(rv[2], rv[3])));\n";
+    if (fz.fuse & NCA_FUSE_MIN)
+      os << "            acc_mn = nca_minop(acc_mn, nca_minnum(nca_minnum(rv[0], rv[1]), nca_minnum(rv[2], rv[3])));\n";
+    os << "          }\n";
+  }
+  os << "        }\n        *reinterpret_cast<lvec*>(&ws[sw][wb[m]]) = wv;\n      }\n    }\n    __syncthreads();\n";
+  os << "    if (zw - 1 >= zb) {\n"
+     << "      const int a0 = nca_mod3(zw - 2), a1 = sm, a2 = sw;\n"
+     << "      const size_t doff = (size_t)(zw - 1) * plane;\n"
+     << "#pragma unroll\n      for (int m = 0; m < NTL; ++m) {\n"
+     << "        if (to[m] >= 0) {\n";
+  emit_reads(wnb, "ws", "a0", "a1", "a2", "tb[m]", RWV, "W");
+  os << "          nca_vec<" << DX << ", 4> dv;\n";
+  for (int j = 0; j < 4; ++j) {
+    os << "          {\n";
+    for (int k = 0; k < n_off; ++k) {
+      const auto& o = p.offsets[k];
+      os << "            const " << C << " v" << k << " = "
+         << operand("W", -static_cast<int>(o[0]), -static_cast<int>(o[1]), -static_cast<int>(o[2]), j) << ";\n";
+    }
+    os << abody;
+    os << "            dv.v[" << j << "] = nca_store_cvt<" << DX << ", " << C << ">::to(" << aroots[0] << ");\n";
+    os << "          }\n";
+  }
+  os << "          *reinterpret_cast<nca_vec<" << DX << ", 4>*>(dxo + doff + (size_t)to[m]) = dv;\n";
+  os << "        }\n      }\n    }\n  }\n";
+  if (fz.fuse) {
+    os << "  __shared__ double red_sd[4];\n  __shared__ " << RC << " red_mx[4];\n  __shared__ " << RC << " red_mn[4];\n"
+       << "  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;\n";
+    if (fz.fuse & NCA_FUSE_SUM) {
+      if (comp) os << "  nca_wave_comp(acc_s, acc_c);\n  if (lane == 0) red_sd[wid] = nca_comp_value(acc_s, acc_c);\n";
+      else os << "  acc_s = nca_wave_sum(acc_s);\n  if (lane == 0) red_sd[wid] = (double)acc_s;\n";
+    }
+    if (fz.fuse & NCA_FUSE_MAX) os << "  acc_mx = nca_wave_max(acc_mx);\n  if (lane == 0) red_mx[wid] = acc_mx;\n";
+    if (fz.fuse & NCA_FUSE_MIN) os << "  acc_mn = nca_wave_min(acc_mn);\n  if (lane == 0) red_mn[wid] = acc_mn;\n";
+    os << "  __syncthreads();\n  if (threadIdx.x == 0) {\n";
+    if (fz.fuse & NCA_FUSE_SUM) os << "    ((double*)psum)[b] = red_sd[0] + red_sd[1] + red_sd[2] + red_sd[3];\n";
+    if (fz.fuse & NCA_FUSE_MAX)
+      os << "    " << RC << " m = red_mx[0];\n    for (int w = 1; w < 4; ++w) m = nca_maxop(m, red_mx[w]);\n    *(" << RC
+         << "*)((char*)pmax + 8 * (size_t)b) = m;\n";
+    if (fz.fuse & NCA_FUSE_MIN)
+      os << "    " << RC << " m2 = red_mn[0];\n    for (int w = 1; w < 4; ++w) m2 = nca_minop(m2, red_mn[w]);\n    *(" << RC
+         << "*)((char*)pmin + 8 * (size_t)b) = m2;\n";
+    os << "  }\n";
+  }
+  os << "}\n";
+  return os.str();
+}
+
 // ---- direct kernel ----------------------------------------------------------------------------------------
 static std::string gen_direct(const StencilProgram& p, int32_t dst_dtype, bool zero_boundary, int fuse, int mode) {
   const Tape& t = p.tape;
@@ -1626,11 +1881,13 @@ static void stencil_expr_vjp_impl(uint64_t stencil_handle, uint64_t expr_handle,
   }
 }
 
-// Tile of the training-step march: TX x TY points per block, 64 x 16 (64 x 8 for 2-byte
-// sources: measured fastest at 256^3); NCA_HIP_TRAIN_TILE = "TXxTY" overrides it, for tuning.
-static void train_tile(int32_t src_dtype, int& tx, int& ty) {
-  tx = 64;
-  ty = itemsize(src_dtype) == 2 ? 8 : 16;
+// Tile of the training-step march: TX x TY points per block, the fastest measured at 256^3:
+// 64 x 16, and for 2-byte sources 128 x 16 (vector form) or 64 x 8 (scalar form);
+// NCA_HIP_TRAIN_TILE = "TXxTY" overrides it, for tuning.
+static void train_tile(int32_t src_dtype, bool vec, int& tx, int& ty) {
+  const bool two_byte = itemsize(src_dtype) == 2;
+  tx = two_byte && vec ? 128 : 64;
+  ty = two_byte && !vec ? 8 : 16;
   if (const char* e = std::getenv("NCA_HIP_TRAIN_TILE")) {
     int a = 0, b = 0;
     if (std::sscanf(e, "%dx%d", &a, &b) == 2 && (a == 32 || a == 64 || a == 128 || a == 256) &&
@@ -1733,8 +1990,14 @@ static void stencil_expr_train_impl(uint64_t stencil_handle, uint64_t expr_handl
     finish_call(stream);
     return;
   }
+  // 4-point vectors when the last extent is a multiple of 4 and every array is aligned to them
+  auto aligned4 = [](const void* ptr, int32_t dt) { return reinterpret_cast<uintptr_t>(ptr) % (4 * itemsize(dt)) == 0; };
+  bool vec = NX % 4 == 0 && aligned4(src->data, src->dtype) && aligned4(grad_src->data, grad_src->dtype);
+  for (int j = 0; j < n_extra; ++j) vec = vec && aligned4(extras[j].data, extras[j].dtype);
+  for (size_t m = 0; m < outs.size(); ++m) vec = vec && aligned4(outs[m], vj.out_dtypes[m + 1]);
+  if (const char* e = std::getenv("NCA_HIP_TRAIN_SCALAR")) vec = vec && std::atoi(e) == 0;
   int TXs = 64, TYs = 16;
-  train_tile(p.src_dtype, TXs, TYs);
+  train_tile(p.src_dtype, vec, TXs, TYs);
   DeviceState& dev = current_device();
   std::string key = "train|" + et.signature + "|";
   for (int o : vj.operand) key += std::to_string(o) + ",";
@@ -1743,14 +2006,20 @@ static void stencil_expr_train_impl(uint64_t stencil_handle, uint64_t expr_handl
   key += "|";
   for (size_t m = 0; m < vj.roots.size(); ++m) key += std::to_string(vj.out_dtypes[m]) + (vj.roots[m] < 0 ? "z," : ",");
   key += "|dx" + std::to_string(grad_src->dtype) + "|f" + std::to_string(fuse) + "," + std::to_string(fuse ? mode : 0) +
-         "|t" + std::to_string(TXs) + "x" + std::to_string(TYs);
-  const std::string kname = std::string("nca_stencil_train_") + dtype_info(p.src_dtype).name + (fuse ? "_sum" : "");
+         "|t" + std::to_string(TXs) + "x" + std::to_string(TYs) + (vec ? "|v4" : "");
+  const std::string kname =
+      std::string("nca_stencil_train_") + dtype_info(p.src_dtype).name + (fuse ? "_sum" : "") + (vec ? "_v4" : "");
   StencilFuse fz;
   fz.with_dst = false;
   fz.fuse = fuse;
   fz.mode = mode;
-  hipFunction_t fn =
-      stencil_kernel(p, key, [&]() { return gen_march_train(p, adj, vj, fz, grad_src->dtype, TXs, TYs); }, kname);
+  hipFunction_t fn = stencil_kernel(
+      p, key,
+      [&]() {
+        return vec ? gen_march_train_vec(p, adj, vj, fz, grad_src->dtype, TXs, TYs)
+                   : gen_march_train(p, adj, vj, fz, grad_src->dtype, TXs, TYs);
+      },
+      kname);
   const int ntx = static_cast<int>((NX + TXs - 1) / TXs), nty = static_cast<int>((NY + TYs - 1) / TYs);
   const int64_t tiles = static_cast<int64_t>(ntx) * nty;
   const int64_t resident = static_cast<int64_t>(dev.n_cus) * blocks_per_cu(fn);
@@ -1840,7 +2109,8 @@ int nca_hip_stencil_expr_train(uint64_t stencil_handle, uint64_t expr_handle, co
 
 int nca_hip_stencil_expr_train_source(uint64_t stencil_handle, uint64_t expr_handle, const int32_t* expr_operand,
                                       int32_t n_operand, const int32_t* extra_dtypes, int32_t n_extra,
-                                      int32_t grad_src_dtype, int32_t fuse, int32_t accum_mode, char* buf, size_t n) {
+                                      int32_t grad_src_dtype, int32_t fuse, int32_t accum_mode, int32_t vec, char* buf,
+                                      size_t n) {
   std::string src;
   int rc = api_call([&] {
     StencilProgram& p = stencil_program(stencil_handle);
@@ -1878,8 +2148,11 @@ int nca_hip_stencil_expr_train_source(uint64_t stencil_handle, uint64_t expr_han
     fz.fuse = fuse;
     fz.mode = accum_mode;
     int TXs = 64, TYs = 16;
-    train_tile(p.src_dtype, TXs, TYs);
-    src = device_prelude() + "\n" + gen_march_train(p, adjoint_tape(p), vj, fz, grad_src_dtype, TXs, TYs);
+    train_tile(p.src_dtype, vec != 0, TXs, TYs);
+    const Tape adj = adjoint_tape(p);
+    src = device_prelude() + "\n" +
+          (vec ? gen_march_train_vec(p, adj, vj, fz, grad_src_dtype, TXs, TYs)
+               : gen_march_train(p, adj, vj, fz, grad_src_dtype, TXs, TYs));
   });
   if (rc != NCA_OK) return -rc;
   if (buf && n > 0) {

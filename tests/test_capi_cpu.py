@@ -185,3 +185,16 @@ def test_training_step_kernel_compiles(dt):
         assert "xs[sm][xb[m] + (0)]" in src and "xs[sp][xb[m] + (0)]" in src   # the z neighbours
         assert "ws[a2][tb[m] + (0)]" in src and "ws[a0][tb[m] + (0)]" in src   # w at z + 1, z - 1
     assert "nca_kahan_add(acc_s" in fused and "acc_s" not in plain and "nca_maxnum(acc_mx" in fused
+
+
+@pytest.mark.parametrize("dt", [capi.FLOAT32, capi.BFLOAT16])
+def test_training_step_vector_kernel_compiles(dt):
+    """The 4-point-vector form: x neighbours from aligned LDS vectors and the scalars beside
+    them, the adjoint's from the w planes the same way."""
+    from ncarray_amd import autograd as A
+    lap, _ = A.laplacian_stencil(dt, capi.FLOAT32)
+    tape = A.c0_tape(capi.FLOAT32, dt, capi.FLOAT32)
+    for fuse in (0, capi.FUSE_SUM | capi.FUSE_MIN):
+        src = lap.train_source(tape, [-1, 0, -1, -1, 0], [dt], dt, fuse=fuse, mode=capi.ACCUM_COMPENSATED, vec=True)
+        _compiles(src)
+        assert "_v4(" in src and "const lvec VX0_0" in src and "const lvec VW1_0" in src and "LW0_0_1" in src

@@ -380,3 +380,36 @@ def test_stencil_expr_vjp_rejects(dev):
     fused = A.fused_loss_stencil(capi.FLOAT32, capi.FLOAT32)  # has an extra array of its own
     with pytest.raises(capi.NcaError, match="reads only its source"):
         fused.expr_vjp(tape, [-1, 0, -1, -1, 0], ti.view(x), ti.view(w), **args)
+
+
+@pytest.mark.parametrize("dtype", [torch.float32, torch.bfloat16])
+@pytest.mark.parametrize("shape", [(20, 24, 64), (9, 13, 36), (5, 70, 132)])
+def test_stencil_expr_train_vector_matches_scalar(dev, dtype, shape, monkeypatch):
+    """The 4-point-vector form of the training pass (last extent a multiple of 4, aligned
+    arrays; partial tiles in x and y) gives the bits of the scalar form for dx and dy, and a
+    loss within the accumulation's bound of it."""
+    A, capi, tapes = _mods()
+    from ncarray_amd import torch_interop as ti
+    g = torch.Generator(device=dev).manual_seed(sum(shape))
+    x = torch.randn(shape, generator=g, device=dev).to(dtype)
+    y = torch.randn(shape, generator=g, device=dev).to(dtype)
+    gs = torch.tensor(0.625, dtype=torch.float32, device=dev)
+    lap, _ = A.laplacian_stencil(ti.TORCH_TO_CODE[dtype], capi.FLOAT32)
+    tape = A.c0_tape(capi.FLOAT32, ti.TORCH_TO_CODE[dtype], capi.FLOAT32)
+    res = []
+    for scalar in ("1", "0"):
+        monkeypatch.setenv("NCA_HIP_TRAIN_SCALAR", scalar)
+        dx, dy = torch.empty_like(x), torch.empty_like(y)
+        loss = torch.empty((), dtype=torch.float32, device=dev)
+        lap.expr_train(tape, [-1, 0, -1, -1, 0], ti.view(x), ti.view(dx), extras=[ti.view(y)],
+                       grad_extras=[ti.view(dy)], scalars=(6.0,), expr_scalars=(0.5,), grad_scalar=gs.data_ptr(),
+                       fuse=capi.FUSE_SUM, mode=capi.ACCUM_COMPENSATED, sum_out=loss.data_ptr())
+        res.append((dx, dy, loss))
+    torch.cuda.synchronize()
+    (dx1, dy1, l1), (dx2, dy2, l2) = res
+    assert torch.equal(dx1, dx2) and torch.equal(dy1, dy2)
+    xn, yn = x.float().cpu().numpy(), y.float().cpu().numpy()
+    a = lap64(xn)
+    mass = float(np.sum(np.abs(a * yn) + np.exp(-a * a) + 0.5 * np.abs(yn)))
+    assert abs(l1.item() - l2.item()) <= 16 * np.finfo(np.float32).eps * mass
+    assert abs(l2.item() - loss64(xn, yn)) <= 8 * np.finfo(np.float32).eps * mass
