@@ -1892,8 +1892,14 @@ static void train_tile(int32_t src_dtype, bool vec, int& tx, int& ty) {
     int a = 0, b = 0;
     if (std::sscanf(e, "%dx%d", &a, &b) == 2 && (a == 32 || a == 64 || a == 128 || a == 256) &&
         (b == 2 || b == 4 || b == 8 || b == 16 || b == 32)) {
-      tx = a;
-      ty = b;
+      // the three x planes, the three w planes and one extra plane (4-byte compute type) must
+      // fit the 64 KiB of LDS one block may hold
+      const int64_t xw = vec ? a + 16 : a + 4, rw = vec ? a + 8 : a + 2;
+      const int64_t lds = 4 * (3 * (b + 4) * xw + 4 * (b + 2) * rw);
+      if (lds <= 60 * 1024) {
+        tx = a;
+        ty = b;
+      }
     }
   }
 }
