@@ -413,3 +413,78 @@ def test_stencil_expr_train_vector_matches_scalar(dev, dtype, shape, monkeypatch
     mass = float(np.sum(np.abs(a * yn) + np.exp(-a * a) + 0.5 * np.abs(yn)))
     assert abs(l1.item() - l2.item()) <= 16 * np.finfo(np.float32).eps * mass
     assert abs(l2.item() - loss64(xn, yn)) <= 8 * np.finfo(np.float32).eps * mass
+
+
+# an asymmetric radius-1 stencil with diagonal offsets and scalar coefficients, and an
+# expression of it and two extra arrays: f = a*a*y + a*z (operands a, a, y, a, z)
+_DIAG_OFFSETS = [[-1, 1, -1], [1, -1, 1], [0, 1, 1], [1, 0, -1], [0, 0, 0], [-1, -1, 0], [0, -1, -1]]
+_DIAG_SCALARS = (0.75, 1.5, -2.0)
+_DIAG_COEF = (0.75, 1.0, -1.5, 1.0, -2.0, 1.0, -1.0)
+
+
+def _diag_tapes(capi, tapes):
+    P = tapes.P
+    st = [P(capi.OP_LOAD_NCARR, 0), P(capi.OP_LOAD_CONST, 0), P(capi.OP_MUL), P(capi.OP_LOAD_NCARR, 1), P(capi.OP_ADD),
+          P(capi.OP_LOAD_NCARR, 2), P(capi.OP_LOAD_CONST, 1), P(capi.OP_MUL), P(capi.OP_SUB),
+          P(capi.OP_LOAD_NCARR, 3), P(capi.OP_ADD), P(capi.OP_LOAD_NCARR, 4), P(capi.OP_LOAD_CONST, 2), P(capi.OP_MUL),
+          P(capi.OP_ADD), P(capi.OP_LOAD_NCARR, 5), P(capi.OP_ADD), P(capi.OP_LOAD_NCARR, 6), P(capi.OP_SUB)]
+    stencil = capi.Stencil(_DIAG_OFFSETS, st, capi.FLOAT32, capi.FLOAT32, capi.FLOAT32, [capi.FLOAT32] * 3)
+    ex = [P(capi.OP_LOAD_NCARR, 0), P(capi.OP_LOAD_NCARR, 1), P(capi.OP_MUL), P(capi.OP_LOAD_NCARR, 2), P(capi.OP_MUL),
+          P(capi.OP_LOAD_NCARR, 3), P(capi.OP_LOAD_NCARR, 4), P(capi.OP_MUL), P(capi.OP_ADD)]
+    expr = capi.Tape(ex, [capi.FLOAT32] * 5, [], capi.FLOAT32, capi.FLOAT32)
+    return stencil, expr
+
+
+def _diag_fp64(x, y, z, g):
+    """FP64 reference: a = S(x) (zero outside), loss = sum(a*a*y + a*z), w = g * (2*a*y + z),
+    dx = S^T w (w zero outside), dy = g * a*a, dz = g * a."""
+    x, y, z = (np.asarray(v, np.float64) for v in (x, y, z))
+    n = x.shape
+    xp = np.pad(x, 1)
+    a = np.zeros(n)
+    for c, o in zip(_DIAG_COEF, _DIAG_OFFSETS):
+        a += c * xp[1 + o[0]:1 + o[0] + n[0], 1 + o[1]:1 + o[1] + n[1], 1 + o[2]:1 + o[2] + n[2]]
+    w = g * (2 * a * y + z)
+    wp = np.pad(w, 1)
+    dx = np.zeros(n)
+    for c, o in zip(_DIAG_COEF, _DIAG_OFFSETS):
+        dx += c * wp[1 - o[0]:1 - o[0] + n[0], 1 - o[1]:1 - o[1] + n[1], 1 - o[2]:1 - o[2] + n[2]]
+    return float(np.sum(a * a * y + a * z)), dx, g * a * a, g * a, a
+
+
+@pytest.mark.parametrize("shape", [(11, 19, 36), (7, 33, 68), (10, 17, 29)])
+def test_stencil_expr_train_general_stencil(dev, shape, monkeypatch):
+    """Diagonal offsets in every axis (the vector form's neighbour scalars beside other planes'
+    and rows' vectors), an adjoint that is not the stencil itself, two extras with their
+    gradients and a loss: scalar and vector forms agree bit for bit and both match FP64."""
+    A, capi, tapes = _mods()
+    from ncarray_amd import torch_interop as ti
+    stencil, expr = _diag_tapes(capi, tapes)
+    rng = np.random.default_rng(sum(shape) + 7)
+    x, y, z = (rng.standard_normal(shape).astype(np.float32) for _ in range(3))
+    xt, yt, zt = (torch.from_numpy(v).to(dev) for v in (x, y, z))
+    g = 1.25
+    gs = torch.tensor(g, dtype=torch.float32, device=dev)
+    forms = ("1", "0") if shape[2] % 4 == 0 else ("1",)
+    res = []
+    for scalar in forms:
+        monkeypatch.setenv("NCA_HIP_TRAIN_SCALAR", scalar)
+        dx, dy, dz = torch.empty_like(xt), torch.empty_like(yt), torch.empty_like(zt)
+        loss = torch.empty((), dtype=torch.float32, device=dev)
+        stencil.expr_train(expr, [-1, -1, 0, -1, 1], ti.view(xt), ti.view(dx), extras=[ti.view(yt), ti.view(zt)],
+                           grad_extras=[ti.view(dy), ti.view(dz)], scalars=_DIAG_SCALARS, grad_scalar=gs.data_ptr(),
+                           fuse=capi.FUSE_SUM, mode=capi.ACCUM_COMPENSATED, sum_out=loss.data_ptr())
+        res.append((dx, dy, dz, loss))
+    torch.cuda.synchronize()
+    l64, dx64, dy64, dz64, a64 = _diag_fp64(x, y, z, g)
+    eps = np.finfo(np.float32).eps
+    amax = 6.0 * max(1.0, float(np.max(np.abs(x))))        # |a| bound: sum of |c_k| |x|
+    for dx, dy, dz, loss in res:
+        assert np.max(np.abs(dx.cpu().numpy() - dx64)) <= 64 * eps * amax * amax * max(1.0, float(np.max(np.abs(y))))
+        assert np.max(np.abs(dy.cpu().numpy() - dy64)) <= 32 * eps * amax * amax
+        assert np.max(np.abs(dz.cpu().numpy() - dz64)) <= 16 * eps * amax
+        mass = float(np.sum(np.abs(a64 * a64 * y) + np.abs(a64 * z)))
+        assert abs(loss.item() - l64) <= 64 * eps * mass
+    if len(res) == 2:
+        for u, v in zip(res[0][:3], res[1][:3]):
+            assert torch.equal(u, v)
