@@ -860,7 +860,6 @@ static std::string gen_march_train(const StencilProgram& p, const Tape& adj, con
   os << "  const " << C << " gsc = gin ? *gin : (" << C << ")1;\n";
   // LDS planes, flat: x with a two-point halo, w and the extras on the ring region
   os << "  __shared__ " << C << " xs[3][XH * XW];\n  __shared__ " << C << " ws[3][RH * RW];\n";
-  for (int j = 0; j < n_x; ++j) os << "  __shared__ " << C << " es" << j << "[RH * RW];\n";
   os << "  const unsigned nb = gridDim.x, b = blockIdx.x;\n"
      << "  const unsigned tix = (nb % 8u == 0u) ? (b % 8u) * (nb / 8u) + b / 8u : b;\n"
      << "  const int txi = tix % ntx, tyi = (tix / ntx) % nty, tzi = tix / (ntx * nty);\n"
@@ -881,7 +880,8 @@ static std::string gen_march_train(const StencilProgram& p, const Tape& adj, con
      << "    const int i = tid + 256 * m, ty = i / TX, tx = i - ty * TX, gy = y0 + ty, gx = x0 + tx;\n"
      << "    to[m] = (i < TX * TY && gy < NYi && gx < NXi) ? gy * NXi + gx : -1;\n    tb[m] = (ty + 1) * RW + tx + 1;\n  }\n";
   os << "  " << C << " xr[NXL];\n";
-  for (int j = 0; j < n_x; ++j) os << "  " << C << " er" << j << "[NRL];\n";
+  // the extras at this thread's ring positions: the plane being evaluated (ec) and the next (er)
+  for (int j = 0; j < n_x; ++j) os << "  " << C << " er" << j << "[NRL], ec" << j << "[NRL];\n";
   os << "  auto load_x = [&](int z) {\n    const bool zin = (unsigned)z < (unsigned)NZi;\n"
      << "    const " << S << "* __restrict__ pl = src + (zin ? (size_t)z * plane : (size_t)0);\n"
      << "#pragma unroll\n    for (int m = 0; m < NXL; ++m)\n"
@@ -897,10 +897,9 @@ static std::string gen_march_train(const StencilProgram& p, const Tape& adj, con
        << ")0; }\n";
   }
   os << "  };\n";
-  os << "  auto store_e = [&]() {\n#pragma unroll\n    for (int m = 0; m < NRL; ++m) {\n"
-     << "      const int i = tid + 256 * m;\n      if (i < RW * RH) {\n";
-  for (int j = 0; j < n_x; ++j) os << "        es" << j << "[i] = er" << j << "[m];\n";
-  os << "      }\n    }\n  };\n";
+  os << "  auto store_e = [&]() {\n#pragma unroll\n    for (int m = 0; m < NRL; ++m) {\n";
+  for (int j = 0; j < n_x; ++j) os << "      ec" << j << "[m] = er" << j << "[m];\n";
+  os << "    }\n  };\n";
   if (fz.fuse & NCA_FUSE_SUM) {
     os << "  " << RC << " acc_s = 0;\n";
     if (comp) os << "  " << RC << " acc_c = 0;\n";
@@ -928,7 +927,7 @@ static std::string gen_march_train(const StencilProgram& p, const Tape& adj, con
     const std::string sl = o[0] < 0 ? "sm" : (o[0] > 0 ? "sp" : "sw");
     os << "          const " << C << " u" << k << " = xs[" << sl << "][xb[m] + (" << o[1] * (TXs + 4) + o[2] << ")];\n";
   }
-  for (int j = 0; j < n_x; ++j) os << "          const " << C << " u" << (n_off + j) << " = es" << j << "[i];\n";
+  for (int j = 0; j < n_x; ++j) os << "          const " << C << " u" << (n_off + j) << " = ec" << j << "[m];\n";
   os << sbody << vbody;
   os << "          wv = (" << C << ")(gsc * " << vnames[0] << ");\n";
   const bool own_work = vj.out_dtypes.size() > 1 || fz.fuse;
@@ -1055,7 +1054,7 @@ static std::string gen_march_train_vec(const StencilProgram& p, const Tape& adj,
   os << "  struct __attribute__((aligned(16))) lvec { " << C << " v[4]; };\n";
   os << "  __shared__ __attribute__((aligned(16))) " << C << " xs[3][XH * XW];\n"
      << "  __shared__ __attribute__((aligned(16))) " << C << " ws[3][RH * RWV];\n";
-  for (int j = 0; j < n_x; ++j) os << "  __shared__ __attribute__((aligned(16))) " << C << " es" << j << "[RH * RWV];\n";
+
   os << "  const unsigned nb = gridDim.x, b = blockIdx.x;\n"
      << "  const unsigned tix = (nb % 8u == 0u) ? (b % 8u) * (nb / 8u) + b / 8u : b;\n"
      << "  const int txi = tix % ntx, tyi = (tix / ntx) % nty, tzi = tix / (ntx * nty);\n"
@@ -1076,7 +1075,9 @@ static std::string gen_march_train_vec(const StencilProgram& p, const Tape& adj,
      << "    const int i = tid + 256 * m, r = i / TV, c = i - r * TV, gy = y0 + r, gx = x0 + 4 * c;\n"
      << "    to[m] = (i < TV * TY && gy < NYi && gx < NXi) ? gy * NXi + gx : -1;\n    tb[m] = (r + 1) * RWV + 4 + 4 * c;\n  }\n";
   os << "  nca_vec<" << S << ", 4> xr[NXL];\n";
-  for (int j = 0; j < n_x; ++j) os << "  nca_vec<" << ctype_of(vj.extra_dtypes[j]) << ", 4> er" << j << "[NWL];\n";
+  // the extras at this thread's w vectors: the plane being evaluated (ec) and the next (er)
+  for (int j = 0; j < n_x; ++j)
+    os << "  nca_vec<" << ctype_of(vj.extra_dtypes[j]) << ", 4> er" << j << "[NWL], ec" << j << "[NWL];\n";
   os << "  auto load_x = [&](int z) {\n    const bool zin = (unsigned)z < (unsigned)NZi;\n"
      << "    const " << S << "* __restrict__ pl = src + (zin ? (size_t)z * plane : (size_t)0);\n"
      << "#pragma unroll\n    for (int m = 0; m < NXL; ++m) {\n"
@@ -1097,13 +1098,9 @@ static std::string gen_march_train_vec(const StencilProgram& p, const Tape& adj,
        << "        if (!ok) er" << j << "[m] = nca_vec<" << E << ", 4>{};\n      } }\n";
   }
   os << "  };\n";
-  os << "  auto store_e = [&]() {\n#pragma unroll\n    for (int m = 0; m < NWL; ++m) {\n"
-     << "      const int i = tid + 256 * m;\n      if (i < WV * RH) {\n";
-  for (int j = 0; j < n_x; ++j)
-    os << "        { lvec w;\n#pragma unroll\n          for (int q = 0; q < 4; ++q) w.v[q] = nca_cvt<" << C << ", "
-       << ctype_of(vj.extra_dtypes[j]) << ">::from(er" << j << "[m].v[q]);\n"
-       << "          *reinterpret_cast<lvec*>(&es" << j << "[wb[m]]) = w; }\n";
-  os << "      }\n    }\n  };\n";
+  os << "  auto store_e = [&]() {\n#pragma unroll\n    for (int m = 0; m < NWL; ++m) {\n";
+  for (int j = 0; j < n_x; ++j) os << "      ec" << j << "[m] = er" << j << "[m];\n";
+  os << "    }\n  };\n";
   if (fz.fuse & NCA_FUSE_SUM) {
     os << "  " << RC << " acc_s = 0;\n";
     if (comp) os << "  " << RC << " acc_c = 0;\n";
@@ -1161,7 +1158,7 @@ static std::string gen_march_train_vec(const StencilProgram& p, const Tape& adj,
      << "        lvec wv = {};\n"
      << "        if (zin && go[m] >= 0) {\n";
   emit_reads(xnb, "xs", "sm", "sw", "sp", "xb[m]", XW, "X");
-  for (int j = 0; j < n_x; ++j) os << "          const lvec E" << j << " = *reinterpret_cast<const lvec*>(&es" << j << "[wb[m]]);\n";
+  for (int j = 0; j < n_x; ++j) os << "          const nca_vec<" << ctype_of(vj.extra_dtypes[j]) << ", 4> E" << j << " = ec" << j << "[m];\n";
   for (size_t m = 1; m < vj.out_dtypes.size(); ++m)
     os << "          nca_vec<" << ctype_of(vj.out_dtypes[m]) << ", 4> g" << m << ";\n";
   if (fz.fuse) os << "          " << RC << " rv[4];\n";
@@ -1172,7 +1169,9 @@ static std::string gen_march_train_vec(const StencilProgram& p, const Tape& adj,
       os << "            const " << C << " u" << k << " = "
          << operand("X", static_cast<int>(o[0]), static_cast<int>(o[1]), static_cast<int>(o[2]), j) << ";\n";
     }
-    for (int jx = 0; jx < n_x; ++jx) os << "            const " << C << " u" << (n_off + jx) << " = E" << jx << ".v[" << j << "];\n";
+    for (int jx = 0; jx < n_x; ++jx)
+      os << "            const " << C << " u" << (n_off + jx) << " = nca_cvt<" << C << ", " << ctype_of(vj.extra_dtypes[jx])
+         << ">::from(E" << jx << ".v[" << j << "]);\n";
     os << sbody << vbody;
     os << "            wv.v[" << j << "] = (" << C << ")(gsc * " << vnames[0] << ");\n";
     for (size_t m = 1; m < vj.out_dtypes.size(); ++m)
@@ -1892,10 +1891,10 @@ static void train_tile(int32_t src_dtype, bool vec, int& tx, int& ty) {
     int a = 0, b = 0;
     if (std::sscanf(e, "%dx%d", &a, &b) == 2 && (a == 32 || a == 64 || a == 128 || a == 256) &&
         (b == 2 || b == 4 || b == 8 || b == 16 || b == 32)) {
-      // the three x planes, the three w planes and one extra plane (4-byte compute type) must
-      // fit the 64 KiB of LDS one block may hold
+      // the three x planes and the three w planes (4-byte compute type) must fit the 64 KiB
+      // of LDS one block may hold
       const int64_t xw = vec ? a + 16 : a + 4, rw = vec ? a + 8 : a + 2;
-      const int64_t lds = 4 * (3 * (b + 4) * xw + 4 * (b + 2) * rw);
+      const int64_t lds = 4 * (3 * (b + 4) * xw + 3 * (b + 2) * rw);
       if (lds <= 60 * 1024) {
         tx = a;
         ty = b;
