@@ -33,7 +33,7 @@ MARCH_KERNEL = "nca_stencil_march_float32_axis0_max"
 # HBM bytes per launch of that kernel from rocprofv3 PMC runs of this command (FETCH_SIZE
 # scaled by 2.0, the factor that makes a 512 MiB device copy read 512 MiB on this hardware;
 # WRITE_SIZE exact), recorded in profiles/nca_stencil_march_laplacian_512cubed.md.
-MARCH_TRAFFIC_BYTES = 1113525248
+MARCH_TRAFFIC_BYTES = 1109746802
 
 
 def parse():
