@@ -189,6 +189,9 @@ const char* nca_hip_version(void);
 /* Device properties the kernels size themselves by. */
 int nca_hip_device_info(int32_t* n_cus, int32_t* lds_per_block, int64_t* total_mem,
                         char* arch_name, size_t arch_len);
+/* The device this thread's calls run on (nca_hip_init's, or the one a worker thread adopted):
+ * the device id of DLPack exports and similar descriptions of the library's memory. */
+int nca_hip_current_device(int32_t* device);
 
 /* Device memory for DevOwnerPolicy (storage.hh:176-186, 242-252).  Like the reference's
  * cudaMallocAsync on one stream, blocks come from the device's default memory pool,

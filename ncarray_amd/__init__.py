@@ -19,7 +19,7 @@ __version__ = "0.1.0"
 
 _ARRAY_NAMES = {"DType", "OpCode", "Boundary", "Accum", "NCDevArray", "DevExpr", "Stencil", "NcaError", "NCArray",
                 "NCArrayRef", "NCDevArrayRef", "set_eager", "is_eager", "materialize", "lazy_mode", "exp", "log", "sqrt", "abs",
-                "init", "from_torch", "numpy_dtype"}
+                "init", "from_torch", "from_dlpack", "numpy_dtype"}
 
 
 def __getattr__(name):

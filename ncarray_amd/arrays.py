@@ -123,6 +123,14 @@ def from_torch(tensor):
     return NCDevArray.from_cuda_array(tensor, _interface_dtype(tensor))
 
 
+def from_dlpack(obj):
+    """Zero-copy NCDevArray view of a DLPack producer (an object with __dlpack__, such as a torch
+    tensor, or a "dltensor" capsule); the producer's memory stays alive while the view does.
+    NCDevArray exports itself the same way (__dlpack__ / __dlpack_device__), so
+    torch.from_dlpack(array) is a zero-copy tensor of it."""
+    return NCDevArray.from_dlpack(obj)
+
+
 _ncarray._set_host_array_type(NCArray)
 
 __all__ = ["DType", "OpCode", "Boundary", "Accum", "NCDevArray", "DevExpr", "Stencil", "NcaError", "NCArray",

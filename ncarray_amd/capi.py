@@ -134,6 +134,7 @@ _SIGS = {
                                    _P, _VP, _VP, _I32, _I32, _P, _P, _P, _I32, _P],
     "nca_hip_stencil_expr_train_source": [_U64, _U64, ctypes.POINTER(_I32), _I32, ctypes.POINTER(_I32), _I32, _I32,
                                           _I32, _I32, _I32, ctypes.c_char_p, _SZ],
+    "nca_hip_current_device": [ctypes.POINTER(_I32)],
     "nca_hip_stencil_source": [_U64, _I32, ctypes.c_char_p, _SZ],
     "nca_hip_scatter_add": [_VP, _VP, _VP, _P],
     "nca_hip_bin": [_VP, _VP, _I32, _I32, _VP, _VP, _P],

@@ -291,6 +291,13 @@ int nca_hip_last_error(char* buf, size_t n) {
 
 const char* nca_hip_version(void) { return "ncarray-hip 0.1.0 gfx950"; }
 
+int nca_hip_current_device(int32_t* device) {
+  return api_call([&] {
+    NCA_REQUIRE(device != nullptr, "null argument");
+    *device = current_device().id;
+  });
+}
+
 int nca_hip_device_info(int32_t* n_cus, int32_t* lds_per_block, int64_t* total_mem,
                         char* arch_name, size_t arch_len) {
   return api_call([&] {

@@ -331,6 +331,149 @@ struct PyStencil {
   DType dest_dtype = DType::float32;
 };
 
+// ---- DLPack --------------------------------------------------------------------------------
+// The array-interchange protocol of torch, numpy and others: a PyCapsule named "dltensor" holds a
+// DLManagedTensor (the producer's deleter frees it); a consumer renames the capsule
+// "used_dltensor" and calls the deleter when it no longer needs the memory.  The C structs are
+// the protocol's ABI (version 0.x "DLManagedTensor"), restated here.
+namespace dl {
+struct Device { int32_t device_type; int32_t device_id; };
+struct DataType { uint8_t code; uint8_t bits; uint16_t lanes; };
+struct Tensor {
+  void* data;
+  Device device;
+  int32_t ndim;
+  DataType dtype;
+  int64_t* shape;
+  int64_t* strides;  // in elements; NULL: C-contiguous
+  uint64_t byte_offset;
+};
+struct Managed {
+  Tensor dl_tensor;
+  void* manager_ctx;
+  void (*deleter)(Managed*);
+};
+enum { kInt = 0, kUInt = 1, kFloat = 2, kBfloat = 4, kBool = 6 };
+enum { kCUDA = 2, kROCM = 10 };
+
+DataType of(DType d) {
+  const uint8_t bits = static_cast<uint8_t>(8 * itemsize(d));
+  switch (d) {
+    case DType::bool_: return {kBool, 8, 1};
+    case DType::char_: case DType::int8: case DType::int16: case DType::int32: case DType::int64: return {kInt, bits, 1};
+    case DType::uint8: case DType::uint16: case DType::uint32: case DType::uint64: return {kUInt, bits, 1};
+    case DType::float32: case DType::float64: case DType::float16: return {kFloat, bits, 1};
+    case DType::bfloat16: return {kBfloat, 16, 1};
+    default: throw Error(NCA_ERR_UNSUPPORTED, std::string("no DLPack type for ") + dtype_name(d));
+  }
+}
+
+DType to_dtype(DataType t) {
+  if (t.lanes != 1) throw Error(NCA_ERR_UNSUPPORTED, "vector DLPack types are not supported");
+  switch (t.code) {
+    case kBool: if (t.bits == 8) return DType::bool_; break;
+    case kInt:
+      if (t.bits == 8) return DType::int8;
+      if (t.bits == 16) return DType::int16;
+      if (t.bits == 32) return DType::int32;
+      if (t.bits == 64) return DType::int64;
+      break;
+    case kUInt:
+      if (t.bits == 8) return DType::uint8;
+      if (t.bits == 16) return DType::uint16;
+      if (t.bits == 32) return DType::uint32;
+      if (t.bits == 64) return DType::uint64;
+      break;
+    case kFloat:
+      if (t.bits == 16) return DType::float16;
+      if (t.bits == 32) return DType::float32;
+      if (t.bits == 64) return DType::float64;
+      break;
+    case kBfloat: if (t.bits == 16) return DType::bfloat16; break;
+    default: break;
+  }
+  throw Error(NCA_ERR_UNSUPPORTED, "unsupported DLPack type (code " + std::to_string(t.code) + ", " +
+                                       std::to_string(t.bits) + " bits)");
+}
+
+// What an exported capsule owns: the array (keeping its memory alive) and the shape/strides
+// arrays the DLTensor points at.
+struct Export {
+  Managed managed{};
+  NCDevArray array;
+  std::vector<int64_t> shape, strides;
+};
+
+void delete_export(Managed* m) { delete static_cast<Export*>(m->manager_ctx); }
+
+void capsule_destructor(PyObject* cap) {
+  // a capsule no consumer took still owns its tensor
+  if (PyCapsule_IsValid(cap, "dltensor")) {
+    auto* m = static_cast<Managed*>(PyCapsule_GetPointer(cap, "dltensor"));
+    if (m && m->deleter) m->deleter(m);
+  } else {
+    PyErr_Clear();
+  }
+}
+
+py::capsule to_capsule(const NCDevArray& a) {
+  if (a.pointer_axis() >= 0)
+    throw Error(NCA_ERR_UNSUPPORTED, "a pointer-axis array has no DLPack form (use dense())");
+  auto* e = new Export();
+  e->array = a;
+  const int64_t item = a.itemsize();
+  for (int d = 0; d < a.ndim(); ++d) {
+    e->shape.push_back(a.shape(d));
+    if (a.strides()[d] % item != 0) {
+      delete e;
+      throw Error(NCA_ERR_UNSUPPORTED, "DLPack strides count elements; this view's byte strides do not");
+    }
+    e->strides.push_back(a.strides()[d] / item);
+  }
+  int32_t dev = 0;
+  (void)nca_hip_current_device(&dev);
+  Tensor& t = e->managed.dl_tensor;
+  t.data = const_cast<void*>(a.data());
+  t.device = {kROCM, dev};
+  t.ndim = a.ndim();
+  t.dtype = of(a.dtype());
+  t.shape = e->shape.empty() ? nullptr : e->shape.data();
+  t.strides = e->strides.empty() ? nullptr : e->strides.data();
+  t.byte_offset = 0;
+  e->managed.manager_ctx = e;
+  e->managed.deleter = delete_export;
+  return py::capsule(&e->managed, "dltensor", &capsule_destructor);
+}
+
+NCDevArray from_capsule(py::capsule cap) {
+  PyObject* o = cap.ptr();
+  if (!PyCapsule_IsValid(o, "dltensor")) throw py::value_error("not an unconsumed DLPack capsule (name \"dltensor\")");
+  auto* m = static_cast<Managed*>(PyCapsule_GetPointer(o, "dltensor"));
+  const Tensor& t = m->dl_tensor;
+  if (t.device.device_type != kROCM && t.device.device_type != kCUDA)
+    throw Error(NCA_ERR_UNSUPPORTED, "DLPack tensor is not in GPU memory (device type " +
+                                         std::to_string(t.device.device_type) + ")");
+  const DType d = to_dtype(t.dtype);
+  std::vector<int64_t> shape(t.shape, t.shape + t.ndim), strides(t.ndim);
+  int64_t s = itemsize(d);
+  for (int k = t.ndim - 1; k >= 0; --k) {
+    strides[k] = t.strides ? t.strides[k] * itemsize(d) : s;
+    s *= shape[k];
+  }
+  // the consumer owns it now: rename the capsule and call the producer's deleter when the last
+  // view of the memory is gone
+  PyCapsule_SetName(o, "used_dltensor");
+  std::shared_ptr<void> keep(m, [](void* p) {
+    auto* mm = static_cast<Managed*>(p);
+    if (mm->deleter) {
+      py::gil_scoped_acquire g;
+      mm->deleter(mm);
+    }
+  });
+  return NCDevArray::wrap(static_cast<char*>(t.data) + t.byte_offset, shape, strides, d, keep);
+}
+}  // namespace dl
+
 }  // namespace
 
 PYBIND11_MODULE(_ncarray, m) {
@@ -499,6 +642,26 @@ PYBIND11_MODULE(_ncarray, m) {
                   "layout and dtype (NCDevArrayRef of a list); the rows are kept alive.")
       .def_property_readonly("pointer_axis", &NCDevArray::pointer_axis, "The NC pointer axis, or -1.")
       .def("dense", &NCDevArray::dense, "This array, or a C-order copy of a pointer-axis array.")
+      .def("__dlpack__", [](const NCDevArray& a, py::object stream, py::object max_version, py::object dl_device,
+                            py::object copy) {
+        (void)stream;  // the library's calls without a stream have completed when they return
+        (void)max_version;
+        (void)dl_device;
+        if (!copy.is_none() && copy.cast<bool>()) return dl::to_capsule(a.copy());
+        return dl::to_capsule(a);
+      }, py::arg("stream") = py::none(), py::arg("max_version") = py::none(), py::arg("dl_device") = py::none(),
+           py::arg("copy") = py::none(), "A DLPack capsule of this array (zero-copy; the capsule keeps it alive).")
+      .def("__dlpack_device__", [](const NCDevArray&) {
+        int32_t dev = 0;
+        (void)nca_hip_current_device(&dev);
+        return py::make_tuple(static_cast<int>(dl::kROCM), dev);
+      })
+      .def_static("from_dlpack", [](const py::object& obj) {
+        if (py::isinstance<py::capsule>(obj)) return dl::from_capsule(obj.cast<py::capsule>());
+        if (!py::hasattr(obj, "__dlpack__")) throw py::type_error("object has no __dlpack__ and is not a DLPack capsule");
+        return dl::from_capsule(obj.attr("__dlpack__")().cast<py::capsule>());
+      }, py::arg("obj"), "Zero-copy view of a DLPack producer (an object with __dlpack__, e.g. a torch tensor, or a "
+                         "capsule); the producer's memory stays alive while the view does.")
       .def_property_readonly("__cuda_array_interface__", [](const NCDevArray& a) {
         if (a.pointer_axis() >= 0)
           throw Error(NCA_ERR_UNSUPPORTED, "a pointer-axis array has no __cuda_array_interface__ (use dense())");

@@ -10,6 +10,7 @@ import subprocess
 
 import numpy as np
 import pytest
+import torch
 
 import ncarray_amd as nca
 from ncarray_amd import capi, tapes
@@ -251,3 +252,51 @@ def test_errors_are_raised():
         a.to_device()
     with pytest.raises(nca.NcaError):
         a[:, ::2].reshape([6])
+
+
+# ---- DLPack ---------------------------------------------------------------------------------------------
+def test_dlpack_from_torch_is_zero_copy():
+    t = torch.arange(12, dtype=torch.float32, device="cuda").reshape(3, 4)
+    a = nca.from_dlpack(t)
+    assert a.shape == (3, 4) and a.data_ptr == t.data_ptr() and a.dtype == nca.DType.float32
+    assert np.array_equal(np.array(a), t.cpu().numpy())
+    a.fill(5.0)                      # writes land in the torch tensor's memory
+    torch.cuda.synchronize()
+    assert bool((t == 5.0).all())
+    tt = torch.arange(12, dtype=torch.int64, device="cuda").reshape(3, 4).t()   # strides (1, 4) elements
+    b = nca.from_dlpack(tt)
+    assert b.strides == (8, 32) and np.array_equal(np.array(b), tt.cpu().numpy())
+
+
+def test_dlpack_to_torch_is_zero_copy():
+    a = nca.NCDevArray.from_host(np.arange(24, dtype=np.int32).reshape(2, 3, 4))
+    t = torch.from_dlpack(a)
+    assert t.is_cuda and t.data_ptr() == a.data_ptr and t.dtype == torch.int32 and tuple(t.shape) == (2, 3, 4)
+    assert np.array_equal(t.cpu().numpy(), np.arange(24).reshape(2, 3, 4))
+    t += 1                           # torch writes are visible through the array
+    torch.cuda.synchronize()
+    assert np.array_equal(np.array(a), np.arange(24).reshape(2, 3, 4) + 1)
+    v = a[:, 1:, ::2]                # a strided view exports its element strides
+    tv = torch.from_dlpack(v)
+    assert np.array_equal(tv.cpu().numpy(), (np.arange(24).reshape(2, 3, 4) + 1)[:, 1:, ::2])
+
+
+@pytest.mark.parametrize("tdt", [torch.bfloat16, torch.float16, torch.bool, torch.uint8, torch.float64])
+def test_dlpack_dtypes_round_trip(tdt):
+    src = (torch.arange(10, device="cuda") % 3).to(tdt)
+    a = nca.from_dlpack(src)
+    back = torch.from_dlpack(a)
+    assert back.dtype == tdt and back.data_ptr() == src.data_ptr() and torch.equal(back, src)
+
+
+def test_dlpack_keeps_the_producer_alive():
+    t = torch.full((1 << 20,), 7.0, device="cuda")
+    ptr = t.data_ptr()
+    a = nca.from_dlpack(t)
+    del t
+    junk = [torch.full((1 << 20,), 1.0, device="cuda") for _ in range(4)]   # would reuse freed blocks
+    torch.cuda.synchronize()
+    assert a.data_ptr == ptr and float(np.array(a).min()) == 7.0 and all(j.data_ptr() != ptr for j in junk)
+    cap = nca.NCDevArray.from_host(np.ones(4, np.float32)).__dlpack__()   # never consumed: freed with it
+    del cap
+    assert nca.from_dlpack(nca.NCDevArray.from_host(np.ones(3, np.float32)).__dlpack__()).shape == (3,)
