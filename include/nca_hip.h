@@ -261,7 +261,8 @@ int nca_hip_expr_vjp(uint64_t handle, const nca_view* ins, int32_t n_in,
                      const void* grad_scalar, const int32_t* wrt, const nca_view* grads,
                      void* stream);
 /* The generated HIP source of a compiled variant (for inspection and tests);
- * variant 0 = contiguous, 1 = strided, 2 = fused-reduce, 3 = vjp, 4 = ragged batch.
+ * variant 0 = contiguous, 1 = strided, 2 = fused-reduce, 3 = vjp, 4 = ragged batch,
+ * 5 = tiled transpose (strided operands staged through LDS along axis 0).
  * Does not need a GPU.  Returns the full length; copies at most n-1 bytes. */
 int nca_hip_expr_source(uint64_t handle, int32_t variant, int32_t ndim, char* buf, size_t n);
 

@@ -611,6 +611,109 @@ hipFunction_t program_kernel(ExprProgram& p, const std::string& key,
   return fn;
 }
 
+// ---- tiled transpose form ------------------------------------------------------------------------
+// A strided expression whose output runs unit-stride along its last coalesced axis L while an
+// operand runs unit-stride along another axis A (a permuted copy such as x.transpose(0, 2) into a
+// dense array) reads that operand 4 bytes per 64-lane row at a large stride in the plain strided
+// form.  The tiled form stages 64 x 64 (A, L) tiles of every such operand through LDS, reading
+// them coalesced along A, and then computes and stores coalesced along L.  Operands already
+// unit-stride along L are read directly in the second phase.
+struct TransposePlan {
+  int axis = -1;          // A, or -1: the plain strided form
+  std::vector<char> lds;  // operand k is staged through LDS
+};
+
+static constexpr int kTransposeTile = 64;
+
+static TransposePlan transpose_plan(const Tape& t, const Operands& ops, const nca_view& out, int fuse) {
+  TransposePlan tp;
+  if (ops.contiguous || fuse || ops.nc || t.uses_idx || ops.sndim < 2) return tp;
+  const int L = ops.sndim - 1;
+  if (ops.sout[L] != itemsize(out.dtype) || ops.sshape[L] < 16) return tp;
+  for (int k : ops.unique) {
+    if (ops.bcast[k]) continue;
+    const int64_t it = itemsize(t.arr_dtypes[k]);
+    if (ops.sstrides[k][L] == it) continue;
+    for (int a = 0; a < L && tp.axis < 0; ++a)
+      if (ops.sstrides[k][a] == it && ops.sshape[a] >= 16) tp.axis = a;
+    if (tp.axis >= 0) break;
+  }
+  if (tp.axis < 0) return tp;
+  tp.lds.assign(t.arr_dtypes.size(), 0);
+  int64_t lds_bytes = 0;
+  for (int k : ops.unique) {
+    if (ops.bcast[k]) continue;
+    const int64_t it = itemsize(t.arr_dtypes[k]);
+    if (ops.sstrides[k][tp.axis] == it && ops.sstrides[k][L] != it &&
+        lds_bytes + it * kTransposeTile * (kTransposeTile + 1) <= (int64_t(64) << 10)) {
+      tp.lds[k] = 1;
+      lds_bytes += it * kTransposeTile * (kTransposeTile + 1);
+    }
+  }
+  return tp;
+}
+
+// Kernel of the tiled form; same parameters as the strided form.  Blocks of 256 threads walk the
+// tiles (L-tile fastest, then A-tile, then the other axes in row-major order); thread (tx, ty) =
+// (tid % 64, tid / 64) covers rows ty, ty + 4, ... of a tile.
+static std::string gen_transpose(const Tape& t, const Operands& ops, const TransposePlan& tp, int32_t out_dtype,
+                                 const std::string& kname) {
+  const std::string C = ctype_of(t.compute_dtype);
+  const std::string D = ctype_of(out_dtype);
+  std::vector<std::string> roots;
+  std::string body = emit_nodes(t, {t.root}, ops.alias, load_names(t), scalar_names(t), "lin", "t", roots);
+  std::ostringstream os;
+  os << strided_prelude(ops.sndim);
+  os << "// tape: " << t.signature << " (tiled transpose, axis " << tp.axis << ")\n";
+  os << "extern \"C\" __global__ __launch_bounds__(256) void " << kname << "("
+     << param_list(t, ops, true, D, true) << ") {\n";
+  os << "  constexpr int T = " << kTransposeTile << ", A = " << tp.axis << ", L = NCA_ND - 1;\n";
+  for (int k : ops.unique)
+    if (!ops.bcast[k] && tp.lds[k]) os << "  __shared__ " << ctype_of(t.arr_dtypes[k]) << " sk" << k << "[T][T + 1];\n";
+  os << bcast_loads(t, ops);
+  os << "  const nca_i64 nA = shp.v[A], nL = shp.v[L];\n"
+     << "  const nca_i64 tA = (nA + T - 1) / T, tL = (nL + T - 1) / T;\n"
+     << "  const nca_i64 ntiles = (n / (nA * nL)) * tA * tL;\n"
+     << "  const int tx = threadIdx.x & (T - 1), ty = threadIdx.x / T;\n"
+     << "  for (nca_i64 tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {\n"
+     << "    nca_i64 rem = tile;\n"
+     << "    const nca_i64 l0 = (rem % tL) * T;\n    rem /= tL;\n"
+     << "    const nca_i64 a0 = (rem % tA) * T;\n    rem /= tA;\n";
+  for (int k : ops.unique)
+    if (!ops.bcast[k]) os << "    nca_i64 o" << k << " = 0;\n";
+  os << "    nca_i64 oo = 0;\n";
+  os << "#pragma unroll\n    for (int d = NCA_ND - 2; d >= 0; --d) {\n      if (d == A) continue;\n"
+     << "      const nca_i64 c = rem % shp.v[d];\n      rem /= shp.v[d];\n";
+  for (int k : ops.unique)
+    if (!ops.bcast[k]) os << "      o" << k << " += c * st" << k << ".v[d];\n";
+  os << "      oo += c * sto.v[d];\n    }\n";
+  // phase 1: stage the operands that are unit-stride along A (tx runs along A)
+  os << "    for (int j = ty; j < T; j += 4) {\n      const nca_i64 a = a0 + tx, l = l0 + j;\n"
+     << "      if (a < nA && l < nL) {\n";
+  for (int k : ops.unique)
+    if (!ops.bcast[k] && tp.lds[k])
+      os << "        sk" << k << "[j][tx] = *reinterpret_cast<const " << ctype_of(t.arr_dtypes[k]) << "*>(p" << k
+         << " + o" << k << " + a * st" << k << ".v[A] + l * st" << k << ".v[L]);\n";
+  os << "      }\n    }\n    __syncthreads();\n";
+  // phase 2: compute and store (tx runs along L)
+  os << "    for (int j = ty; j < T; j += 4) {\n      const nca_i64 a = a0 + j, l = l0 + tx;\n"
+     << "      if (a < nA && l < nL) {\n";
+  for (int k : ops.unique) {
+    if (ops.bcast[k]) continue;
+    const std::string Tk = ctype_of(t.arr_dtypes[k]);
+    if (tp.lds[k])
+      os << "        const " << C << " u" << k << " = nca_cvt<" << C << ", " << Tk << ">::from(sk" << k << "[tx][j]);\n";
+    else
+      os << "        const " << C << " u" << k << " = nca_ld<" << C << ">(reinterpret_cast<const " << Tk << "*>(p" << k
+         << " + o" << k << " + a * st" << k << ".v[A] + l * st" << k << ".v[L]));\n";
+  }
+  os << body;
+  os << "        *reinterpret_cast<" << D << "*>(outp + oo + a * sto.v[A] + l * sto.v[L]) = nca_store_cvt<" << D
+     << ", " << C << ">::to(" << roots[0] << ");\n";
+  os << "      }\n    }\n    __syncthreads();\n  }\n}\n";
+  return os.str();
+}
+
 void expr_launch_impl(uint64_t handle, const nca_view* ins, int n_in, const nca_scalar* scalars,
                       int n_scalars, const nca_view* outp, int fuse, int mode, void* sum_out,
                       void* max_out, void* min_out, hipStream_t s) {
@@ -626,6 +729,22 @@ void expr_launch_impl(uint64_t handle, const nca_view* ins, int n_in, const nca_
   DeviceState& dev = current_device();
   const bool strided = !ops.contiguous;
   const bool with_out = true;
+  const TransposePlan tp = transpose_plan(t, ops, out, fuse);
+  if (tp.axis >= 0 && ops.n > 0) {
+    std::string key = "t" + std::to_string(ops.sndim) + "a" + std::to_string(tp.axis) + "|" + sig_operands(t, ops) + "|";
+    for (int k : ops.unique) key += tp.lds[k] ? '1' : '0';
+    const std::string kname = "nca_expr_transpose";
+    hipFunction_t fn = program_kernel(p, key, [&]() { return gen_transpose(t, ops, tp, out.dtype, kname); }, kname);
+    LaunchArgs la;
+    push_common(la, t, ops, ins, out, true, with_out, sc);
+    const int64_t nA = ops.sshape[tp.axis], nL = ops.sshape[ops.sndim - 1];
+    const int64_t tiles = (ops.n / (nA * nL)) * ((nA + kTransposeTile - 1) / kTransposeTile) *
+                          ((nL + kTransposeTile - 1) / kTransposeTile);
+    const int blocks = static_cast<int>(std::min<int64_t>(tiles, int64_t(dev.n_cus) * 8));
+    ProfScope ps("nca_expr_transpose", s);
+    jit_launch(fn, dim3(blocks), dim3(256), 0, s, la.params);
+    return;
+  }
   std::string key = std::string(strided ? "s" : "c") + std::to_string(strided ? ops.sndim : ops.vec) + "|" +
                     sig_operands(t, ops) + "|f" + std::to_string(fuse) + "m" + std::to_string(mode);
   const std::string kname = fuse ? "nca_expr_reduce" : (strided ? "nca_expr_strided" : "nca_expr_contig");
@@ -867,6 +986,19 @@ int nca_hip_expr_source(uint64_t handle, int32_t variant, int32_t ndim, char* bu
         break;
       }
       case 4: src = gen_batch_source(t, NCA_FUSE_SUM | NCA_FUSE_MAX, NCA_ACCUM_PLAIN, 4, true); break;
+      case 5: {
+        NCA_REQUIRE(ops.sndim >= 2, "the tiled transpose form needs ndim >= 2");
+        TransposePlan tp;
+        tp.axis = 0;
+        tp.lds.assign(t.arr_dtypes.size(), 0);
+        int64_t lds_bytes = 0;
+        for (size_t k = 0; k < t.arr_dtypes.size(); ++k) {
+          const int64_t b = itemsize(t.arr_dtypes[k]) * kTransposeTile * (kTransposeTile + 1);
+          if (lds_bytes + b <= (int64_t(64) << 10)) { tp.lds[k] = 1; lds_bytes += b; }
+        }
+        src = gen_transpose(t, ops, tp, t.dest_dtype, "nca_expr_transpose");
+        break;
+      }
       default: NCA_THROW(NCA_ERR_INVALID, "unknown variant");
     }
     src = device_prelude() + "\n" + src;
