@@ -262,7 +262,8 @@ int nca_hip_expr_vjp(uint64_t handle, const nca_view* ins, int32_t n_in,
                      void* stream);
 /* The generated HIP source of a compiled variant (for inspection and tests);
  * variant 0 = contiguous, 1 = strided, 2 = fused-reduce, 3 = vjp, 4 = ragged batch,
- * 5 = tiled transpose (strided operands staged through LDS along axis 0).
+ * 5 = tiled transpose (strided operands staged through LDS along axis 0),
+ * 6 = rows (one wave per row of the last axis).
  * Does not need a GPU.  Returns the full length; copies at most n-1 bytes. */
 int nca_hip_expr_source(uint64_t handle, int32_t variant, int32_t ndim, char* buf, size_t n);
 

@@ -348,7 +348,8 @@ static void segment_update(std::ostringstream& os, const std::vector<int>& a, co
   os << o << "_" << a.size() << " += c * " << st << ".v[d];\n";
 }
 
-static std::string strided_offsets(const Tape& t, const Operands& ops, bool with_out) {
+static std::string strided_offsets(const Tape& t, const Operands& ops, bool with_out,
+                                   const std::string& var = "idx", bool skip_last = false) {
   std::ostringstream os;
   auto indirect = [&](int k) { return ops.nc && !ops.ind[k].empty(); };
   const bool indo = with_out && ops.nc && !ops.indo.empty();
@@ -357,7 +358,7 @@ static std::string strided_offsets(const Tape& t, const Operands& ops, bool with
     for (size_t j = 1; j <= m; ++j) os << ", " << o << "_" << j << " = 0";
     os << ";\n";
   };
-  os << "      nca_i64 rem = idx;\n";
+  os << "      nca_i64 rem = " << var << ";\n";
   for (int k : ops.unique)
     if (!ops.bcast[k]) {
       if (indirect(k)) decl("o" + std::to_string(k), ops.ind[k].size());
@@ -367,7 +368,7 @@ static std::string strided_offsets(const Tape& t, const Operands& ops, bool with
     if (indo) decl("oo", ops.indo.size());
     else os << "      nca_i64 oo = 0;\n";
   }
-  os << "#pragma unroll\n      for (int d = NCA_ND - 1; d >= 0; --d) {\n"
+  os << "#pragma unroll\n      for (int d = NCA_ND - " << (skip_last ? 2 : 1) << "; d >= 0; --d) {\n"
      << "        const nca_i64 c = rem % shp.v[d];\n        rem /= shp.v[d];\n";
   for (int k : ops.unique) {
     if (ops.bcast[k]) continue;
@@ -714,6 +715,72 @@ static std::string gen_transpose(const Tape& t, const Operands& ops, const Trans
   return os.str();
 }
 
+// ---- row form ------------------------------------------------------------------------------------
+// The strided form unravels every element index over all axes (a 64-bit divide per axis) and, for
+// indirect views, follows the pointer chain per element.  When the last coalesced axis L holds a
+// wave's worth of elements and every pointer step lies before L, one wave takes one row: the row
+// index is unravelled and each chain followed once per row, and the lanes step along L, two
+// elements per lane per step.
+static bool row_form_ok(const Operands& ops, int fuse) {
+  if (ops.contiguous || fuse || ops.sndim < 2) return false;
+  const int L = ops.sndim - 1;
+  if (ops.sshape[L] < 64) return false;
+  if (ops.nc) {
+    for (int k : ops.unique)
+      if (!ops.bcast[k] && !ops.ind[k].empty() && ops.ind[k].back() >= L) return false;
+    if (!ops.indo.empty() && ops.indo.back() >= L) return false;
+  }
+  return true;
+}
+
+static std::string gen_rows(const Tape& t, const Operands& ops, int32_t out_dtype, const std::string& kname) {
+  const std::string C = ctype_of(t.compute_dtype);
+  const std::string D = ctype_of(out_dtype);
+  std::vector<std::string> roots;
+  std::string body = emit_nodes(t, {t.root}, ops.alias, load_names(t), scalar_names(t), "lin", "t", roots);
+  std::ostringstream os;
+  os << strided_prelude(ops.sndim);
+  os << "// tape: " << t.signature << " (rows)\n";
+  os << "extern \"C\" __global__ __launch_bounds__(256) void " << kname << "("
+     << param_list(t, ops, true, D, true) << ") {\n";
+  os << bcast_loads(t, ops);
+  os << "  constexpr int L = NCA_ND - 1, U = 2;\n"
+     << "  const nca_i64 nL = shp.v[L], nrows = n / nL;\n"
+     << "  const int lane = threadIdx.x & 63;\n"
+     << "  const nca_i64 nwaves = ((nca_i64)gridDim.x * blockDim.x) >> 6;\n"
+     << "  for (nca_i64 row = ((nca_i64)blockIdx.x * blockDim.x + threadIdx.x) >> 6; row < nrows; row += nwaves) {\n";
+  os << strided_offsets(t, ops, true, "row", true);
+  for (int k : ops.unique)
+    if (!ops.bcast[k])
+      os << "      const char* rb" << k << " = " << operand_address(ops, k) << ";\n";
+  if (ops.nc && !ops.indo.empty())
+    os << "      char* rbo = (char*)" << chain_address("outp", "oo", "qo_", ops.indo.size()) << ";\n";
+  else
+    os << "      char* rbo = outp + oo;\n";
+  os << "      for (nca_i64 l0 = lane; l0 < nL; l0 += 64 * U) {\n";
+  for (int k : ops.unique)
+    if (!ops.bcast[k]) os << "        " << ctype_of(t.arr_dtypes[k]) << " x" << k << "[U];\n";
+  os << "#pragma unroll\n        for (int q = 0; q < U; ++q) {\n          const nca_i64 l = l0 + 64 * q;\n"
+     << "          if (l < nL) {\n";
+  for (int k : ops.unique)
+    if (!ops.bcast[k])
+      os << "            x" << k << "[q] = *reinterpret_cast<const " << ctype_of(t.arr_dtypes[k]) << "*>(rb" << k
+         << " + l * st" << k << ".v[L]);\n";
+  os << "          }\n        }\n";
+  os << "#pragma unroll\n        for (int q = 0; q < U; ++q) {\n          const nca_i64 l = l0 + 64 * q;\n"
+     << "          if (l < nL) {\n";
+  for (int k : ops.unique)
+    if (!ops.bcast[k])
+      os << "            const " << C << " u" << k << " = nca_cvt<" << C << ", " << ctype_of(t.arr_dtypes[k])
+         << ">::from(x" << k << "[q]);\n";
+  if (t.uses_idx) os << "            const nca_i64 lin = row * nL + l;\n";
+  os << body;
+  os << "            *reinterpret_cast<" << D << "*>(rbo + l * sto.v[L]) = nca_store_cvt<" << D << ", " << C << ">::to("
+     << roots[0] << ");\n";
+  os << "          }\n        }\n      }\n  }\n}\n";
+  return os.str();
+}
+
 void expr_launch_impl(uint64_t handle, const nca_view* ins, int n_in, const nca_scalar* scalars,
                       int n_scalars, const nca_view* outp, int fuse, int mode, void* sum_out,
                       void* max_out, void* min_out, hipStream_t s) {
@@ -742,6 +809,18 @@ void expr_launch_impl(uint64_t handle, const nca_view* ins, int n_in, const nca_
                           ((nL + kTransposeTile - 1) / kTransposeTile);
     const int blocks = static_cast<int>(std::min<int64_t>(tiles, int64_t(dev.n_cus) * 8));
     ProfScope ps("nca_expr_transpose", s);
+    jit_launch(fn, dim3(blocks), dim3(256), 0, s, la.params);
+    return;
+  }
+  if (row_form_ok(ops, fuse) && ops.n > 0) {
+    const std::string key = "r" + std::to_string(ops.sndim) + "|" + sig_operands(t, ops);
+    const std::string kname = "nca_expr_rows";
+    hipFunction_t fn = program_kernel(p, key, [&]() { return gen_rows(t, ops, out.dtype, kname); }, kname);
+    LaunchArgs la;
+    push_common(la, t, ops, ins, out, true, with_out, sc);
+    const int64_t rows = ops.n / ops.sshape[ops.sndim - 1];
+    const int blocks = grid_blocks(rows * 64, 256, dev.n_cus * 16);
+    ProfScope ps("nca_expr_rows", s);
     jit_launch(fn, dim3(blocks), dim3(256), 0, s, la.params);
     return;
   }
@@ -999,6 +1078,10 @@ int nca_hip_expr_source(uint64_t handle, int32_t variant, int32_t ndim, char* bu
         src = gen_transpose(t, ops, tp, t.dest_dtype, "nca_expr_transpose");
         break;
       }
+      case 6:
+        NCA_REQUIRE(ops.sndim >= 2, "the row form needs ndim >= 2");
+        src = gen_rows(t, ops, t.dest_dtype, "nca_expr_rows");
+        break;
       default: NCA_THROW(NCA_ERR_INVALID, "unknown variant");
     }
     src = device_prelude() + "\n" + src;

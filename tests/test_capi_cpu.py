@@ -78,7 +78,7 @@ def _compiles(src):
 @pytest.mark.parametrize("dt", [capi.FLOAT32, capi.FLOAT64, capi.BFLOAT16, capi.FLOAT16])
 def test_expression_kernels_compile(dt):
     t = capi.Tape(tapes.fused_c0(), [dt] * 5, [capi.FLOAT32], dt, dt)
-    for variant in (0, 1, 2, 3, 4, 5):
+    for variant in (0, 1, 2, 3, 4, 5, 6):
         _compiles(t.source(variant, 3))
 
 

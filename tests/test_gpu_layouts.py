@@ -1,10 +1,11 @@
-"""Permuted (transposing) copies and expressions on the GPU through the C ABI.
+"""Permuted (transposing), strided and indirect copies and expressions on the GPU through the C ABI.
 
 A strided expression whose output is unit-stride along its last axis while an operand is
 unit-stride along another axis runs in the tiled transpose form (64 x 64 tiles staged through
 LDS).  Copies are checked bit-exactly against torch's own permuted copy; expressions are checked
 bit-exactly against the same tape evaluated in the contiguous form on contiguous copies of the
-operands (the per-element code is the same, only the addressing differs)."""
+operands (the per-element code is the same, only the addressing differs).  Strided and NC layouts
+whose last axis holds 64 or more elements run in the row form (one wave per row)."""
 import numpy as np
 import pytest
 import torch
@@ -106,3 +107,57 @@ def test_large_transpose_exact(dev):
         got = _copy(x.permute(*perm))
         assert torch.equal(got, x.permute(*perm).contiguous()), perm
         del got
+
+
+# ---- row form: one wave per row of the last axis (strided and indirect layouts) ----------------
+
+@pytest.mark.parametrize("shape,sl", [
+    ((40, 33, 300), (slice(None), slice(None), slice(None, None, 2))),
+    ((7, 5, 3, 130), (slice(None), slice(1, 4), slice(None), slice(None, 129))),
+    ((65, 191), (slice(None, None, 3), slice(None))),
+    ((9, 64), (slice(None, None, 2), slice(None))),
+])
+@pytest.mark.parametrize("dtype", [torch.float32, torch.bfloat16, torch.int64, torch.uint8])
+def test_strided_rows_exact(dev, shape, sl, dtype):
+    x = _rand(shape, dtype, dev, seed=len(shape))
+    xs = x[sl]
+    assert torch.equal(_copy(xs), xs.contiguous())
+
+
+def test_nc_rows_read_write_exact(dev):
+    capi, _, ti = _capi()
+    P = capi.pack
+    ident = capi.Tape([P(capi.OP_LOAD_NCARR, 0)], [capi.FLOAT32], [], capi.FLOAT32, capi.FLOAT32)
+    x = _rand((37, 20, 130), torch.float32, dev, seed=11)
+    tbl = torch.tensor([x[i].data_ptr() for i in range(37)], dtype=torch.int64, device=dev)
+    vx = capi.make_nc_view(tbl.data_ptr(), capi.FLOAT32, (37, 20, 130), [1, 130 * 4, 4], 0)
+    out = torch.empty_like(x)
+    ident.launch([vx], ti.view(out))
+    assert torch.equal(out, x)
+    # row_offset: every row read from its third element on
+    vo8 = capi.make_nc_view(tbl.data_ptr(), capi.FLOAT32, (37, 20, 128), [1, 130 * 4, 4], 0, row_offset=8)
+    out8 = torch.empty((37, 20, 128), dtype=torch.float32, device=dev)
+    ident.launch([vo8], ti.view(out8))
+    assert torch.equal(out8, x[:, :, 2:])
+    # an NC output: planes of y written through their table, from a transposed operand and a
+    # plain one
+    y = torch.zeros_like(x)
+    tbly = torch.tensor([y[i].data_ptr() for i in range(37)], dtype=torch.int64, device=dev)
+    vy = capi.make_nc_view(tbly.data_ptr(), capi.FLOAT32, (37, 20, 130), [1, 130 * 4, 4], 0)
+    z = _rand((130, 20, 37), torch.float32, dev, seed=12)
+    add = capi.Tape([P(capi.OP_LOAD_NCARR, 0), P(capi.OP_LOAD_NCARR, 1), P(capi.OP_ADD)],
+                    [capi.FLOAT32] * 2, [], capi.FLOAT32, capi.FLOAT32)
+    add.launch([ti.view(x), ti.view(z.permute(2, 1, 0))], vy)
+    assert torch.equal(y, x + z.permute(2, 1, 0))
+
+
+def test_rows_linear_index(dev):
+    """IDX is the C-order linear index of the output element in the row form too."""
+    capi, _, ti = _capi()
+    P = capi.pack
+    ins = [P(capi.OP_IDX), P(capi.OP_LOAD_CONST, 0), P(capi.OP_MUL), P(capi.OP_LOAD_CONST, 1), P(capi.OP_ADD)]
+    big = torch.zeros((3, 10, 200), dtype=torch.int64, device=dev)
+    o = big[:, ::2, :]
+    capi.Tape(ins, [], [capi.INT64, capi.INT64], capi.INT64, capi.INT64).launch([], ti.view(o), scalars=(2, 1))
+    assert torch.equal(o.cpu(), (torch.arange(3 * 5 * 200) * 2 + 1).view(3, 5, 200))
+    assert (big[:, 1::2, :] == 0).all()
