@@ -47,9 +47,10 @@ def parse():
                     help="also run a plain 512 MiB device copy (for PMC byte-counter calibration)")
     ap.add_argument("--only-headline", action="store_true",
                     help="skip the other workloads (expression, binning, autograd, ragged batch)")
-    ap.add_argument("--workloads", default="c0,c2,c3,c4",
+    ap.add_argument("--workloads", default="c0,c2,c3,c4,cp",
                     help="comma list of the other workloads to run: c0 (fused expression + sum), c2 (binning), "
-                         "c3 (autograd pipeline), c4 (ragged batch)")
+                         "c3 (autograd pipeline), c4 (ragged batch), cp (casting, strided, permuted and "
+                         "pointer-table copies)")
     return ap.parse_args()
 
 
@@ -188,7 +189,8 @@ def main():
     if not args.only_headline:
         chosen = {w.strip() for w in args.workloads.split(",") if w.strip()}
         for name, fn in (("c0_fused_expression_sum", bench_c0), ("c2_binning", bench_c2),
-                         ("c3_autograd_pipeline", bench_c3), ("c4_ragged_batch", bench_c4)):
+                         ("c3_autograd_pipeline", bench_c3), ("c4_ragged_batch", bench_c4),
+                         ("cp_layout_copies", bench_cp)):
             if name.split("_")[0] not in chosen:
                 continue
             try:
@@ -515,6 +517,46 @@ def bench_c3(torch, capi, tapes, ti, dev, sptr, time_loop):
                    "those of a laplacian_c0_value_and_grad step (HIP events per launch); 134 MB working set: "
                    "cache-resident, except hbm_cold_* (device time after reading 512 MiB flushed the "
                    "Infinity Cache)")
+    return res
+
+
+def bench_cp(torch, capi, tapes, ti, dev, sptr, time_loop):
+    """Copies of a 512^3 float32 array (512 MiB) into dense arrays: a casting copy to BF16, a plain
+    copy, a strided slice, two permuted copies and a gather of its 512 planes through an NC table of
+    plane pointers.  Kernel time from the HIP events the library records around each launch;
+    algorithmic bytes = elements x (source + destination itemsize)."""
+    n = 512
+    x = torch.randn((n, n, n), device=dev)
+    one = lambda sd, dd: capi.Tape([capi.pack(capi.OP_LOAD_NCARR, 0)], [sd], [], capi.FLOAT32, dd)  # noqa: E731
+    f32, to_bf16 = one(capi.FLOAT32, capi.FLOAT32), one(capi.FLOAT32, capi.BFLOAT16)
+    dense = torch.empty_like(x)
+    half = torch.empty((n, n, n // 2), device=dev)
+    ob = torch.empty((n, n, n), dtype=torch.bfloat16, device=dev)
+    tbl = torch.tensor([x[i].data_ptr() for i in range(n)], dtype=torch.int64, device=dev)
+    vnc = capi.make_nc_view(tbl.data_ptr(), capi.FLOAT32, (n, n, n), [1, n * 4, 4], 0)
+    S = capi.STREAM_DEFAULT_ASYNC
+    cases = [
+        ("cast_f32_to_bf16", lambda: to_bf16.launch([ti.view(x)], ti.view(ob), stream=S), n ** 3 * 6),
+        ("copy_f32", lambda: f32.launch([ti.view(x)], ti.view(dense), stream=S), n ** 3 * 8),
+        ("slice_last_axis_step2", lambda: f32.launch([ti.view(x[:, :, ::2])], ti.view(half), stream=S), n ** 3 * 4),
+        ("transpose_1_2", lambda: f32.launch([ti.view(x.transpose(1, 2))], ti.view(dense), stream=S), n ** 3 * 8),
+        ("transpose_0_2", lambda: f32.launch([ti.view(x.transpose(0, 2))], ti.view(dense), stream=S), n ** 3 * 8),
+        ("nc_planes_to_dense", lambda: f32.launch([vnc], ti.view(dense), stream=S), n ** 3 * 8),
+    ]
+    res = {}
+    for name, fn, nbytes in cases:
+        for _ in range(3):
+            fn()
+        torch.cuda.synchronize()
+        capi.profile_enable(True)
+        for _ in range(10):
+            fn()
+        torch.cuda.synchronize()
+        recs = capi.profile_read()
+        capi.profile_enable(False)
+        ms = float(np.median([k for _, k in recs]))
+        res[name] = {"kernel": recs[0][0], "kernel_ms": round(ms, 4), "GBps": gbps(nbytes, ms),
+                     "frac_of_8TBps": round(nbytes / (ms * 1e-3) / 1e9 / HBM_PEAK_GBPS, 4)}
     return res
 
 
