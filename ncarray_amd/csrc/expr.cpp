@@ -14,6 +14,7 @@
 //   vjp:        derivative tapes (symbolic) times the upstream gradient.
 #include <cstdio>
 #include <sstream>
+#include <tuple>
 #include <unordered_map>
 
 #include "expr_internal.hh"
@@ -618,15 +619,28 @@ hipFunction_t program_kernel(ExprProgram& p, const std::string& key,
 // A strided expression whose output runs unit-stride along its last coalesced axis L while an
 // operand runs unit-stride along another axis A (a permuted copy such as x.transpose(0, 2) into a
 // dense array) reads that operand 4 bytes per 64-lane row at a large stride in the plain strided
-// form.  The tiled form stages 64 x 64 (A, L) tiles of every such operand through LDS, reading
+// form.  The tiled form stages TA x TL (A, L) tiles of every such operand through LDS, reading
 // them coalesced along A, and then computes and stores coalesced along L.  Operands already
 // unit-stride along L are read directly in the second phase.
 struct TransposePlan {
-  int axis = -1;          // A, or -1: the plain strided form
+  int axis = -1;          // A, or -1: another form
+  int ta = 64, tl = 64;   // tile extents along A and L (powers of two, 16..256)
   std::vector<char> lds;  // operand k is staged through LDS
 };
 
-static constexpr int kTransposeTile = 64;
+// Tile extents: 64 x 64 unless NCA_HIP_TRANSPOSE_TILE = "<TA>x<TL>" names two powers of two in
+// 16..256.
+static std::pair<int, int> transpose_tile() {
+  static const std::pair<int, int> tile = [] {
+    std::pair<int, int> d{64, 64};
+    const char* e = std::getenv("NCA_HIP_TRANSPOSE_TILE");
+    int a = 0, l = 0;
+    if (!e || std::sscanf(e, "%dx%d", &a, &l) != 2) return d;
+    auto ok = [](int v) { return v >= 16 && v <= 256 && (v & (v - 1)) == 0; };
+    return ok(a) && ok(l) ? std::pair<int, int>{a, l} : d;
+  }();
+  return tile;
+}
 
 static TransposePlan transpose_plan(const Tape& t, const Operands& ops, const nca_view& out, int fuse) {
   TransposePlan tp;
@@ -642,23 +656,27 @@ static TransposePlan transpose_plan(const Tape& t, const Operands& ops, const nc
     if (tp.axis >= 0) break;
   }
   if (tp.axis < 0) return tp;
+  std::tie(tp.ta, tp.tl) = transpose_tile();
   tp.lds.assign(t.arr_dtypes.size(), 0);
   int64_t lds_bytes = 0;
   for (int k : ops.unique) {
     if (ops.bcast[k]) continue;
     const int64_t it = itemsize(t.arr_dtypes[k]);
-    if (ops.sstrides[k][tp.axis] == it && ops.sstrides[k][L] != it &&
-        lds_bytes + it * kTransposeTile * (kTransposeTile + 1) <= (int64_t(64) << 10)) {
+    const int64_t b = it * tp.tl * (tp.ta + 1);
+    if (ops.sstrides[k][tp.axis] == it && ops.sstrides[k][L] != it && lds_bytes + b <= (int64_t(64) << 10)) {
       tp.lds[k] = 1;
-      lds_bytes += it * kTransposeTile * (kTransposeTile + 1);
+      lds_bytes += b;
     }
   }
+  if (lds_bytes == 0) tp.axis = -1;  // no tile fits: another form
   return tp;
 }
 
 // Kernel of the tiled form; same parameters as the strided form.  Blocks of 256 threads walk the
-// tiles (L-tile fastest, then A-tile, then the other axes in row-major order); thread (tx, ty) =
-// (tid % 64, tid / 64) covers rows ty, ty + 4, ... of a tile.
+// tiles (L-tile fastest, then A-tile, then the other axes in row-major order).  Staging: thread
+// tid reads A position tid % TA of rows tid / TA, + 256 / TA, ...; evaluation: it takes L
+// position tid % TL of A rows tid / TL, + 256 / TL, ...  The LDS rows hold TA + 1 values, so a
+// column read by consecutive lanes falls on distinct banks.
 static std::string gen_transpose(const Tape& t, const Operands& ops, const TransposePlan& tp, int32_t out_dtype,
                                  const std::string& kname) {
   const std::string C = ctype_of(t.compute_dtype);
@@ -670,18 +688,18 @@ static std::string gen_transpose(const Tape& t, const Operands& ops, const Trans
   os << "// tape: " << t.signature << " (tiled transpose, axis " << tp.axis << ")\n";
   os << "extern \"C\" __global__ __launch_bounds__(256) void " << kname << "("
      << param_list(t, ops, true, D, true) << ") {\n";
-  os << "  constexpr int T = " << kTransposeTile << ", A = " << tp.axis << ", L = NCA_ND - 1;\n";
+  os << "  constexpr int TA = " << tp.ta << ", TL = " << tp.tl << ", A = " << tp.axis << ", L = NCA_ND - 1;\n";
   for (int k : ops.unique)
-    if (!ops.bcast[k] && tp.lds[k]) os << "  __shared__ " << ctype_of(t.arr_dtypes[k]) << " sk" << k << "[T][T + 1];\n";
+    if (!ops.bcast[k] && tp.lds[k]) os << "  __shared__ " << ctype_of(t.arr_dtypes[k]) << " sk" << k << "[TL][TA + 1];\n";
   os << bcast_loads(t, ops);
   os << "  const nca_i64 nA = shp.v[A], nL = shp.v[L];\n"
-     << "  const nca_i64 tA = (nA + T - 1) / T, tL = (nL + T - 1) / T;\n"
+     << "  const nca_i64 tA = (nA + TA - 1) / TA, tL = (nL + TL - 1) / TL;\n"
      << "  const nca_i64 ntiles = (n / (nA * nL)) * tA * tL;\n"
-     << "  const int tx = threadIdx.x & (T - 1), ty = threadIdx.x / T;\n"
+     << "  const int sx = threadIdx.x % TA, sy = threadIdx.x / TA, ex = threadIdx.x % TL, ey = threadIdx.x / TL;\n"
      << "  for (nca_i64 tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {\n"
      << "    nca_i64 rem = tile;\n"
-     << "    const nca_i64 l0 = (rem % tL) * T;\n    rem /= tL;\n"
-     << "    const nca_i64 a0 = (rem % tA) * T;\n    rem /= tA;\n";
+     << "    const nca_i64 l0 = (rem % tL) * TL;\n    rem /= tL;\n"
+     << "    const nca_i64 a0 = (rem % tA) * TA;\n    rem /= tA;\n";
   for (int k : ops.unique)
     if (!ops.bcast[k]) os << "    nca_i64 o" << k << " = 0;\n";
   os << "    nca_i64 oo = 0;\n";
@@ -691,21 +709,21 @@ static std::string gen_transpose(const Tape& t, const Operands& ops, const Trans
     if (!ops.bcast[k]) os << "      o" << k << " += c * st" << k << ".v[d];\n";
   os << "      oo += c * sto.v[d];\n    }\n";
   // phase 1: stage the operands that are unit-stride along A (tx runs along A)
-  os << "    for (int j = ty; j < T; j += 4) {\n      const nca_i64 a = a0 + tx, l = l0 + j;\n"
+  os << "    for (int j = sy; j < TL; j += 256 / TA) {\n      const nca_i64 a = a0 + sx, l = l0 + j;\n"
      << "      if (a < nA && l < nL) {\n";
   for (int k : ops.unique)
     if (!ops.bcast[k] && tp.lds[k])
-      os << "        sk" << k << "[j][tx] = *reinterpret_cast<const " << ctype_of(t.arr_dtypes[k]) << "*>(p" << k
+      os << "        sk" << k << "[j][sx] = *reinterpret_cast<const " << ctype_of(t.arr_dtypes[k]) << "*>(p" << k
          << " + o" << k << " + a * st" << k << ".v[A] + l * st" << k << ".v[L]);\n";
   os << "      }\n    }\n    __syncthreads();\n";
-  // phase 2: compute and store (tx runs along L)
-  os << "    for (int j = ty; j < T; j += 4) {\n      const nca_i64 a = a0 + j, l = l0 + tx;\n"
+  // phase 2: compute and store (ex runs along L)
+  os << "    for (int j = ey; j < TA; j += 256 / TL) {\n      const nca_i64 a = a0 + j, l = l0 + ex;\n"
      << "      if (a < nA && l < nL) {\n";
   for (int k : ops.unique) {
     if (ops.bcast[k]) continue;
     const std::string Tk = ctype_of(t.arr_dtypes[k]);
     if (tp.lds[k])
-      os << "        const " << C << " u" << k << " = nca_cvt<" << C << ", " << Tk << ">::from(sk" << k << "[tx][j]);\n";
+      os << "        const " << C << " u" << k << " = nca_cvt<" << C << ", " << Tk << ">::from(sk" << k << "[ex][j]);\n";
     else
       os << "        const " << C << " u" << k << " = nca_ld<" << C << ">(reinterpret_cast<const " << Tk << "*>(p" << k
          << " + o" << k << " + a * st" << k << ".v[A] + l * st" << k << ".v[L]));\n";
@@ -800,15 +818,15 @@ void expr_launch_impl(uint64_t handle, const nca_view* ins, int n_in, const nca_
   const bool with_out = true;
   const TransposePlan tp = transpose_plan(t, ops, out, fuse);
   if (tp.axis >= 0 && ops.n > 0) {
-    std::string key = "t" + std::to_string(ops.sndim) + "a" + std::to_string(tp.axis) + "|" + sig_operands(t, ops) + "|";
+    std::string key = "t" + std::to_string(ops.sndim) + "a" + std::to_string(tp.axis) + "x" + std::to_string(tp.ta) + "x" +
+                      std::to_string(tp.tl) + "|" + sig_operands(t, ops) + "|";
     for (int k : ops.unique) key += tp.lds[k] ? '1' : '0';
     const std::string kname = "nca_expr_transpose";
     hipFunction_t fn = program_kernel(p, key, [&]() { return gen_transpose(t, ops, tp, out.dtype, kname); }, kname);
     LaunchArgs la;
     push_common(la, t, ops, ins, out, true, with_out, sc);
     const int64_t nA = ops.sshape[tp.axis], nL = ops.sshape[ops.sndim - 1];
-    const int64_t tiles = (ops.n / (nA * nL)) * ((nA + kTransposeTile - 1) / kTransposeTile) *
-                          ((nL + kTransposeTile - 1) / kTransposeTile);
+    const int64_t tiles = (ops.n / (nA * nL)) * ((nA + tp.ta - 1) / tp.ta) * ((nL + tp.tl - 1) / tp.tl);
     const int blocks = static_cast<int>(std::min<int64_t>(tiles, int64_t(dev.n_cus) * 8));
     ProfScope ps("nca_expr_transpose", s);
     jit_launch(fn, dim3(blocks), dim3(256), 0, s, la.params);
@@ -1082,7 +1100,7 @@ int nca_hip_expr_source(uint64_t handle, int32_t variant, int32_t ndim, char* bu
         tp.lds.assign(t.arr_dtypes.size(), 0);
         int64_t lds_bytes = 0;
         for (size_t k = 0; k < t.arr_dtypes.size(); ++k) {
-          const int64_t b = itemsize(t.arr_dtypes[k]) * kTransposeTile * (kTransposeTile + 1);
+          const int64_t b = itemsize(t.arr_dtypes[k]) * tp.tl * (tp.ta + 1);
           if (lds_bytes + b <= (int64_t(64) << 10)) { tp.lds[k] = 1; lds_bytes += b; }
         }
         src = gen_transpose(t, ops, tp, t.dest_dtype, "nca_expr_transpose");
