@@ -625,6 +625,7 @@ hipFunction_t program_kernel(ExprProgram& p, const std::string& key,
 struct TransposePlan {
   int axis = -1;          // A, or -1: another form
   int ta = 64, tl = 64;   // tile extents along A and L (powers of two, 16..256)
+  bool pair = false;      // 2-byte data moved as 4-byte pairs (see transpose_plan)
   std::vector<char> lds;  // operand k is staged through LDS
 };
 
@@ -669,6 +670,18 @@ static TransposePlan transpose_plan(const Tape& t, const Operands& ops, const nc
     }
   }
   if (lds_bytes == 0) tp.axis = -1;  // no tile fits: another form
+  // 2-byte output and staged operands with 4-byte aligned bases and strides (the unit axis
+  // aside): each lane loads two A-neighbours and stores two L-neighbours as one 4-byte word
+  auto al4 = [](const void* p) { return reinterpret_cast<uintptr_t>(p) % 4 == 0; };
+  tp.pair = tp.axis >= 0 && tp.ta >= 32 && tp.tl >= 32 && itemsize(out.dtype) == 2 && al4(ops.baseo);
+  for (int d = 0; d < ops.sndim && tp.pair; ++d)
+    if (d != L && ops.sout[d] % 4 != 0) tp.pair = false;
+  for (int k : ops.unique) {
+    if (!tp.pair || ops.bcast[k] || !tp.lds[k]) continue;
+    if (itemsize(t.arr_dtypes[k]) != 2 || !al4(ops.base[k])) tp.pair = false;
+    for (int d = 0; d < ops.sndim && tp.pair; ++d)
+      if (d != tp.axis && ops.sstrides[k][d] % 4 != 0) tp.pair = false;
+  }
   return tp;
 }
 
@@ -695,7 +708,9 @@ static std::string gen_transpose(const Tape& t, const Operands& ops, const Trans
   os << "  const nca_i64 nA = shp.v[A], nL = shp.v[L];\n"
      << "  const nca_i64 tA = (nA + TA - 1) / TA, tL = (nL + TL - 1) / TL;\n"
      << "  const nca_i64 ntiles = (n / (nA * nL)) * tA * tL;\n"
-     << "  const int sx = threadIdx.x % TA, sy = threadIdx.x / TA, ex = threadIdx.x % TL, ey = threadIdx.x / TL;\n"
+     << (tp.pair ? "  const int sx = (threadIdx.x % (TA / 2)) * 2, sy = threadIdx.x / (TA / 2), ex = (threadIdx.x % (TL / 2)) * 2, "
+                   "ey = threadIdx.x / (TL / 2);\n"
+                 : "  const int sx = threadIdx.x % TA, sy = threadIdx.x / TA, ex = threadIdx.x % TL, ey = threadIdx.x / TL;\n")
      << "  for (nca_i64 tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {\n"
      << "    nca_i64 rem = tile;\n"
      << "    const nca_i64 l0 = (rem % tL) * TL;\n    rem /= tL;\n"
@@ -709,28 +724,59 @@ static std::string gen_transpose(const Tape& t, const Operands& ops, const Trans
     if (!ops.bcast[k]) os << "      o" << k << " += c * st" << k << ".v[d];\n";
   os << "      oo += c * sto.v[d];\n    }\n";
   // phase 1: stage the operands that are unit-stride along A (tx runs along A)
-  os << "    for (int j = sy; j < TL; j += 256 / TA) {\n      const nca_i64 a = a0 + sx, l = l0 + j;\n"
-     << "      if (a < nA && l < nL) {\n";
-  for (int k : ops.unique)
-    if (!ops.bcast[k] && tp.lds[k])
-      os << "        sk" << k << "[j][sx] = *reinterpret_cast<const " << ctype_of(t.arr_dtypes[k]) << "*>(p" << k
-         << " + o" << k << " + a * st" << k << ".v[A] + l * st" << k << ".v[L]);\n";
-  os << "      }\n    }\n    __syncthreads();\n";
-  // phase 2: compute and store (ex runs along L)
-  os << "    for (int j = ey; j < TA; j += 256 / TL) {\n      const nca_i64 a = a0 + j, l = l0 + ex;\n"
-     << "      if (a < nA && l < nL) {\n";
-  for (int k : ops.unique) {
-    if (ops.bcast[k]) continue;
-    const std::string Tk = ctype_of(t.arr_dtypes[k]);
-    if (tp.lds[k])
-      os << "        const " << C << " u" << k << " = nca_cvt<" << C << ", " << Tk << ">::from(sk" << k << "[ex][j]);\n";
-    else
-      os << "        const " << C << " u" << k << " = nca_ld<" << C << ">(reinterpret_cast<const " << Tk << "*>(p" << k
-         << " + o" << k << " + a * st" << k << ".v[A] + l * st" << k << ".v[L]));\n";
+  if (tp.pair) {
+    os << "    for (int j = sy; j < TL; j += 512 / TA) {\n      const nca_i64 a = a0 + sx, l = l0 + j;\n"
+       << "      if (l < nL && a < nA) {\n";
+    for (int k : ops.unique)
+      if (!ops.bcast[k] && tp.lds[k]) {
+        const std::string Tk = ctype_of(t.arr_dtypes[k]);
+        const std::string ad = "(p" + std::to_string(k) + " + o" + std::to_string(k) + " + a * st" + std::to_string(k) +
+                               ".v[A] + l * st" + std::to_string(k) + ".v[L])";
+        os << "        if (a + 1 < nA) {\n          const nca_vec<" << Tk << ", 2> w = *reinterpret_cast<const nca_vec<" << Tk
+           << ", 2>*>" << ad << ";\n          sk" << k << "[j][sx] = w.v[0];\n          sk" << k << "[j][sx + 1] = w.v[1];\n"
+           << "        } else {\n          sk" << k << "[j][sx] = *reinterpret_cast<const " << Tk << "*>" << ad << ";\n        }\n";
+      }
+    os << "      }\n    }\n    __syncthreads();\n";
+  } else {
+    os << "    for (int j = sy; j < TL; j += 256 / TA) {\n      const nca_i64 a = a0 + sx, l = l0 + j;\n"
+       << "      if (a < nA && l < nL) {\n";
+    for (int k : ops.unique)
+      if (!ops.bcast[k] && tp.lds[k])
+        os << "        sk" << k << "[j][sx] = *reinterpret_cast<const " << ctype_of(t.arr_dtypes[k]) << "*>(p" << k
+           << " + o" << k << " + a * st" << k << ".v[A] + l * st" << k << ".v[L]);\n";
+    os << "      }\n    }\n    __syncthreads();\n";
   }
-  os << body;
-  os << "        *reinterpret_cast<" << D << "*>(outp + oo + a * sto.v[A] + l * sto.v[L]) = nca_store_cvt<" << D
-     << ", " << C << ">::to(" << roots[0] << ");\n";
+  // phase 2: compute and store (ex runs along L).  value(dl) evaluates the tape at L offset dl.
+  auto value = [&](const std::string& dl, const std::string& dst) {
+    os << "        {\n";
+    for (int k : ops.unique) {
+      if (ops.bcast[k]) continue;
+      const std::string Tk = ctype_of(t.arr_dtypes[k]);
+      if (tp.lds[k])
+        os << "        const " << C << " u" << k << " = nca_cvt<" << C << ", " << Tk << ">::from(sk" << k << "[ex" << dl
+           << "][j]);\n";
+      else
+        os << "        const " << C << " u" << k << " = nca_ld<" << C << ">(reinterpret_cast<const " << Tk << "*>(p" << k
+           << " + o" << k << " + a * st" << k << ".v[A] + (l" << dl << ") * st" << k << ".v[L]));\n";
+    }
+    os << body;
+    os << "        " << dst << " = nca_store_cvt<" << D << ", " << C << ">::to(" << roots[0] << ");\n        }\n";
+  };
+  if (tp.pair) {
+    os << "    for (int j = ey; j < TA; j += 512 / TL) {\n      const nca_i64 a = a0 + j, l = l0 + ex;\n"
+       << "      if (a < nA && l < nL) {\n        " << D << " r0, r1;\n";
+    value("", "r0");
+    os << "        char* q = outp + oo + a * sto.v[A] + l * sto.v[L];\n        if (l + 1 < nL) {\n";
+    value(" + 1", "r1");
+    os << "          nca_vec<" << D << ", 2> w;\n          w.v[0] = r0;\n          w.v[1] = r1;\n"
+       << "          *reinterpret_cast<nca_vec<" << D << ", 2>*>(q) = w;\n"
+       << "        } else {\n          *reinterpret_cast<" << D << "*>(q) = r0;\n        }\n";
+  } else {
+    os << "    for (int j = ey; j < TA; j += 256 / TL) {\n      const nca_i64 a = a0 + j, l = l0 + ex;\n"
+       << "      if (a < nA && l < nL) {\n        " << D << " r0;\n";
+    value("", "r0");
+    os << "        *reinterpret_cast<" << D << "*>(outp + oo + a * sto.v[A] + l * sto.v[L]) = r0;\n";
+  }
   os << "      }\n    }\n    __syncthreads();\n  }\n}\n";
   return os.str();
 }
@@ -819,7 +865,7 @@ void expr_launch_impl(uint64_t handle, const nca_view* ins, int n_in, const nca_
   const TransposePlan tp = transpose_plan(t, ops, out, fuse);
   if (tp.axis >= 0 && ops.n > 0) {
     std::string key = "t" + std::to_string(ops.sndim) + "a" + std::to_string(tp.axis) + "x" + std::to_string(tp.ta) + "x" +
-                      std::to_string(tp.tl) + "|" + sig_operands(t, ops) + "|";
+                      std::to_string(tp.tl) + (tp.pair ? "p" : "") + "|" + sig_operands(t, ops) + "|";
     for (int k : ops.unique) key += tp.lds[k] ? '1' : '0';
     const std::string kname = "nca_expr_transpose";
     hipFunction_t fn = program_kernel(p, key, [&]() { return gen_transpose(t, ops, tp, out.dtype, kname); }, kname);
@@ -1103,6 +1149,9 @@ int nca_hip_expr_source(uint64_t handle, int32_t variant, int32_t ndim, char* bu
           const int64_t b = itemsize(t.arr_dtypes[k]) * tp.tl * (tp.ta + 1);
           if (lds_bytes + b <= (int64_t(64) << 10)) { tp.lds[k] = 1; lds_bytes += b; }
         }
+        tp.pair = itemsize(t.dest_dtype) == 2;
+        for (size_t k = 0; k < t.arr_dtypes.size(); ++k)
+          if (tp.lds[k] && itemsize(t.arr_dtypes[k]) != 2) tp.pair = false;
         src = gen_transpose(t, ops, tp, t.dest_dtype, "nca_expr_transpose");
         break;
       }

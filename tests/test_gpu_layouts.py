@@ -161,3 +161,22 @@ def test_rows_linear_index(dev):
     capi.Tape(ins, [], [capi.INT64, capi.INT64], capi.INT64, capi.INT64).launch([], ti.view(o), scalars=(2, 1))
     assert torch.equal(o.cpu(), (torch.arange(3 * 5 * 200) * 2 + 1).view(3, 5, 200))
     assert (big[:, 1::2, :] == 0).all()
+
+
+@pytest.mark.parametrize("dtype", [torch.bfloat16, torch.float16])
+def test_two_byte_transpose_pairs(dev, dtype):
+    """2-byte data with 4-byte aligned strides moves as 4-byte pairs in the tiled form; odd extents
+    along the staged axis (97) and the stored axis (67) take the single-element tails, and a base
+    one element off 4-byte alignment takes the unpaired form."""
+    base = _rand((66, 130, 98), dtype, dev, seed=21)
+    for x in (base, base[..., :97], base[:67]):
+        for perm in ((2, 1, 0), (0, 2, 1)):
+            xp = x.permute(*perm)
+            assert torch.equal(_copy(xp), xp.contiguous()), (tuple(x.shape), perm)
+    flat = _rand((1 + 66 * 130 * 98,), dtype, dev, seed=22)
+    off = flat[1:].view(66, 130, 98)
+    assert off.data_ptr() % 4 == 2
+    assert torch.equal(_copy(off.transpose(0, 2)), off.transpose(0, 2).contiguous())
+    # 4-byte source cast to a 2-byte output: the staged operand is not 2-byte, unpaired form
+    x32 = _rand((66, 130, 98), torch.float32, dev, seed=23).transpose(0, 2)
+    assert torch.equal(_copy(x32, dtype), _copy(x32.contiguous(), dtype))
