@@ -273,9 +273,11 @@ int nca_hip_expr_source(uint64_t handle, int32_t variant, int32_t ndim, char* bu
 
 /* Full reduction to one element written to the device pointer out_dev
  * (GPUEngine::execute_full_reduce, gpuengine.hh:301-333).  argmax/argmin return the
- * int64 C-order linear index of the first occurrence (host/reductions.hh:148-169);
- * max/min/argmax/argmin skip NaN as the host fold does (strict comparisons against
- * the running value, starting from lowest()/max()). */
+ * int64 C-order linear index of the first occurrence (host/reductions.hh:148-169: strict
+ * comparisons against the running value, zeros equal, NaN never taken).  max/min return
+ * the IEEE 754-2019 maximumNumber/minimumNumber of the values and lowest()/max() of the
+ * dtype: NaN never selected and -0 below +0, so every order of combination gives the same
+ * bits (the host fold differs only in the sign of a zero extremum when both zeros occur). */
 int nca_hip_reduce_full(const nca_view* in, int32_t op, int32_t accum_mode, void* out_dev,
                         void* stream);
 /* Reduction along `axes` into out (GPUEngine::execute_reduce_axes,

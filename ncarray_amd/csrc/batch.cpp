@@ -103,8 +103,8 @@ std::string gen_batch_source(const Tape& t, int fuse, int mode, int vec, bool wi
     os << "    " << SA << " acc_s = 0;\n";
     if (comp) os << "    " << SA << " acc_c = 0;\n";
   }
-  if (fuse & NCA_FUSE_MAX) os << "    " << RC << " acc_mx = nca_lim<" << RC << ">::lowest();\n";
-  if (fuse & NCA_FUSE_MIN) os << "    " << RC << " acc_mn = nca_lim<" << RC << ">::max();\n";
+  if (fuse & NCA_FUSE_MAX) os << "    " << RC << " acc_mx = nca_ident<" << D << ", " << RC << ">::lowest();\n";
+  if (fuse & NCA_FUSE_MIN) os << "    " << RC << " acc_mn = nca_ident<" << D << ", " << RC << ">::max();\n";
   std::ostringstream upd;
   upd << "        { const " << RC << " rv = nca_cvt<" << RC << ", " << D << ">::from(st);\n";
   if (fuse & NCA_FUSE_SUM) upd << (comp ? "          nca_neumaier_add(acc_s, acc_c, (" + SA + ")rv);\n" : "          acc_s += (" + SA + ")rv;\n");
@@ -191,10 +191,10 @@ std::string gen_batch_source(const Tape& t, int fuse, int mode, int vec, bool wi
     else os << "  if (lane == 0) sums[i] = (" << ctype_of(sum_dt) << ")s;\n";
   }
   if (fuse & NCA_FUSE_MAX)
-    os << "  " << RC << " m = nca_lim<" << RC << ">::lowest();\n  for (int t = t0 + lane; t < t1; t += 64) m = nca_maxop(m, *(const "
+    os << "  " << RC << " m = nca_ident<" << D << ", " << RC << ">::lowest();\n  for (int t = t0 + lane; t < t1; t += 64) m = nca_maxop(m, *(const "
        << RC << "*)(part + 24 * (size_t)t + 8));\n  m = nca_wave_max(m);\n  if (lane == 0) nca_st(maxs + i, m);\n";
   if (fuse & NCA_FUSE_MIN)
-    os << "  " << RC << " m2 = nca_lim<" << RC << ">::max();\n  for (int t = t0 + lane; t < t1; t += 64) m2 = nca_minop(m2, *(const "
+    os << "  " << RC << " m2 = nca_ident<" << D << ", " << RC << ">::max();\n  for (int t = t0 + lane; t < t1; t += 64) m2 = nca_minop(m2, *(const "
        << RC << "*)(part + 24 * (size_t)t + 16));\n  m2 = nca_wave_min(m2);\n  if (lane == 0) nca_st(mins + i, m2);\n";
   os << "}\n";
   return os.str();

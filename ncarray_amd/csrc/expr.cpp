@@ -425,8 +425,8 @@ static ReduceCode reduce_code(int32_t dest, const ReduceSpec& rs) {
     d << "  " << SA << " acc_s = 0;\n";
     if (fsum && rs.mode == NCA_ACCUM_COMPENSATED) d << "  " << SA << " acc_c = 0;\n";
   }
-  if (rs.fuse & NCA_FUSE_MAX) d << "  " << RC << " acc_mx = nca_lim<" << RC << ">::lowest();\n";
-  if (rs.fuse & NCA_FUSE_MIN) d << "  " << RC << " acc_mn = nca_lim<" << RC << ">::max();\n";
+  if (rs.fuse & NCA_FUSE_MAX) d << "  " << RC << " acc_mx = nca_ident<" << ctype_of(dest) << ", " << RC << ">::lowest();\n";
+  if (rs.fuse & NCA_FUSE_MIN) d << "  " << RC << " acc_mn = nca_ident<" << ctype_of(dest) << ", " << RC << ">::max();\n";
   u << "      { const " << RC << " rv = nca_cvt<" << RC << ", " << ctype_of(dest) << ">::from(st);\n";
   if (rs.fuse & NCA_FUSE_SUM) {
     if (fsum && rs.mode == NCA_ACCUM_COMPENSATED) u << "        nca_neumaier_add(acc_s, acc_c, (" << SA << ")rv);\n";

@@ -89,6 +89,22 @@ template <> struct nca_lim<long long> { __device__ static long long lowest() { r
 template <> struct nca_lim<unsigned long long> { __device__ static unsigned long long lowest() { return 0ull; } __device__ static unsigned long long max() { return 18446744073709551615ull; } };
 template <> struct nca_lim<float> { __device__ static float lowest() { return -3.402823466e+38f; } __device__ static float max() { return 3.402823466e+38f; } };
 template <> struct nca_lim<double> { __device__ static double lowest() { return -1.7976931348623157e+308; } __device__ static double max() { return 1.7976931348623157e+308; } };
+// Identities of max/min over values stored as T and compared in compute type W: T's own
+// lowest() / max() (the fill values of the reduction traits, reductions.hh:144-283); for the
+// 2-byte float storage types the largest finite values of those formats (bfloat16 0xff7f /
+// 0x7f7f, float16 -+65504), which a float32 identity would round to -+inf when stored.
+template <typename T, typename W> struct nca_ident {
+  __device__ static W lowest() { return static_cast<W>(nca_lim<T>::lowest()); }
+  __device__ static W max() { return static_cast<W>(nca_lim<T>::max()); }
+};
+template <typename W> struct nca_ident<nca_bf16, W> {
+  __device__ static W lowest() { return static_cast<W>(-3.38953138925153547590470800371487867e+38f); }
+  __device__ static W max() { return static_cast<W>(3.38953138925153547590470800371487867e+38f); }
+};
+template <typename W> struct nca_ident<nca_f16, W> {
+  __device__ static W lowest() { return static_cast<W>(-65504.0f); }
+  __device__ static W max() { return static_cast<W>(65504.0f); }
+};
 
 // ---- compensated summation ----------------------------------------------------------
 // Neumaier's variant of Kahan summation: s is the running sum, c the running
@@ -163,20 +179,21 @@ __device__ __forceinline__ void nca_wave_comp(F& s, F& c) {
     nca_comp_merge(s, c, s2, c2);
   }
 }
-// max/min with the host fold's rule: a candidate replaces the running value only when
-// it is strictly greater (less); NaN never wins (host/reductions.hh:130-146).
+// max/min of values.  Integers: the larger (smaller) one.  Floating types: IEEE 754-2019
+// maximumNumber / minimumNumber: a NaN operand never wins over a number (the host fold's
+// strict comparison never selects NaN, host/reductions.hh:130-146), and -0 orders below +0,
+// so the result is the same for every order of combination (a fold, a butterfly or a tree
+// of partials all give the same bits).  One v_max_f32 / v_min_f32 on gfx950.
 template <typename T> __device__ __forceinline__ T nca_maxop(T a, T b) { return (b > a) ? b : a; }
 template <typename T> __device__ __forceinline__ T nca_add2(T a, T b) { return a + b; }
 template <typename T> __device__ __forceinline__ T nca_minop(T a, T b) { return (b < a) ? b : a; }
-// max/min that never return a NaN operand when the other is a number (IEEE maxNum/minNum,
-// one v_max/v_min instruction for floating types); for pairwise trees over values that may hold
-// NaN, where nca_maxop(NaN, x) would return the NaN and drop x.  Integers: nca_maxop/nca_minop.
+template <> __device__ __forceinline__ float nca_maxop<float>(float a, float b) { return __builtin_fmaximum_numf(a, b); }
+template <> __device__ __forceinline__ float nca_minop<float>(float a, float b) { return __builtin_fminimum_numf(a, b); }
+template <> __device__ __forceinline__ double nca_maxop<double>(double a, double b) { return __builtin_fmaximum_num(a, b); }
+template <> __device__ __forceinline__ double nca_minop<double>(double a, double b) { return __builtin_fminimum_num(a, b); }
+// The same operation under the name the pairwise trees of the march stencils use.
 template <typename T> __device__ __forceinline__ T nca_maxnum(T a, T b) { return nca_maxop(a, b); }
 template <typename T> __device__ __forceinline__ T nca_minnum(T a, T b) { return nca_minop(a, b); }
-template <> __device__ __forceinline__ float nca_maxnum<float>(float a, float b) { return __builtin_fmaxf(a, b); }
-template <> __device__ __forceinline__ float nca_minnum<float>(float a, float b) { return __builtin_fminf(a, b); }
-template <> __device__ __forceinline__ double nca_maxnum<double>(double a, double b) { return __builtin_fmax(a, b); }
-template <> __device__ __forceinline__ double nca_minnum<double>(double a, double b) { return __builtin_fmin(a, b); }
 template <typename T>
 __device__ __forceinline__ T nca_wave_max(T v) {
 #pragma unroll

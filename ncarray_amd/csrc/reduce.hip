@@ -12,10 +12,11 @@
 //     reduced axis (split into S chunks with a fixed-order combine when the outputs alone
 //     do not fill the GPU); "row" kernels give each output row to one wavefront; any
 //     other layout uses a generic kernel (one thread per output, unravelled strides).
-// Semantics follow the host fold (host/reductions.hh:35-212): sums in sum_type, max/min
-// keep the running value unless a candidate is strictly greater/less (NaN never
-// selected, start at lowest()/max()), argmax/argmin return the first occurrence as a
-// C-order linear index into the input, 0 when nothing was selected.
+// Semantics: sums in sum_type (host/reductions.hh:35-212); max/min are IEEE 754-2019
+// maximumNumber/minimumNumber from lowest()/max() (NaN never selected as in the host fold,
+// -0 below +0, so the bits do not depend on the combination order); argmax/argmin follow the
+// host fold exactly: the first occurrence (strict comparison, zeros equal, NaN never taken)
+// as a C-order linear index into the input, 0 when nothing was selected.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -96,7 +97,7 @@ template <int DT, bool COMP> struct Acc<DT, NCA_REDUCE_MEAN, COMP> : AccSum<DT, 
 template <int DT, bool IsMax> struct AccExt {
   typedef typename RT<DT>::V V;
   V m;
-  __device__ void init() { m = IsMax ? nca_lim<V>::lowest() : nca_lim<V>::max(); }
+  __device__ void init() { m = IsMax ? nca_ident<typename RT<DT>::T, V>::lowest() : nca_ident<typename RT<DT>::T, V>::max(); }
   __device__ void fold(V v, nca_i64) { m = IsMax ? nca_maxop(m, v) : nca_minop(m, v); }
   __device__ void wave() { m = IsMax ? nca_wave_max(m) : nca_wave_min(m); }
   __device__ void merge(const AccExt& o) { m = IsMax ? nca_maxop(m, o.m) : nca_minop(m, o.m); }
@@ -109,7 +110,7 @@ template <int DT, bool IsMax> struct AccArg {
   typedef typename RT<DT>::V V;
   V m;
   nca_i64 i;
-  __device__ void init() { m = IsMax ? nca_lim<V>::lowest() : nca_lim<V>::max(); i = -1; }
+  __device__ void init() { m = IsMax ? nca_ident<typename RT<DT>::T, V>::lowest() : nca_ident<typename RT<DT>::T, V>::max(); i = -1; }
   // the host fold: replace only when strictly better than the running value
   __device__ void fold(V v, nca_i64 idx) {
     if (IsMax ? (v > m) : (v < m)) { m = v; i = idx; }
@@ -192,7 +193,7 @@ __device__ void combine_slots_and_store(const Slot* slots, int n, nca_i64 count,
     }
   } else if constexpr (OP == NCA_REDUCE_MAX || OP == NCA_REDUCE_MIN) {
     constexpr bool mx = OP == NCA_REDUCE_MAX;
-    V m = mx ? nca_lim<V>::lowest() : nca_lim<V>::max();
+    V m = mx ? nca_ident<typename RT<DT>::T, V>::lowest() : nca_ident<typename RT<DT>::T, V>::max();
     for (int k = tid; k < n; k += blockDim.x) {
       V v;
       __builtin_memcpy(&v, slots[k].a.raw, sizeof(V));
@@ -211,7 +212,7 @@ __device__ void combine_slots_and_store(const Slot* slots, int n, nca_i64 count,
     }
   } else if constexpr (OP == NCA_REDUCE_ARGMAX || OP == NCA_REDUCE_ARGMIN) {
     constexpr bool mx = OP == NCA_REDUCE_ARGMAX;
-    V m = mx ? nca_lim<V>::lowest() : nca_lim<V>::max();
+    V m = mx ? nca_ident<typename RT<DT>::T, V>::lowest() : nca_ident<typename RT<DT>::T, V>::max();
     nca_i64 i = -1;
     for (int k = tid; k < n; k += blockDim.x) {
       V v;
@@ -409,7 +410,7 @@ __global__ __launch_bounds__(256) void k_combine_cols(const Slot* __restrict__ s
     }
   } else if constexpr (OP == NCA_REDUCE_MAX || OP == NCA_REDUCE_MIN) {
     constexpr bool mx = OP == NCA_REDUCE_MAX;
-    V m = mx ? nca_lim<V>::lowest() : nca_lim<V>::max();
+    V m = mx ? nca_ident<typename RT<DT>::T, V>::lowest() : nca_ident<typename RT<DT>::T, V>::max();
     for (int s = 0; s < S; ++s) {
       V v;
       __builtin_memcpy(&v, slots[(nca_i64)s * K + o].a.raw, sizeof(V));
@@ -418,7 +419,7 @@ __global__ __launch_bounds__(256) void k_combine_cols(const Slot* __restrict__ s
     nca_st(reinterpret_cast<typename RT<DT>::T*>(out) + o, m);
   } else if constexpr (OP == NCA_REDUCE_ARGMAX || OP == NCA_REDUCE_ARGMIN) {
     constexpr bool mx = OP == NCA_REDUCE_ARGMAX;
-    V m = mx ? nca_lim<V>::lowest() : nca_lim<V>::max();
+    V m = mx ? nca_ident<typename RT<DT>::T, V>::lowest() : nca_ident<typename RT<DT>::T, V>::max();
     nca_i64 i = -1;
     for (int s = 0; s < S; ++s) {
       V v;
@@ -562,7 +563,7 @@ __device__ void finalize_scalar_body(const void* __restrict__ partials, int n, v
     }
   } else {
     constexpr bool mx = KIND == NCA_FUSE_MAX;
-    V m = mx ? nca_lim<V>::lowest() : nca_lim<V>::max();
+    V m = mx ? nca_ident<typename RT<DT>::T, V>::lowest() : nca_ident<typename RT<DT>::T, V>::max();
     for (int k = tid; k < n; k += blockDim.x) {
       V v;
       __builtin_memcpy(&v, reinterpret_cast<const char*>(partials) + 8 * (size_t)k, sizeof(V));

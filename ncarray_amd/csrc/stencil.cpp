@@ -233,8 +233,8 @@ static std::string gen_march(const StencilProgram& p, const StencilFuse& fz, int
     os << "  " << RC << " acc_s = 0;\n";
     if (comp) os << "  " << RC << " acc_c = 0;\n";
   }
-  if (fz.fuse & NCA_FUSE_MAX) os << "  " << RC << " acc_mx = nca_lim<" << RC << ">::lowest();\n";
-  if (fz.fuse & NCA_FUSE_MIN) os << "  " << RC << " acc_mn = nca_lim<" << RC << ">::max();\n";
+  if (fz.fuse & NCA_FUSE_MAX) os << "  " << RC << " acc_mx = nca_ident<" << ctype_of(dst_dtype) << ", " << RC << ">::lowest();\n";
+  if (fz.fuse & NCA_FUSE_MIN) os << "  " << RC << " acc_mn = nca_ident<" << ctype_of(dst_dtype) << ", " << RC << ">::max();\n";
   // prologue: planes z0-RZ .. z0+RZ-1 straight to LDS, planes z0+RZ and z0+RZ+1 in flight
   os << "  const int zb = (int)z0, ze = (int)z1;\n";
   os << "#pragma unroll\n  for (int k = 0; k < 2 * RZ; ++k) { gload(zb - RZ + k, riA, rhA, rcA); lstore(k, riA, rhA, rcA); }\n";
@@ -581,8 +581,8 @@ static std::string gen_march_q(const StencilProgram& p, const StencilFuse& fz, i
     os << "  " << RC << " acc_s = 0;\n";
     if (comp) os << "  " << RC << " acc_c = 0;\n";
   }
-  if (fz.fuse & NCA_FUSE_MAX) os << "  " << RC << " acc_mx = nca_lim<" << RC << ">::lowest();\n";
-  if (fz.fuse & NCA_FUSE_MIN) os << "  " << RC << " acc_mn = nca_lim<" << RC << ">::max();\n";
+  if (fz.fuse & NCA_FUSE_MAX) os << "  " << RC << " acc_mx = nca_ident<" << VD << ", " << RC << ">::lowest();\n";
+  if (fz.fuse & NCA_FUSE_MIN) os << "  " << RC << " acc_mn = nca_ident<" << VD << ", " << RC << ">::max();\n";
   const std::string zero = "(" + C + ")0";
   // plane pz (an int expression) into register slot q; the plane starts at element offset off
   // one row vector of VX source values into dst[0..VX) (compute type), zero when !ok.  2-byte
@@ -926,8 +926,8 @@ static std::string gen_march_train(const StencilProgram& p, const Tape& adj, con
     os << "  " << RC << " acc_s = 0;\n";
     if (comp) os << "  " << RC << " acc_c = 0;\n";
   }
-  if (fz.fuse & NCA_FUSE_MAX) os << "  " << RC << " acc_mx = nca_lim<" << RC << ">::lowest();\n";
-  if (fz.fuse & NCA_FUSE_MIN) os << "  " << RC << " acc_mn = nca_lim<" << RC << ">::max();\n";
+  if (fz.fuse & NCA_FUSE_MAX) os << "  " << RC << " acc_mx = nca_ident<" << VD << ", " << RC << ">::lowest();\n";
+  if (fz.fuse & NCA_FUSE_MIN) os << "  " << RC << " acc_mn = nca_ident<" << VD << ", " << RC << ">::max();\n";
   // prologue: x planes zb-2, zb-1 in LDS, x(zb) and the extras' plane zb-1 in registers
   os << "  load_x(zb - 2);\n  store_x(nca_mod3(zb - 2));\n  load_x(zb - 1);\n  store_x(nca_mod3(zb - 1));\n"
      << "  load_x(zb);\n  load_e(zb - 1);\n";
@@ -1127,8 +1127,8 @@ static std::string gen_march_train_vec(const StencilProgram& p, const Tape& adj,
     os << "  " << RC << " acc_s = 0;\n";
     if (comp) os << "  " << RC << " acc_c = 0;\n";
   }
-  if (fz.fuse & NCA_FUSE_MAX) os << "  " << RC << " acc_mx = nca_lim<" << RC << ">::lowest();\n";
-  if (fz.fuse & NCA_FUSE_MIN) os << "  " << RC << " acc_mn = nca_lim<" << RC << ">::max();\n";
+  if (fz.fuse & NCA_FUSE_MAX) os << "  " << RC << " acc_mx = nca_ident<" << VD << ", " << RC << ">::lowest();\n";
+  if (fz.fuse & NCA_FUSE_MIN) os << "  " << RC << " acc_mn = nca_ident<" << VD << ", " << RC << ">::max();\n";
   // neighbour reads around an aligned 4-vector at LDS index `base` of plane arrays `arr[slot]`
   // with row pitch `pitch`: per (dz, dy) the aligned vector V and the scalars L / R beside it
   struct Nb { int dz, dy, left = 0, right = 0; };
@@ -1292,8 +1292,8 @@ static std::string gen_direct(const StencilProgram& p, int32_t dst_dtype, bool z
   os << ") {\n";
   if (fuse) {
     if (fuse & NCA_FUSE_SUM) { os << "  " << RC << " acc_s = 0;\n"; if (comp) os << "  " << RC << " acc_c = 0;\n"; }
-    if (fuse & NCA_FUSE_MAX) os << "  " << RC << " acc_mx = nca_lim<" << RC << ">::lowest();\n";
-    if (fuse & NCA_FUSE_MIN) os << "  " << RC << " acc_mn = nca_lim<" << RC << ">::max();\n";
+    if (fuse & NCA_FUSE_MAX) os << "  " << RC << " acc_mx = nca_ident<" << ctype_of(dst_dtype) << ", " << RC << ">::lowest();\n";
+    if (fuse & NCA_FUSE_MIN) os << "  " << RC << " acc_mn = nca_ident<" << ctype_of(dst_dtype) << ", " << RC << ">::max();\n";
   }
   // with hooks: one output per thread (a `return` in the prologue ends the thread, as in
   // the reference); without: grid-stride loop

@@ -211,26 +211,49 @@ int64_t nca_cpu_sum_i64(const void* x, int dt, int64_t n) {
   return (int64_t)acc;
 }
 
+/* Value max/min (the rule this backend documents for every max/min it computes): NaN is
+ * never selected (the host fold's strict comparison never takes it, host/reductions.hh:
+ * 130-146) and -0 orders below +0 (IEEE 754-2019 maximumNumber/minimumNumber), so the
+ * result does not depend on the order of combination.  The host fold returns the same
+ * value except for the sign of a zero extremum when both zeros occur (it keeps the first).
+ * The reference's device traits (reductions.hh:160-166, 262-268) keep the later operand on
+ * ties and combine through atomics in no fixed order. */
+static double max_num(double a, double b) {
+  if (isnan(b)) return a;
+  if (isnan(a)) return b;
+  if (a == b) return signbit(a) ? b : a;
+  return a > b ? a : b;
+}
+static double min_num(double a, double b) {
+  if (isnan(b)) return a;
+  if (isnan(a)) return b;
+  if (a == b) return signbit(a) ? a : b;
+  return a < b ? a : b;
+}
+
 /* max_recursive / argmax_recursive (host/reductions.hh:130-169): the running value starts
- * at lowest() and is replaced only when an element is strictly greater; the index is
- * the C-order position of the first such maximum, 0 if none was ever taken. */
+ * at lowest(); the index is the C-order position of the first element strictly greater
+ * than the running value (zeros compare equal, NaN never taken), 0 if none was ever taken.
+ * The value follows max_num from lowest(). */
 double nca_cpu_max(const void* x, int dt, int64_t n, double lowest, int64_t* argidx) {
-  double acc = lowest;
+  double acc = lowest, best = lowest;
   int64_t idx = 0;
   for (int64_t i = 0; i < n; ++i) {
     double v = load_d(x, dt, i);
-    if (v > acc) { acc = v; idx = i; }
+    acc = max_num(acc, v);
+    if (v > best) { best = v; idx = i; }
   }
   if (argidx) *argidx = idx;
   return acc;
 }
 /* min_recursive / argmin_recursive (host/reductions.hh:171-212) */
 double nca_cpu_min(const void* x, int dt, int64_t n, double highest, int64_t* argidx) {
-  double acc = highest;
+  double acc = highest, best = highest;
   int64_t idx = 0;
   for (int64_t i = 0; i < n; ++i) {
     double v = load_d(x, dt, i);
-    if (v < acc) { acc = v; idx = i; }
+    acc = min_num(acc, v);
+    if (v < best) { best = v; idx = i; }
   }
   if (argidx) *argidx = idx;
   return acc;
@@ -240,8 +263,9 @@ double nca_cpu_min(const void* x, int dt, int64_t n, double highest, int64_t* ar
  * For every input index i in C order: j = output index from build_reduction_params
  * (custom_types.hh:132-176: reduced axes have output stride 0), then
  * res[j] = reduce(res[j], x[i]) with res filled by the traits' fill() value.
- * op: 0 sum (float32 accumulate), 1 max, 2 min (reductions.hh:144-283), 3 argmax,
- * 4 argmin (first occurrence; returns the C-order linear input index). */
+ * op: 0 sum (float32 accumulate), 1 max, 2 min (max_num / min_num from lowest() / max(), the
+ * fill values of reductions.hh:144-283), 3 argmax, 4 argmin (the host fold's strict
+ * comparison: first occurrence, NaN never taken; returns the C-order linear input index). */
 void nca_cpu_reduce_axes_f32(const float* x, int ndim, const int64_t* shape, const int* reduced, int op,
                              void* out) {
   int64_t ostride[16];
@@ -271,8 +295,8 @@ void nca_cpu_reduce_axes_f32(const float* x, int ndim, const int64_t* shape, con
     }
     float v = x[i];
     if (op == 0) of[j] = of[j] + v;
-    else if (op == 1) of[j] = (of[j] > v) ? of[j] : v;   /* MaxTraits::reduce, reductions.hh:160-166 */
-    else if (op == 2) of[j] = (of[j] < v) ? of[j] : v;   /* MinTraits::reduce, reductions.hh:262-268 */
+    else if (op == 1) of[j] = (float)max_num(of[j], v);
+    else if (op == 2) of[j] = (float)min_num(of[j], v);
     else if (op == 3) { if (v > best[j]) { best[j] = v; oi[j] = i; } }
     else { if (v < best[j]) { best[j] = v; oi[j] = i; } }
   }
@@ -349,16 +373,16 @@ void nca_cpu_bin_backward_f32(const float* gsums, int64_t F, int nb, const int32
 /* ---- CPU baseline: the 3-D Laplacian pipeline ------------------------------------------
  * lap = 7-point Laplacian with zero Dirichlet (x[z-1]+x[z+1]+x[y-1]+x[y+1]+x[x-1]+x[x+1]
  * - 6*x, summed in that order in float32), then colsum = sum over axis 0 (float32 fold
- * in z order, as HostEngine::execute_reduce_axes does) and mx = max(lap) (strict-greater
- * fold).  threads > 1 splits the z planes (OpenMP, as the reference VM path,
+ * in z order, as HostEngine::execute_reduce_axes does) and mx = max(lap) (max_num).
+ * threads > 1 splits the z planes (OpenMP, as the reference VM path,
  * engines/hostengine.hh:97-118); the fold order per output is unchanged.  Returns mx. */
 float nca_cpu_laplacian_pipeline(const float* x, int64_t n0, int64_t n1, int64_t n2, float* lap, float* colsum,
                                  int threads) {
   const int64_t plane = n1 * n2;
-  float mx = -3.402823466e+38f;
+  float* pmx = (float*)malloc(sizeof(float) * (size_t)(n0 > 0 ? n0 : 1));
 #ifdef _OPENMP
   if (threads < 1) threads = 1;
-#pragma omp parallel for num_threads(threads) schedule(static) reduction(max : mx)
+#pragma omp parallel for num_threads(threads) schedule(static)
 #endif
   for (int64_t z = 0; z < n0; ++z) {
     float lmx = -3.402823466e+38f;
@@ -379,11 +403,14 @@ float nca_cpu_laplacian_pipeline(const float* x, int64_t n0, int64_t n1, int64_t
         v = v + xp;
         v = v - 6.0f * row[xx];
         out[xx] = v;
-        if (v > lmx) lmx = v;
+        lmx = (float)max_num(lmx, v);
       }
     }
-    if (lmx > mx) mx = lmx;
+    pmx[z] = lmx;
   }
+  float mx = -3.402823466e+38f;
+  for (int64_t z = 0; z < n0; ++z) mx = (float)max_num(mx, pmx[z]);
+  free(pmx);
   for (int64_t k = 0; k < plane; ++k) colsum[k] = 0.0f;
   for (int64_t z = 0; z < n0; ++z) {
     const float* l = lap + z * plane;

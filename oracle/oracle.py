@@ -89,21 +89,29 @@ def sum_fold(x):
     return int(lib.nca_cpu_sum_i64(x.ctypes.data, CODE[x.dtype], x.size))
 
 
-def max_fold(x):
-    """host::max_recursive + argmax_recursive (host/reductions.hh:130-169)."""
+# largest finite bfloat16 (0x7f7f) and float16: the max/min identities of those storage types
+BF16_MAX = 3.3895313892515355e38
+F16_MAX = 65504.0
+
+
+def max_fold(x, lowest=None):
+    """host::max_recursive + argmax_recursive (host/reductions.hh:130-169): the value by the
+    max rule of nca_cpu_max (NaN never selected, -0 below +0), the index of the first element
+    strictly above the running value.  `lowest` overrides the dtype's identity (for values of
+    a storage type held in a wider array, e.g. bfloat16 as float64)."""
     x = np.ascontiguousarray(x).ravel()
     c = CODE[x.dtype]
     idx = ctypes.c_int64()
-    v = lib.nca_cpu_max(x.ctypes.data, c, x.size, _lim(c, True), ctypes.byref(idx))
+    v = lib.nca_cpu_max(x.ctypes.data, c, x.size, _lim(c, True) if lowest is None else lowest, ctypes.byref(idx))
     return v, idx.value
 
 
-def min_fold(x):
+def min_fold(x, highest=None):
     """host::min_recursive + argmin_recursive (host/reductions.hh:171-212)."""
     x = np.ascontiguousarray(x).ravel()
     c = CODE[x.dtype]
     idx = ctypes.c_int64()
-    v = lib.nca_cpu_min(x.ctypes.data, c, x.size, _lim(c, False), ctypes.byref(idx))
+    v = lib.nca_cpu_min(x.ctypes.data, c, x.size, _lim(c, False) if highest is None else highest, ctypes.byref(idx))
     return v, idx.value
 
 

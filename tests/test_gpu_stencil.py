@@ -260,7 +260,7 @@ def test_march_general_offsets(dev, offs, src_code, shape):
 @pytest.mark.parametrize("march", [True, False], ids=["march", "direct"])
 def test_fused_max_min_skip_nan(dev, dt, march):
     """Fused max/min of a stencil output that holds NaN: NaN is never selected and every other
-    value is still considered (the host fold's rule), also when the NaN and the extreme value
+    value is still considered (maximumNumber/minimumNumber), also when the NaN and the extreme value
     sit in the same vector of one thread.  The NaN input at (4, 4, 0) makes the Laplacian NaN at
     (3, 4, 0) but not at (3, 4, 1); input -1000 at (3, 4, 1) puts the maximum +6000 + ... there
     and the minimum, -1000 + ..., at its neighbours."""

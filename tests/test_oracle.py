@@ -1,5 +1,8 @@
 """The CPU oracle pinned against the reference's own known answers, numpy, closed forms
 and brute force (no GPU)."""
+import itertools
+import math
+
 import numpy as np
 import pytest
 
@@ -59,6 +62,71 @@ def test_argmax_first_occurrence():
     assert O.max_fold(x) == (2.0, 1)
     v, i = O.max_fold(np.full(3, np.nan, dtype=np.float32))
     assert v == np.finfo(np.float32).min and i == 0
+
+
+def _key(v):
+    """Total order of the max/min rule: numbers by value, -0 below +0 (NaN excluded)."""
+    return (v, math.copysign(1.0, v))
+
+
+def _brute_extremes(vals, lowest, highest):
+    """Closed forms of the max/min rule and the host fold's argmax/argmin, by brute force:
+    max/min = the largest/smallest number under _key (lowest()/max() when there is none);
+    argmax/argmin = the first position whose value compares strictly beyond every earlier
+    number (zeros equal, NaN never taken; host/reductions.hh:148-212)."""
+    nums = [v for v in vals if not math.isnan(v)]
+    mx = max(nums + [lowest], key=_key)
+    mn = min(nums + [highest], key=_key)
+    amx, amn, bmx, bmn = 0, 0, lowest, highest
+    for i, v in enumerate(vals):
+        if v > bmx:
+            bmx, amx = v, i
+        if v < bmn:
+            bmn, amn = v, i
+    return mx, mn, amx, amn
+
+
+def test_max_min_rule_nan_and_signed_zero():
+    """Value max/min: NaN never selected, -0 below +0, hence independent of the order (every
+    permutation gives the same bits); argmax/argmin: the host fold's first occurrence.  The old
+    'keep the running value unless strictly greater' value rule fails the permutation check on
+    [-0, +0] / [+0, -0]."""
+    lo, hi = float(np.finfo(np.float32).min), float(np.finfo(np.float32).max)
+    pool = [np.nan, -0.0, 0.0, -1.0, 1.0, 2.0]
+    cases = [[-0.0, 0.0], [0.0, -0.0], [np.nan, -0.0, 0.0], [-0.0, np.nan], [np.nan, np.nan], [],
+             [2.0, np.nan, 2.0], [-1.0, -0.0, -1.0]]
+    rng = np.random.default_rng(11)
+    cases += [list(rng.choice(pool, rng.integers(1, 7))) for _ in range(200)]
+    for vals in cases:
+        x = np.array(vals, dtype=np.float32)
+        mx, mn, amx, amn = _brute_extremes([float(v) for v in x], lo, hi)
+        v, i = O.max_fold(x)
+        assert _key(v) == _key(mx) and i == amx, (vals, v, i)
+        v, i = O.min_fold(x)
+        assert _key(v) == _key(mn) and i == amn, (vals, v, i)
+        for perm in itertools.islice(itertools.permutations(range(x.size)), 24):
+            y = x[list(perm)]
+            assert _key(O.max_fold(y)[0]) == _key(mx) and _key(O.min_fold(y)[0]) == _key(mn)
+    assert math.copysign(1.0, O.max_fold(np.array([-0.0, 0.0], dtype=np.float32))[0]) == 1.0
+    assert math.copysign(1.0, O.min_fold(np.array([0.0, -0.0], dtype=np.float32))[0]) == -1.0
+    assert O.max_fold(np.array([-0.0, 0.0], dtype=np.float32))[1] == 0   # host first occurrence
+
+
+def test_axis_max_min_rule_brute_force():
+    """Axis max/min/argmax/argmin of the oracle against the brute-force closed forms per output
+    line, on data dense in NaN and signed zeros (first, middle and last positions all occur)."""
+    lo, hi = float(np.finfo(np.float32).min), float(np.finfo(np.float32).max)
+    rng = np.random.default_rng(12)
+    pool = np.array([np.nan, -0.0, 0.0, -1.0, 1.0], dtype=np.float32)
+    x = rng.choice(pool, (4, 5, 6)).astype(np.float32)
+    flat = np.arange(x.size).reshape(x.shape)
+    for ax in range(3):
+        got = [O.reduce_axes_f32(x, [ax], op) for op in (1, 2, 3, 4)]
+        xm, fm = np.moveaxis(x, ax, -1), np.moveaxis(flat, ax, -1)
+        for idx in np.ndindex(xm.shape[:-1]):
+            mx, mn, amx, amn = _brute_extremes([float(v) for v in xm[idx]], lo, hi)
+            assert _key(float(got[0][idx])) == _key(mx) and _key(float(got[1][idx])) == _key(mn)
+            assert got[2][idx] == fm[idx][amx] and got[3][idx] == fm[idx][amn]
 
 
 def test_axis_reductions_vs_numpy():
