@@ -310,6 +310,24 @@ int nca_hip_stencil_apply_reduce(uint64_t handle, const nca_view* src,
                                  const nca_view* axis0_sum, int32_t fuse,
                                  int32_t accum_mode, void* sum_out, void* max_out,
                                  void* min_out, void* stream);
+/* nca_hip_stencil_apply_reduce that allocates nothing.  No counterpart in the reference: it
+ * is for callers that cannot allocate device memory (code under hipGraph capture, tight
+ * serving loops, harnesses that forbid allocation).  The block partials and z-run column
+ * partials live in a fixed scratch array of 32 MiB per device (4 MiB of block partials, 28 MiB
+ * of column partials), held from nca_hip_init until the process ends; the caller still passes
+ * the inputs and outputs.  A launch that needs more block partials than that fails with
+ * NCA_ERR_INVALID; the fused axis-0 sum uses fewer z runs when its column partials would not
+ * fit.  Kernels are not compiled or loaded here (loading a code object allocates): the first
+ * call with a given stencil, layout and fusion must be an ordinary nca_hip_stencil_apply_reduce,
+ * else NCA_ERR_INVALID.  Every call on one GPU shares that GPU's array: calls that overlap on
+ * one GPU (other streams, other threads) race and must be serialised by the caller. */
+int nca_hip_stencil_apply_reduce_noalloc(uint64_t handle, const nca_view* src,
+                                         const nca_view* extras, int32_t n_extra,
+                                         const nca_scalar* scalars, int32_t n_scalars,
+                                         const nca_view* dst, int32_t boundary,
+                                         const nca_view* axis0_sum, int32_t fuse,
+                                         int32_t accum_mode, void* sum_out, void* max_out,
+                                         void* min_out, void* stream);
 /* The adjoint (transpose) of a stencil that is linear in its source loads:
  * grad_src[p] = sum_k c_k * grad_dst[p - offsets[k]], with c_k = d f / d v_k derived
  * symbolically (they may depend on the scalars, not on the loads).  Unit step only;

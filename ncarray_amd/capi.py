@@ -125,6 +125,8 @@ _SIGS = {
     "nca_hip_stencil_create": [ctypes.POINTER(StencilDesc), ctypes.POINTER(_U64)],
     "nca_hip_stencil_apply": [_U64, _VP, _VP, _I32, _SP, _I32, _VP, _I32, ctypes.POINTER(_P), _I32, _P],
     "nca_hip_stencil_apply_reduce": [_U64, _VP, _VP, _I32, _SP, _I32, _VP, _I32, _VP, _I32, _I32, _P, _P, _P, _P],
+    "nca_hip_stencil_apply_reduce_noalloc": [_U64, _VP, _VP, _I32, _SP, _I32, _VP, _I32, _VP, _I32, _I32, _P, _P, _P,
+                                             _P],
     "nca_hip_stencil_adjoint_create": [_U64, ctypes.POINTER(_U64)],
     "nca_hip_stencil_expr_vjp": [_U64, _U64, ctypes.POINTER(_I32), _I32, _VP, _VP, _I32, _SP, _I32, _SP, _I32, _P, _VP,
                                  _VP, _I32, _P],

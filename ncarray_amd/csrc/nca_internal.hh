@@ -83,7 +83,15 @@ struct DeviceState {
   // the device's default stream-ordered memory pool (nca_hip_malloc / nca_hip_free); null when
   // the runtime has none, then they fall back to hipMalloc / hipFree
   hipMemPool_t pool = nullptr;
+  // this device's copy of the library's fixed scratch array (the entry points that allocate
+  // nothing), looked up once when the device is initialised
+  void* fixed_scratch = nullptr;
 };
+
+// The fixed scratch array of the allocation-free entry points: kFixedScratchBytes per device,
+// the first kFixedPartialBytes for per-block partials, the rest for column partials.
+constexpr size_t kFixedScratchBytes = size_t(32) << 20;
+constexpr size_t kFixedPartialBytes = size_t(4) << 20;
 
 DeviceState& current_device();           // throws NCA_ERR_NOT_INIT when not initialised
 bool device_available();                  // a device was initialised on this thread

@@ -7,6 +7,11 @@
 
 #include "nca_internal.hh"
 
+// The fixed scratch array of the allocation-free entry points (nca_hip_*_noalloc).  Every GPU
+// the library runs on holds its own copy until the process ends; host code reaches it only
+// through the address hipGetSymbolAddress returns for that device (DeviceState::fixed_scratch).
+static __device__ __align__(256) char scratch[nca::kFixedScratchBytes];
+
 namespace nca {
 
 namespace {
@@ -66,6 +71,11 @@ static void init_device(int device) {
       uint64_t keep = UINT64_MAX;
       if (hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep) == hipSuccess) st->pool = pool;
     }
+    // this device's copy of the fixed scratch array (the device is current here)
+    void* fixed = nullptr;
+    HIP_CHECK(hipGetSymbolAddress(&fixed, HIP_SYMBOL(scratch)));
+    NCA_REQUIRE(fixed != nullptr, "no device address for the fixed scratch array");
+    st->fixed_scratch = fixed;
     (void)hipGetLastError();
     g_devices[device] = std::move(st);
   }

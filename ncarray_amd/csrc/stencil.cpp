@@ -1361,13 +1361,19 @@ static std::string gen_direct(const StencilProgram& p, int32_t dst_dtype, bool z
   return os.str();
 }
 
+// The kernel of this program for `key`, generated and loaded on first use.  loaded_only: return
+// only a kernel already loaded (loading a code object allocates device memory, which the
+// allocation-free entry points must not do) and fail otherwise.
 static hipFunction_t stencil_kernel(StencilProgram& p, const std::string& key, const std::function<std::string()>& gen,
-                                    const std::string& kname) {
+                                    const std::string& kname, bool loaded_only = false) {
   {
     std::lock_guard<std::mutex> lk(p.mu);
     auto it = p.kernels.find(key);
     if (it != p.kernels.end()) return it->second;
   }
+  NCA_REQUIRE(!loaded_only, "kernel " + kname + " of this stencil is not loaded yet: the allocation-free entry point "
+                                "only launches loaded kernels (make one nca_hip_stencil_apply_reduce call with the same "
+                                "arguments first)");
   std::string src = gen();
   hipFunction_t fn = jit_get(src, kname);
   std::lock_guard<std::mutex> lk(p.mu);
@@ -1400,10 +1406,12 @@ static int blocks_per_cu(hipFunction_t fn) {
 }
 
 // ---- apply ------------------------------------------------------------------------------------------------
+// fixed: NULL, or this device's fixed scratch array (kFixedScratchBytes) used instead of the
+// per-stream workspace; the call then allocates nothing (see nca_hip_stencil_apply_reduce_noalloc).
 static void stencil_apply_impl(uint64_t handle, const nca_view* src, const nca_view* extras, int n_extra,
                                const nca_scalar* scalars, int n_scalars, const nca_view* dst, int boundary,
                                void* const* ext_args, int n_ext_args, const nca_view* axis0, int fuse, int mode,
-                               void* sum_out, void* max_out, void* min_out, hipStream_t s) {
+                               void* sum_out, void* max_out, void* min_out, hipStream_t s, void* fixed = nullptr) {
   StencilProgram& p = stencil_program(handle);
   const Tape& t = p.tape;
   NCA_REQUIRE(src && dst, "null src/dst");
@@ -1494,19 +1502,33 @@ static void stencil_apply_impl(uint64_t handle, const nca_view* src, const nca_v
                       std::to_string(mp.vx) + "," + std::to_string(mp.rpt) + "," + std::to_string(mp.txt) + "," + std::to_string(mp.pd);
     const std::string kname = march_name(p, fz, dst->dtype);
     hipFunction_t fn = stencil_kernel(
-        p, key, [&]() { return mp.queue ? gen_march_q(p, fz, dst->dtype, mp) : gen_march(p, fz, dst->dtype); }, kname);
+        p, key, [&]() { return mp.queue ? gen_march_q(p, fz, dst->dtype, mp) : gen_march(p, fz, dst->dtype); }, kname,
+        fixed != nullptr);
     // One resident wave of blocks: (tiles x z runs) = CUs x blocks per CU for this kernel's
     // registers and LDS, so no block waits for another to finish (no tail), and runs are
     // as long as possible (fewer halo planes).
     const int per_cu = blocks_per_cu(fn);
     const int64_t resident = static_cast<int64_t>(dev.n_cus) * per_cu;
     int64_t nzc = std::max<int64_t>(1, std::min<int64_t>(resident / std::max<int64_t>(tiles, 1), NZ));
+    if (fixed && axis0) {
+      // the column partials of the z runs must fit the fixed array: fewer, longer runs
+      const int64_t fit = static_cast<int64_t>((kFixedScratchBytes - kFixedPartialBytes) / (sizeof(double) * NY * NX));
+      nzc = std::max<int64_t>(1, std::min<int64_t>(nzc, fit));
+    }
     int zlen = static_cast<int>((NZ + nzc - 1) / nzc);
     nzc = (NZ + zlen - 1) / zlen;
     const int64_t blocks = tiles * nzc;
     NCA_REQUIRE(blocks < (int64_t(1) << 31), "grid too large");
     size_t colbytes = (axis0 && nzc > 1) ? static_cast<size_t>(nzc * NY * NX) * sizeof(double) : 0;
-    Workspace& w = workspace(s, static_cast<size_t>(blocks) * 24, colbytes);
+    Workspace fw;
+    if (fixed) {
+      NCA_REQUIRE(static_cast<size_t>(blocks) * 24 <= kFixedPartialBytes,
+                  "this launch needs " + std::to_string(blocks * 24) + " bytes of block partials, more than the " +
+                      std::to_string(kFixedPartialBytes) + " the allocation-free entry point has");
+      fw.partials = fixed;
+      fw.big = static_cast<char*>(fixed) + kFixedPartialBytes;
+    }
+    Workspace& w = fixed ? fw : workspace(s, static_cast<size_t>(blocks) * 24, colbytes);
     std::deque<const void*> ptrs;
     std::vector<void*> params;
     ptrs.push_back(src->data);
@@ -1560,7 +1582,8 @@ static void stencil_apply_impl(uint64_t handle, const nca_view* src, const nca_v
   // direct
   const bool zero = boundary == NCA_BOUNDARY_ZERO;
   std::string key = "d|" + std::to_string(zero) + "|" + std::to_string(fuse) + "|" + std::to_string(mode);
-  hipFunction_t fn = stencil_kernel(p, key, [&]() { return gen_direct(p, dst->dtype, zero, fuse, mode); }, "nca_stencil_direct");
+  hipFunction_t fn = stencil_kernel(p, key, [&]() { return gen_direct(p, dst->dtype, zero, fuse, mode); }, "nca_stencil_direct",
+                                    fixed != nullptr);
   int blocks;
   if (p.has_ext) {
     NCA_REQUIRE((n + 255) / 256 < (int64_t(1) << 31), "grid too large");
@@ -1569,7 +1592,12 @@ static void stencil_apply_impl(uint64_t handle, const nca_view* src, const nca_v
   } else {
     blocks = grid_blocks(n, 256, fuse ? dev.n_cus * 4 : dev.n_cus * 16);
   }
-  Workspace& w = workspace(s, static_cast<size_t>(blocks) * 24);
+  Workspace fw;
+  if (fixed) {
+    NCA_REQUIRE(static_cast<size_t>(blocks) * 24 <= kFixedPartialBytes, "too many blocks for the fixed scratch array");
+    fw.partials = fixed;
+  }
+  Workspace& w = fixed ? fw : workspace(s, static_cast<size_t>(blocks) * 24);
   std::deque<const void*> ptrs;
   std::deque<std::array<int64_t, NCA_MAX_NDIM>> dims;
   std::vector<void*> params;
@@ -1686,6 +1714,24 @@ int nca_hip_stencil_apply_reduce(uint64_t handle, const nca_view* src, const nca
     stencil_apply_impl(handle, src, extras, n_extra, scalars, n_scalars, dst, boundary, nullptr, 0, axis0_sum, fuse,
                        accum_mode, sum_out, max_out, min_out, as_stream(stream));
     check_launch("fused stencil");
+    finish_call(stream);
+  });
+}
+
+int nca_hip_stencil_apply_reduce_noalloc(uint64_t handle, const nca_view* src, const nca_view* extras, int32_t n_extra,
+                                         const nca_scalar* scalars, int32_t n_scalars, const nca_view* dst,
+                                         int32_t boundary, const nca_view* axis0_sum, int32_t fuse, int32_t accum_mode,
+                                         void* sum_out, void* max_out, void* min_out, void* stream) {
+  return api_call([&] {
+    NCA_REQUIRE((fuse & ~(NCA_FUSE_SUM | NCA_FUSE_MAX | NCA_FUSE_MIN)) == 0, "bad fuse flags");
+    NCA_REQUIRE(!(fuse & NCA_FUSE_SUM) || sum_out, "sum requested without sum_out");
+    NCA_REQUIRE(!(fuse & NCA_FUSE_MAX) || max_out, "max requested without max_out");
+    NCA_REQUIRE(!(fuse & NCA_FUSE_MIN) || min_out, "min requested without min_out");
+    DeviceState& dev = current_device();
+    NCA_REQUIRE(dev.fixed_scratch != nullptr, "the fixed scratch array of this device was not found at init");
+    stencil_apply_impl(handle, src, extras, n_extra, scalars, n_scalars, dst, boundary, nullptr, 0, axis0_sum, fuse,
+                       accum_mode, sum_out, max_out, min_out, as_stream(stream), dev.fixed_scratch);
+    check_launch("fused stencil (allocation-free)");
     finish_call(stream);
   });
 }
