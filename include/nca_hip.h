@@ -115,8 +115,8 @@ enum { NCA_COUNT_ALL = 0, NCA_COUNT_NONZERO = 1 };
  *     p += i_d * strides[d];  if (suboffsets[d] >= 0) p = *(char**)p + suboffsets[d];
  * so any number of axes may be indirect.  suboffsets is read only for this layout; an SO
  * view with no suboffset >= 0 is a plain strided view and is accepted everywhere.
- * Only the expression entry points (expr_launch, expr_launch_reduce) accept views with an
- * NC pointer axis or an SO suboffset >= 0. */
+ * The expression entry points (expr_launch, expr_launch_reduce, expr_vjp) and the stencil
+ * apply entry points accept views with an NC pointer axis or an SO suboffset >= 0. */
 typedef struct nca_view {
   void* data;
   int32_t dtype;
@@ -158,7 +158,12 @@ typedef struct nca_expr_desc {
  * In expr, LOAD_NCARR k < n_offsets reads src at offsets[k]; LOAD_NCARR
  * n_offsets + j reads extra pointwise array j at the output coordinate (dest shape).
  * extra_params / prologue / epilogue are StencilJITExtensions strings
- * (jit/device/extensions.hh:68-72); any of them may be NULL. */
+ * (jit/device/extensions.hh:68-72); any of them may be NULL.
+ * is_pointer_axis (ndim flags) and is_soarr are Stencil<NDIM>::create's (stencil.hh:76-81):
+ * when is_pointer_axis is given, apply accepts only sources whose indirect axes are exactly
+ * the flagged ones (an NC pointer axis, or with is_soarr the SO axes with suboffset >= 0);
+ * NULL accepts any layout.  Sources and destinations may be NC pointer-axis or SO views
+ * either way (the row pointers are read in the kernel). */
 typedef struct nca_stencil_desc {
   int32_t ndim;
   int32_t n_offsets;
@@ -170,6 +175,8 @@ typedef struct nca_stencil_desc {
   const char* extra_params;
   const char* prologue;
   const char* epilogue;
+  const uint8_t* is_pointer_axis; /* ndim flags, or NULL */
+  int32_t is_soarr;
 } nca_stencil_desc;
 
 /* ================================================================================ */
@@ -395,7 +402,9 @@ int nca_hip_stencil_expr_train_source(uint64_t stencil_handle, uint64_t expr_han
                                       int32_t vec, char* buf, size_t n);
 /* The generated source of a stencil kernel (for inspection and nca_hip_rtc_compile):
  * variant 0 the march kernel, 1 the march with the fused axis-0 sum and max, 4 the same
- * with compensated sums, 2 / 3 the direct kernel without / with the zero boundary.
+ * with compensated sums, 2 / 3 the direct kernel without / with the zero boundary, 5 the
+ * tile kernel at unit steps, 6 the tile kernel with fused sum/max/min and the first axis
+ * read through a table of row pointers.
  * Returns the text length (negative status on error); writes at most n bytes. */
 int nca_hip_stencil_source(uint64_t handle, int32_t variant, char* buf, size_t n);
 

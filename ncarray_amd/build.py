@@ -70,8 +70,10 @@ def build_host(verbose=False):
     test_src = os.path.join(ROOT, "tests", "cpp", "test_host_api.cpp")
     if os.path.exists(test_src):
         exe = os.path.join(ROOT, "tests", "cpp", "test_host_api")
-        _run([CXX] + HOST_FLAGS + [test_src, "-o", exe + ".tmp", f"-L{HERE}", "-lnca_host", "-lnca_hip",
-              f"-Wl,-rpath,{HERE}"], "C++ host API test")
+        # the test also creates HIP streams itself (HIP's host API, as hipcc compiles host code)
+        _run([CXX] + HOST_FLAGS + ["-I/opt/rocm/include", "-D__HIP_PLATFORM_AMD__", test_src, "-o", exe + ".tmp",
+              f"-L{HERE}", "-lnca_host", "-lnca_hip", "-L/opt/rocm/lib", "-lamdhip64",
+              f"-Wl,-rpath,{HERE}", "-Wl,-rpath,/opt/rocm/lib"], "C++ host API test")
         os.replace(exe + ".tmp", exe)
     if verbose:
         print("built", HOST_LIB, mod)

@@ -71,7 +71,8 @@ class StencilDesc(ctypes.Structure):
                 ("offsets", ctypes.POINTER(ctypes.c_int64)), ("src_dtype", ctypes.c_int32),
                 ("n_extra", ctypes.c_int32), ("extra_dtypes", ctypes.POINTER(ctypes.c_int32)),
                 ("expr", ctypes.POINTER(ExprDesc)), ("extra_params", ctypes.c_char_p),
-                ("prologue", ctypes.c_char_p), ("epilogue", ctypes.c_char_p)]
+                ("prologue", ctypes.c_char_p), ("epilogue", ctypes.c_char_p),
+                ("is_pointer_axis", ctypes.POINTER(ctypes.c_uint8)), ("is_soarr", ctypes.c_int32)]
 
 
 class NcaError(RuntimeError):
@@ -310,7 +311,8 @@ class Stencil:
     """Stencil<NDIM>: offsets + a tape over the offset loads (and extra pointwise arrays)."""
 
     def __init__(self, offsets, tape_instrs, src_dtype, work_dtype, dest_dtype, scalar_dtypes=(),
-                 extra_dtypes=(), extra_params=None, prologue=None, epilogue=None, handle=None):
+                 extra_dtypes=(), extra_params=None, prologue=None, epilogue=None, handle=None,
+                 is_pointer_axis=None, is_soarr=False):
         self.offsets = [list(o) for o in offsets]
         self.ndim = len(self.offsets[0])
         self.src_dtype = src_dtype
@@ -345,6 +347,11 @@ class Stencil:
         sd.extra_params = extra_params.encode() if extra_params else None
         sd.prologue = prologue.encode() if prologue else None
         sd.epilogue = epilogue.encode() if epilogue else None
+        if is_pointer_axis is not None:
+            pa = (ctypes.c_uint8 * self.ndim)(*[1 if f else 0 for f in is_pointer_axis])
+            self._keep.append(pa)
+            sd.is_pointer_axis = ctypes.cast(pa, ctypes.POINTER(ctypes.c_uint8))
+        sd.is_soarr = 1 if is_soarr else 0
         h = _U64()
         call("nca_hip_stencil_create", ctypes.byref(sd), ctypes.byref(h))
         self.handle = h.value

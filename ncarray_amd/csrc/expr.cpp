@@ -84,18 +84,7 @@ ScalarSlots make_scalar_slots(const Tape& t, const nca_scalar* s, int n) {
 }
 
 // ---- operand analysis ----------------------------------------------------------------------
-// The dereference chain of a view: start address, byte strides, indirect axes and the byte
-// offset added after each pointer read.  An NC pointer axis P (NCOffsetsPolicy::advance,
-// layout.hh:468-480) is one step: the start moves by ptr_offset table entries, strides[P]
-// (table entries) becomes bytes and row_offset is the offset after the read.  An SO view
-// (SOArrayPolicy::advance, layout.hh:677-700) has a step at each axis with suboffset >= 0.
-struct Chain {
-  uintptr_t base = 0;
-  std::array<int64_t, NCA_MAX_NDIM> strides{};
-  std::vector<int> axes;
-  std::vector<int64_t> subs;
-};
-static Chain chain_of(const nca_view& v) {
+Chain chain_of(const nca_view& v) {
   Chain c;
   c.base = reinterpret_cast<uintptr_t>(v.data);
   for (int d = 0; d < v.ndim; ++d) c.strides[d] = v.strides[d];

@@ -36,6 +36,10 @@ struct StencilProgram {
   Tape tape;
   std::string extra_params, prologue, epilogue;
   bool has_ext = false;
+  // Stencil<NDIM>::create's is_pointer_axis / is_soarr, when given: the source layout it serves
+  bool layout_fixed = false;
+  std::array<uint8_t, NCA_MAX_NDIM> pointer_axes{};
+  bool is_soarr = false;
   uint64_t forward = 0;  // for adjoints: the forward stencil
   std::mutex mu;
   std::unordered_map<std::string, hipFunction_t> kernels;
@@ -1266,7 +1270,10 @@ static std::string gen_march_train_vec(const StencilProgram& p, const Tape& adj,
 }
 
 // ---- direct kernel ----------------------------------------------------------------------------------------
-static std::string gen_direct(const StencilProgram& p, int32_t dst_dtype, bool zero_boundary, int fuse, int mode) {
+// sind / dind: the indirect axes of the source's and the destination's dereference chains (a
+// pointer read after the step of that axis; the kernel's strides are the chains' byte strides).
+static std::string gen_direct(const StencilProgram& p, int32_t dst_dtype, bool zero_boundary, int fuse, int mode,
+                              const std::vector<int>& sind = {}, const std::vector<int>& dind = {}) {
   const Tape& t = p.tape;
   const int nd = p.ndim;
   const std::string C = ctype_of(t.compute_dtype);
@@ -1284,8 +1291,11 @@ static std::string gen_direct(const StencilProgram& p, int32_t dst_dtype, bool z
   std::ostringstream os;
   os << "#define NDIM_VAL " << nd << "\nstruct nca_dims { nca_i64 v[NDIM_VAL]; };\n";
   os << "extern \"C\" __global__ __launch_bounds__(256) void nca_stencil_direct(const void* src_data, nca_dims sshape, nca_dims sstr";
+  if (!sind.empty()) os << ", nca_dims ssub";
   for (size_t j = 0; j < p.extra_dtypes.size(); ++j) os << ", const char* __restrict__ e" << j << ", nca_dims est" << j;
-  os << ", void* dest_data, nca_dims dshape, nca_dims dstr, nca_dims step, nca_i64 n";
+  os << ", void* dest_data, nca_dims dshape, nca_dims dstr";
+  if (!dind.empty()) os << ", nca_dims dsub";
+  os << ", nca_dims step, nca_i64 n";
   if (fuse) os << ", void* __restrict__ psum, void* __restrict__ pmax, void* __restrict__ pmin";
   os << p.extra_params;
   for (size_t i = 0; i < t.scalar_dtypes.size(); ++i) os << ", " << C << " s" << i;
@@ -1307,15 +1317,18 @@ static std::string gen_direct(const StencilProgram& p, int32_t dst_dtype, bool z
   os << p.prologue << "\n";
   os << "    nca_i64 c_out[NDIM_VAL], c_src[NDIM_VAL];\n"
      << "    { nca_i64 rem = b_idx;\n#pragma unroll\n      for (int d = NDIM_VAL - 1; d >= 0; --d) { c_out[d] = rem % dshape.v[d]; rem /= dshape.v[d]; c_src[d] = c_out[d] * step.v[d]; } }\n";
+  auto is_in = [](const std::vector<int>& v, int a) { return std::find(v.begin(), v.end(), a) != v.end(); };
   for (int k = 0; k < p.n_off; ++k) {
-    os << "    " << C << " u" << k << ";\n    {\n      nca_i64 off = 0; bool inb = true;\n";
+    os << "    " << C << " u" << k << ";\n    {\n      const char* pk = reinterpret_cast<const char*>(src_data); bool inb = true;\n";
     for (int d = 0; d < nd; ++d) {
       os << "      { const nca_i64 c = c_src[" << d << "] + (" << p.offsets[k][d] << "); ";
       if (zero_boundary) os << "inb = inb && (c >= 0) && (c < sshape.v[" << d << "]); ";
-      os << "off += c * sstr.v[" << d << "]; }\n";
+      os << "pk += c * sstr.v[" << d << "];";
+      if (is_in(sind, d)) os << " if (inb) pk = nca_chase(pk, ssub.v[" << d << "]);";
+      os << " }\n";
     }
     os << "      u" << k << " = inb ? nca_ld<" << C << ">(reinterpret_cast<const " << S
-       << "*>(reinterpret_cast<const char*>(src_data) + off)) : (" << C << ")0;\n    }\n";
+       << "*>(pk)) : (" << C << ")0;\n    }\n";
   }
   for (size_t j = 0; j < p.extra_dtypes.size(); ++j) {
     os << "    " << C << " u" << (p.n_off + j) << ";\n    { nca_i64 off = 0;\n";
@@ -1326,9 +1339,13 @@ static std::string gen_direct(const StencilProgram& p, int32_t dst_dtype, bool z
   if (t.uses_idx) os << "    const nca_i64 lin = b_idx;\n";
   os << body;
   os << "    const " << D << " st = nca_store_cvt<" << D << ", " << C << ">::to(" << roots[0] << ");\n";
-  os << "    if (dest_data) { nca_i64 off = 0;\n";
-  for (int d = 0; d < nd; ++d) os << "      off += c_out[" << d << "] * dstr.v[" << d << "];\n";
-  os << "      *reinterpret_cast<" << D << "*>(reinterpret_cast<char*>(dest_data) + off) = st; }\n";
+  os << "    if (dest_data) { char* pd = reinterpret_cast<char*>(dest_data);\n";
+  for (int d = 0; d < nd; ++d) {
+    os << "      pd += c_out[" << d << "] * dstr.v[" << d << "];";
+    if (is_in(dind, d)) os << " pd = nca_chase(pd, dsub.v[" << d << "]);";
+    os << "\n";
+  }
+  os << "      *reinterpret_cast<" << D << "*>(pd) = st; }\n";
   if (fuse) {
     os << "    { const " << RC << " rv = nca_cvt<" << RC << ", " << D << ">::from(st);\n";
     if (fuse & NCA_FUSE_SUM) os << (comp ? "      nca_neumaier_add(acc_s, acc_c, rv);\n" : "      acc_s += rv;\n");
@@ -1341,6 +1358,264 @@ static std::string gen_direct(const StencilProgram& p, int32_t dst_dtype, bool z
   if (fuse) {
     os << "  __shared__ double red_sd[4];\n  __shared__ " << RC << " red_mx[4];\n  __shared__ " << RC << " red_mn[4];\n"
        << "  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;\n";
+    if (fuse & NCA_FUSE_SUM) {
+      if (comp) os << "  nca_wave_comp(acc_s, acc_c);\n  if (lane == 0) red_sd[wid] = nca_comp_value(acc_s, acc_c);\n";
+      else os << "  acc_s = nca_wave_sum(acc_s);\n  if (lane == 0) red_sd[wid] = (double)acc_s;\n";
+    }
+    if (fuse & NCA_FUSE_MAX) os << "  acc_mx = nca_wave_max(acc_mx);\n  if (lane == 0) red_mx[wid] = acc_mx;\n";
+    if (fuse & NCA_FUSE_MIN) os << "  acc_mn = nca_wave_min(acc_mn);\n  if (lane == 0) red_mn[wid] = acc_mn;\n";
+    os << "  __syncthreads();\n  if (threadIdx.x == 0) {\n";
+    if (fuse & NCA_FUSE_SUM) os << "    ((double*)psum)[blockIdx.x] = red_sd[0] + red_sd[1] + red_sd[2] + red_sd[3];\n";
+    if (fuse & NCA_FUSE_MAX)
+      os << "    " << RC << " m = red_mx[0];\n    for (int w = 1; w < 4; ++w) m = nca_maxop(m, red_mx[w]);\n    *(" << RC
+         << "*)((char*)pmax + 8 * (size_t)blockIdx.x) = m;\n";
+    if (fuse & NCA_FUSE_MIN)
+      os << "    " << RC << " m2 = red_mn[0];\n    for (int w = 1; w < 4; ++w) m2 = nca_minop(m2, red_mn[w]);\n    *(" << RC
+         << "*)((char*)pmin + 8 * (size_t)blockIdx.x) = m2;\n";
+    os << "  }\n";
+  }
+  os << "}\n";
+  return os.str();
+}
+
+// ---- tile kernel ---------------------------------------------------------------------------------------
+// Any rank >= 2 (a 1-D stencil is launched as 2-D with a leading axis of extent 1), any integer
+// steps, strided, NC pointer-axis and SO sources and destinations whose last axis is direct.  A
+// block owns a TY x TX tile of the output's last two axes at one coordinate of the outer axes
+// and walks a run of such tiles (XCD-aware order).  The offsets are grouped by their outer-axis
+// part (one group for a 2-D stencil); each group's source region (the tile's rows and columns
+// times the steps, plus the group's halo) is staged in LDS once per tile, in two passes: the
+// address of every staged row is resolved once (the pointer chain of NC / SO views is followed
+// once per row, as the reference resolves one pointer per offset prefix, rtcompiler.hh:247-308)
+// into an LDS table, then the block loads the region as 16-byte vectors (rows whose start is not
+// 16-byte aligned, and a strided last axis, element by element).  Reads outside the source give
+// 0 (the zero boundary; with no boundary handling the host has checked that every read a
+// result uses is in range).  Outputs: one wave per output row, lanes along x, PX outputs per
+// lane; fused sum/max/min as in the direct kernel.
+struct TileSpec {
+  int nd = 2;                                       // rank as launched (>= 2)
+  std::vector<std::array<int64_t, NCA_MAX_NDIM>> offsets;  // as launched
+  int stepy = 1, stepx = 1;                         // steps of the last two axes (compiled in)
+  int px = 2, py = 4;                               // TX = 64 px, TY = 4 py
+  int vw = 1;                                       // elements per staging load (16 bytes; 1: strided x)
+  bool x32 = false, d32 = false;                    // 32-bit offsets along the last axis (source / dest)
+  std::vector<int> src_ind, dst_ind;                // indirect axes (pointer read after the step)
+  int fuse = 0, mode = NCA_ACCUM_PLAIN;
+  bool with_dst = true;
+  int32_t dst_dtype = NCA_FLOAT32;
+  // derived: groups of offsets sharing their outer-axis part
+  struct Group {
+    std::vector<int64_t> outer;
+    int64_t miny = 0, maxy = 0, minx = 0, maxx = 0;
+    int64_t x0 = 0;                 // first staged column relative to tile x * stepx (a multiple of vw)
+    int64_t ry = 0, rx = 0, off = 0, rb = 0;
+  };
+  std::vector<Group> groups;
+  std::vector<int> group_of;
+  int64_t lds_elems = 0, rows = 0;
+  int TX() const { return 64 * px; }
+  int TY() const { return 4 * py; }
+  void derive() {
+    groups.clear();
+    group_of.clear();
+    for (auto& o : offsets) {
+      std::vector<int64_t> outer(o.begin(), o.begin() + (nd - 2));
+      int g = -1;
+      for (size_t i = 0; i < groups.size(); ++i)
+        if (groups[i].outer == outer) g = static_cast<int>(i);
+      if (g < 0) {
+        Group gr;
+        gr.outer = outer;
+        gr.miny = gr.maxy = o[nd - 2];
+        gr.minx = gr.maxx = o[nd - 1];
+        groups.push_back(gr);
+        g = static_cast<int>(groups.size()) - 1;
+      }
+      Group& gr = groups[g];
+      gr.miny = std::min(gr.miny, o[nd - 2]);
+      gr.maxy = std::max(gr.maxy, o[nd - 2]);
+      gr.minx = std::min(gr.minx, o[nd - 1]);
+      gr.maxx = std::max(gr.maxx, o[nd - 1]);
+      group_of.push_back(g);
+    }
+    lds_elems = 0;
+    rows = 0;
+    for (auto& gr : groups) {
+      gr.x0 = gr.minx >= 0 ? gr.minx / vw * vw : -((-gr.minx + vw - 1) / vw) * vw;
+      gr.ry = static_cast<int64_t>(TY() - 1) * stepy + (gr.maxy - gr.miny) + 1;
+      const int64_t span = static_cast<int64_t>(TX() - 1) * stepx + gr.maxx - gr.x0 + 1;
+      gr.rx = (span + vw - 1) / vw * vw;
+      gr.off = lds_elems;
+      gr.rb = rows;
+      lds_elems += gr.ry * gr.rx;
+      rows += gr.ry;
+    }
+  }
+  std::string key() const {
+    std::ostringstream k;
+    k << "t|" << nd << "|" << stepy << "," << stepx << "|" << px << "," << py << "|" << vw << x32 << d32 << "|s";
+    for (int a : src_ind) k << a << ",";
+    k << "|d";
+    for (int a : dst_ind) k << a << ",";
+    k << "|" << fuse << "," << mode << "," << with_dst << "," << dst_dtype << "|";
+    for (auto& o : offsets)
+      for (int d = 0; d < nd; ++d) k << o[d] << (d + 1 < nd ? "," : ";");
+    return k.str();
+  }
+};
+
+static std::string gen_tile(const StencilProgram& p, const TileSpec& ts) {
+  const Tape& t = p.tape;
+  const int nd = ts.nd;
+  const std::string C = ctype_of(t.compute_dtype);
+  const std::string S = ctype_of(p.src_dtype);
+  const std::string D = ctype_of(ts.dst_dtype);
+  const int n_arr = static_cast<int>(t.arr_dtypes.size());
+  std::vector<int> alias(n_arr);
+  for (int k = 0; k < n_arr; ++k) alias[k] = k;
+  std::vector<std::string> roots;
+  std::string body = emit_nodes(t, {t.root}, alias, names_u(n_arr), names_s(static_cast<int>(t.scalar_dtypes.size())),
+                                "lin", "t", roots);
+  const int32_t rcd = compute_dtype(ts.dst_dtype);
+  const std::string RC = ctype_of(rcd);
+  const bool comp = ts.mode == NCA_ACCUM_COMPENSATED;
+  const int fuse = ts.fuse;
+  const int TX = ts.TX(), TY = ts.TY();
+  const int X = nd - 1, Y = nd - 2;
+  const std::string XI = ts.x32 ? "int" : "nca_i64", DI = ts.d32 ? "int" : "nca_i64";
+  auto is_in = [](const std::vector<int>& v, int a) { return std::find(v.begin(), v.end(), a) != v.end(); };
+  std::ostringstream os;
+  os << "#define NDIM_VAL " << nd << "\nstruct nca_dims { nca_i64 v[NDIM_VAL]; };\n";
+  os << "extern \"C\" __global__ __launch_bounds__(256) void nca_stencil_tile(const char* __restrict__ src, nca_dims sshape, "
+        "nca_dims sstr, nca_dims ssub";
+  for (size_t j = 0; j < p.extra_dtypes.size(); ++j) os << ", const char* __restrict__ e" << j << ", nca_dims est" << j;
+  os << ", char* __restrict__ dst, nca_dims dshape, nca_dims dstr, nca_dims dsub, nca_dims step, nca_i64 ntiles, "
+        "nca_i64 ntx, nca_i64 nty, int xcd_order";
+  if (fuse) os << ", void* __restrict__ psum, void* __restrict__ pmax, void* __restrict__ pmin";
+  for (size_t i = 0; i < t.scalar_dtypes.size(); ++i) os << ", " << C << " s" << i;
+  os << ") {\n";
+  os << "  __shared__ __attribute__((aligned(16))) " << C << " lds[" << std::max<int64_t>(ts.lds_elems, 1) << "];\n";
+  os << "  __shared__ const char* rowp[" << std::max<int64_t>(ts.rows, 1) << "];\n";
+  if (fuse) {
+    if (fuse & NCA_FUSE_SUM) { os << "  " << RC << " acc_s = 0;\n"; if (comp) os << "  " << RC << " acc_c = 0;\n"; }
+    if (fuse & NCA_FUSE_MAX) os << "  " << RC << " acc_mx = nca_ident<" << D << ", " << RC << ">::lowest();\n";
+    if (fuse & NCA_FUSE_MIN) os << "  " << RC << " acc_mn = nca_ident<" << D << ", " << RC << ">::max();\n";
+  }
+  os << "  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;\n";
+  os << "  const " << XI << " sx_n = (" << XI << ")sshape.v[" << X << "], sx_s = (" << XI << ")sstr.v[" << X << "];\n";
+  os << "  const " << DI << " dx_n = (" << DI << ")dshape.v[" << X << "], dx_s = (" << DI << ")dstr.v[" << X << "];\n";
+  // XCD-aware order: block b runs on XCD b % 8; each XCD walks its own contiguous range of tiles,
+  // so the halos neighbouring tiles share stay in one L2
+  os << "  nca_i64 t_first, t_end, t_step;\n"
+     << "  if (xcd_order) {\n"
+     << "    const int xcd = blockIdx.x & 7;\n"
+     << "    const nca_i64 per = (ntiles + 7) >> 3;\n"
+     << "    t_first = (nca_i64)xcd * per + (blockIdx.x >> 3);\n"
+     << "    t_end = ((nca_i64)xcd + 1) * per < ntiles ? ((nca_i64)xcd + 1) * per : ntiles;\n"
+     << "    t_step = (gridDim.x - xcd + 7) >> 3;\n"
+     << "  } else { t_first = blockIdx.x; t_end = ntiles; t_step = gridDim.x; }\n";
+  os << "  for (nca_i64 tile = t_first; tile < t_end; tile += t_step) {\n";
+  os << "    const nca_i64 txi = tile % ntx, rq = tile / ntx, tyi = rq % nty, oi = rq / nty;\n";
+  os << "    nca_i64 co[NDIM_VAL];\n";
+  os << "    { nca_i64 rem = oi;\n";
+  for (int d = nd - 3; d >= 0; --d) os << "      co[" << d << "] = rem % dshape.v[" << d << "]; rem /= dshape.v[" << d << "];\n";
+  os << "      (void)rem;\n    }\n    co[" << Y << "] = tyi * " << TY << "; co[" << X << "] = txi * " << TX << ";\n";
+  // pass 1: the address of each staged row (x = 0), or null outside the source
+  for (size_t g = 0; g < ts.groups.size(); ++g) {
+    const auto& gr = ts.groups[g];
+    os << "    for (int rr = threadIdx.x; rr < " << gr.ry << "; rr += 256) {\n"
+       << "      const char* rp = src; bool ok = true;\n";
+    for (int d = 0; d < X; ++d) {
+      if (d < Y)
+        os << "      { const nca_i64 c = co[" << d << "] * step.v[" << d << "] + (" << gr.outer[d] << ");";
+      else
+        os << "      { const nca_i64 c = co[" << d << "] * " << ts.stepy << " + (" << gr.miny << ") + rr;";
+      os << " ok = ok && c >= 0 && c < sshape.v[" << d << "]; rp += c * sstr.v[" << d << "];";
+      if (is_in(ts.src_ind, d)) os << " if (ok) rp = nca_chase(rp, ssub.v[" << d << "]);";
+      os << " }\n";
+    }
+    os << "      rowp[" << gr.rb << " + rr] = ok ? rp : nullptr;\n    }\n";
+  }
+  os << "    __syncthreads();\n";
+  // pass 2: the regions, 16-byte vectors where the row allows
+  const int vw = ts.vw;
+  for (size_t g = 0; g < ts.groups.size(); ++g) {
+    const auto& gr = ts.groups[g];
+    const int64_t nv = gr.rx / vw;
+    os << "    {\n      const " << XI << " xa = (" << XI << ")(co[" << X << "] * " << ts.stepx << ") + (" << gr.x0 << ");\n"
+       << "      for (int e = threadIdx.x; e < " << gr.ry * nv << "; e += 256) {\n"
+       << "        const int r = e / " << nv << ", v = e - r * " << nv << ";\n"
+       << "        const char* rp = rowp[" << gr.rb << " + r];\n"
+       << "        const " << XI << " xs = xa + v * " << vw << ";\n"
+       << "        " << C << "* ld = &lds[" << gr.off << " + r * " << gr.rx << " + v * " << vw << "];\n";
+    if (vw > 1) {
+      os << "        if (rp != nullptr && xs >= 0 && xs + " << vw << " <= sx_n && (reinterpret_cast<unsigned long long>(rp) & 15) == 0) {\n"
+         << "          const nca_vec<" << S << ", " << vw << "> q = *reinterpret_cast<const nca_vec<" << S << ", " << vw
+         << ">*>(rp + xs * (" << XI << ")sizeof(" << S << "));\n"
+         << "          nca_vec<" << C << ", " << vw << "> w;\n"
+         << "#pragma unroll\n          for (int k = 0; k < " << vw << "; ++k) w.v[k] = nca_cvt<" << C << ", " << S << ">::from(q.v[k]);\n"
+         << "          *reinterpret_cast<nca_vec<" << C << ", " << vw << ">*>(ld) = w;\n"
+         << "        } else {\n"
+         << "#pragma unroll\n          for (int k = 0; k < " << vw << "; ++k) {\n"
+         << "            const " << XI << " xk = xs + k;\n"
+         << "            ld[k] = (rp != nullptr && xk >= 0 && xk < sx_n) ? nca_ld<" << C << ">(reinterpret_cast<const " << S
+         << "*>(rp + xk * sx_s)) : (" << C << ")0;\n"
+         << "          }\n        }\n";
+    } else {
+      os << "        ld[0] = (rp != nullptr && xs >= 0 && xs < sx_n) ? nca_ld<" << C << ">(reinterpret_cast<const " << S
+         << "*>(rp + xs * sx_s)) : (" << C << ")0;\n";
+    }
+    os << "      }\n    }\n";
+  }
+  os << "    __syncthreads();\n";
+  // outputs
+  os << "#pragma unroll\n    for (int i = 0; i < " << ts.py << "; ++i) {\n"
+     << "      const int ly = wid + 4 * i;\n"
+     << "      const nca_i64 yo = co[" << Y << "] + ly;\n"
+     << "      if (yo < dshape.v[" << Y << "]) {\n";
+  if (ts.with_dst) {
+    os << "        char* rd = dst;\n";
+    for (int d = 0; d < X; ++d) {
+      os << "        rd += " << (d < Y ? "co[" + std::to_string(d) + "]" : std::string("yo")) << " * dstr.v[" << d << "];";
+      if (is_in(ts.dst_ind, d)) os << " rd = nca_chase(rd, dsub.v[" << d << "]);";
+      os << "\n";
+    }
+  }
+  for (size_t j = 0; j < p.extra_dtypes.size(); ++j) {
+    os << "        const char* re" << j << " = e" << j << ";\n";
+    for (int d = 0; d < X; ++d)
+      os << "        re" << j << " += " << (d < Y ? "co[" + std::to_string(d) + "]" : std::string("yo")) << " * est" << j
+         << ".v[" << d << "];\n";
+  }
+  if (t.uses_idx) os << "        const nca_i64 lrow = (oi * dshape.v[" << Y << "] + yo) * dshape.v[" << X << "];\n";
+  os << "        const " << DI << " xt = (" << DI << ")co[" << X << "];\n";
+  os << "#pragma unroll\n        for (int jx = 0; jx < " << ts.px << "; ++jx) {\n"
+     << "          const int lx = lane + 64 * jx;\n"
+     << "          const " << DI << " xo = xt + lx;\n"
+     << "          if (xo < dx_n) {\n";
+  for (int k = 0; k < p.n_off; ++k) {
+    const auto& gr = ts.groups[ts.group_of[k]];
+    const int64_t oy = ts.offsets[k][Y] - gr.miny, ox = ts.offsets[k][X] - gr.x0;
+    os << "            const " << C << " u" << k << " = lds[" << gr.off << " + (ly * " << ts.stepy << " + " << oy << ") * "
+       << gr.rx << " + lx * " << ts.stepx << " + " << ox << "];\n";
+  }
+  for (size_t j = 0; j < p.extra_dtypes.size(); ++j)
+    os << "            const " << C << " u" << (p.n_off + j) << " = nca_ld<" << C << ">(reinterpret_cast<const "
+       << ctype_of(p.extra_dtypes[j]) << "*>(re" << j << " + (nca_i64)xo * est" << j << ".v[" << X << "]));\n";
+  if (t.uses_idx) os << "            const nca_i64 lin = lrow + xo;\n";
+  os << body;
+  os << "            const " << D << " st = nca_store_cvt<" << D << ", " << C << ">::to(" << roots[0] << ");\n";
+  if (ts.with_dst) os << "            *reinterpret_cast<" << D << "*>(rd + xo * dx_s) = st;\n";
+  if (fuse) {
+    os << "            { const " << RC << " rv = nca_cvt<" << RC << ", " << D << ">::from(st);\n";
+    if (fuse & NCA_FUSE_SUM) os << (comp ? "              nca_neumaier_add(acc_s, acc_c, rv);\n" : "              acc_s += rv;\n");
+    if (fuse & NCA_FUSE_MAX) os << "              acc_mx = nca_maxop(acc_mx, rv);\n";
+    if (fuse & NCA_FUSE_MIN) os << "              acc_mn = nca_minop(acc_mn, rv);\n";
+    os << "            }\n";
+  }
+  os << "          }\n        }\n      }\n    }\n    __syncthreads();\n  }\n";
+  if (fuse) {
+    os << "  __shared__ double red_sd[4];\n  __shared__ " << RC << " red_mx[4];\n  __shared__ " << RC << " red_mn[4];\n";
     if (fuse & NCA_FUSE_SUM) {
       if (comp) os << "  nca_wave_comp(acc_s, acc_c);\n  if (lane == 0) red_sd[wid] = nca_comp_value(acc_s, acc_c);\n";
       else os << "  acc_s = nca_wave_sum(acc_s);\n  if (lane == 0) red_sd[wid] = (double)acc_s;\n";
@@ -1405,6 +1680,153 @@ static int blocks_per_cu(hipFunction_t fn) {
   return per_cu;
 }
 
+// Tile kernel launch (see gen_tile).  A 1-D problem runs as 2-D with a leading axis of extent 1.
+static void launch_tile(StencilProgram& p, const nca_view& src, const Chain& sch, const nca_view* extras, int n_extra,
+                        const nca_view& dst, const Chain& dch, bool with_dst, const int64_t* step, int64_t n, int fuse,
+                        int mode, void* sum_out, void* max_out, void* min_out, ScalarSlots& sc, hipStream_t s,
+                        void* fixed) {
+  DeviceState& dev = current_device();
+  const int lift = p.ndim == 1 ? 1 : 0;  // a leading axis of extent 1
+  const int nd = p.ndim + lift;
+  TileSpec ts;
+  ts.nd = nd;
+  for (auto& o : p.offsets) {
+    std::array<int64_t, NCA_MAX_NDIM> q{};
+    for (int d = 0; d < p.ndim; ++d) q[d + lift] = o[d];
+    ts.offsets.push_back(q);
+  }
+  std::array<int64_t, NCA_MAX_NDIM> sshape{}, sstr{}, ssub{}, dshape{}, dstr{}, dsub{}, stp{};
+  for (int d = 0; d < nd; ++d) { sshape[d] = dshape[d] = 1; stp[d] = 1; }
+  for (int d = 0; d < p.ndim; ++d) {
+    sshape[d + lift] = src.shape[d];
+    sstr[d + lift] = sch.strides[d];
+    dshape[d + lift] = dst.shape[d];
+    dstr[d + lift] = with_dst ? dch.strides[d] : 0;
+    stp[d + lift] = step[d];
+  }
+  for (size_t i = 0; i < sch.axes.size(); ++i) {
+    ts.src_ind.push_back(sch.axes[i] + lift);
+    ssub[sch.axes[i] + lift] = sch.subs[i];
+  }
+  if (with_dst)
+    for (size_t i = 0; i < dch.axes.size(); ++i) {
+      ts.dst_ind.push_back(dch.axes[i] + lift);
+      dsub[dch.axes[i] + lift] = dch.subs[i];
+    }
+  NCA_REQUIRE(stp[nd - 2] <= 64 && stp[nd - 1] <= 64, "stencil steps above 64 along the last two axes");
+  ts.stepy = static_cast<int>(stp[nd - 2]);
+  ts.stepx = static_cast<int>(stp[nd - 1]);
+  ts.fuse = fuse;
+  ts.mode = mode;
+  ts.with_dst = with_dst;
+  ts.dst_dtype = dst.dtype;
+  // staging loads of 16 bytes when the source's last axis is contiguous
+  const int64_t sisz = itemsize(src.dtype);
+  ts.vw = (sstr[nd - 1] == sisz && 16 % sisz == 0) ? static_cast<int>(16 / sisz) : 1;
+  // 32-bit offsets along the last axis when every position a tile touches fits
+  int64_t maxoff = 0;
+  for (auto& o : ts.offsets) maxoff = std::max<int64_t>(maxoff, std::llabs(o[nd - 1]));
+  const int64_t xreach = (dshape[nd - 1] + 2 * 64 * 8) * stp[nd - 1] + maxoff + 64;
+  ts.x32 = xreach * std::max<int64_t>(std::llabs(sstr[nd - 1]), sisz) < (int64_t(1) << 31);
+  ts.d32 = (dshape[nd - 1] + 64 * 8) * std::max<int64_t>(std::llabs(dstr[nd - 1]), 16) < (int64_t(1) << 31);
+  // tile: TX = 64 px, TY = 4 py, the first of these whose LDS regions fit 24 KiB (six blocks of
+  // 256 threads per CU; NCA_HIP_STENCIL_TILE = "px,py" puts another first, for tuning).  Measured
+  // on 16384^2 float32 5-point Laplacians: 4x4 429 us, 2x8 440, 2x4 448, 2x2 518; on 2x2 pixel
+  // binning of 256 x 1024^2 float32 (16 KiB at 2x2): 2x2 257 us, 2x4 (32 KiB) 283.
+  const size_t csz = itemsize(p.tape.compute_dtype);
+  std::vector<std::pair<int, int>> cand = {{4, 4}, {2, 8}, {2, 4}, {2, 2}, {1, 4}, {1, 2}, {1, 1}};
+  if (lift) cand = {{4, 1}, {2, 1}, {1, 1}};  // a 1-D problem has one row per tile
+  bool overridden = false;
+  if (const char* e = std::getenv("NCA_HIP_STENCIL_TILE")) {
+    int a = 0, b = 0;
+    if (std::sscanf(e, "%d,%d", &a, &b) == 2 && a >= 1 && a <= 8 && b >= 1 && b <= 16) {
+      cand.insert(cand.begin(), {a, b});
+      overridden = true;
+    }
+  }
+  bool fits = false;
+  for (size_t i = 0; i < cand.size() && !fits; ++i) {
+    ts.px = cand[i].first;
+    ts.py = cand[i].second;
+    ts.derive();
+    const size_t budget = (overridden && i == 0) ? 64 * 1024 : 24 * 1024;
+    fits = static_cast<size_t>(ts.lds_elems) * csz + static_cast<size_t>(ts.rows) * 8 <= budget;
+  }
+  NCA_REQUIRE(fits, "stencil offsets too wide for the tile kernel's LDS");
+  const int64_t ntx = (dshape[nd - 1] + ts.TX() - 1) / ts.TX(), nty = (dshape[nd - 2] + ts.TY() - 1) / ts.TY();
+  int64_t nouter = 1;
+  for (int d = 0; d < nd - 2; ++d) nouter *= dshape[d];
+  const int64_t ntiles = nouter * nty * ntx;
+  hipFunction_t fn = stencil_kernel(p, ts.key(), [&]() { return gen_tile(p, ts); }, "nca_stencil_tile", fixed != nullptr);
+  const int per_cu = blocks_per_cu(fn);
+  const int64_t blocks64 = std::min<int64_t>(ntiles, static_cast<int64_t>(dev.n_cus) * per_cu);
+  const int blocks = static_cast<int>(std::max<int64_t>(blocks64, 1));
+  int xcd_order = blocks >= 8 ? 1 : 0;
+  Workspace fw;
+  if (fixed) {
+    NCA_REQUIRE(static_cast<size_t>(blocks) * 24 <= kFixedPartialBytes, "too many blocks for the fixed scratch array");
+    fw.partials = fixed;
+  }
+  Workspace& w = fixed ? fw : workspace(s, static_cast<size_t>(blocks) * 24);
+  std::deque<const void*> ptrs;
+  std::deque<std::array<int64_t, NCA_MAX_NDIM>> dims;
+  std::vector<void*> params;
+  auto push_dims = [&](const std::array<int64_t, NCA_MAX_NDIM>& v) {
+    dims.push_back(v);
+    params.push_back(dims.back().data());
+  };
+  ptrs.push_back(reinterpret_cast<const void*>(sch.base));
+  params.push_back(&ptrs.back());
+  push_dims(sshape);
+  push_dims(sstr);
+  push_dims(ssub);
+  for (int j = 0; j < n_extra; ++j) {
+    std::array<int64_t, NCA_MAX_NDIM> es{};
+    for (int d = 0; d < p.ndim; ++d) es[d + lift] = extras[j].strides[d];
+    ptrs.push_back(extras[j].data);
+    params.push_back(&ptrs.back());
+    push_dims(es);
+  }
+  ptrs.push_back(with_dst ? reinterpret_cast<const void*>(dch.base) : nullptr);
+  params.push_back(&ptrs.back());
+  push_dims(dshape);
+  push_dims(dstr);
+  push_dims(dsub);
+  push_dims(stp);
+  int64_t a_ntiles = ntiles, a_ntx = ntx, a_nty = nty;
+  params.push_back(&a_ntiles);
+  params.push_back(&a_ntx);
+  params.push_back(&a_nty);
+  params.push_back(&xcd_order);
+  void* psum = w.partials;
+  void* pmax = static_cast<char*>(w.partials) + static_cast<size_t>(blocks) * 8;
+  void* pmin = static_cast<char*>(w.partials) + static_cast<size_t>(blocks) * 16;
+  if (fuse) {
+    params.push_back(&psum);
+    params.push_back(&pmax);
+    params.push_back(&pmin);
+  }
+  for (auto& x : sc.slots) params.push_back(&x);
+  (void)n;
+  {
+    ProfScope ps("nca_stencil_tile", s);
+    jit_launch(fn, dim3(static_cast<unsigned>(blocks)), dim3(256), 0, s, params);
+  }
+  if (fuse) {
+    FinalizeArgs fa;
+    fa.value_dtype = dst.dtype;
+    fa.fuse = fuse;
+    fa.psum = psum;
+    fa.pmax = pmax;
+    fa.pmin = pmin;
+    fa.nblocks = blocks;
+    fa.sum_out = sum_out;
+    fa.max_out = max_out;
+    fa.min_out = min_out;
+    launch_finalize_fused(fa, s);
+  }
+}
+
 // ---- apply ------------------------------------------------------------------------------------------------
 // fixed: NULL, or this device's fixed scratch array (kFixedScratchBytes) used instead of the
 // per-stream workspace; the call then allocates nothing (see nca_hip_stencil_apply_reduce_noalloc).
@@ -1415,9 +1837,19 @@ static void stencil_apply_impl(uint64_t handle, const nca_view* src, const nca_v
   StencilProgram& p = stencil_program(handle);
   const Tape& t = p.tape;
   NCA_REQUIRE(src && dst, "null src/dst");
-  validate_view(*src, "src");
+  validate_view_nc(*src, "src");
   const bool with_dst = dst->data != nullptr;
-  validate_view(*dst, "dst", true);
+  validate_view_nc(*dst, "dst", true);
+  if (p.layout_fixed) {
+    // Stencil<NDIM>::create's is_pointer_axis / is_soarr (stencil.hh:76-81): the kernel serves
+    // sources with these indirect axes only
+    const Chain srcc = chain_of(*src);
+    std::vector<int> want;
+    for (int d = 0; d < p.ndim; ++d)
+      if (p.pointer_axes[d]) want.push_back(d);
+    NCA_REQUIRE(srcc.axes == want && (src->layout == NCA_LAYOUT_SO) == p.is_soarr,
+                "src layout differs from the pointer axes / SO flag the stencil was created for");
+  }
   NCA_REQUIRE(src->ndim == p.ndim && dst->ndim == p.ndim, "src/dst rank differs from the stencil's NDIM");
   NCA_REQUIRE(src->dtype == p.src_dtype, std::string("src dtype must be ") + dtype_info(p.src_dtype).name);
   NCA_REQUIRE(dst->dtype == t.dest_dtype, std::string("dst dtype must be ") + dtype_info(t.dest_dtype).name);
@@ -1579,11 +2011,30 @@ static void stencil_apply_impl(uint64_t handle, const nca_view* src, const nca_v
     }
     return;
   }
+  // tile: every stencil without hooks whose source and destination read their last axis directly
+  {
+    const Chain sch = chain_of(*src);
+    const Chain dch = with_dst ? chain_of(*dst) : Chain{};
+    const int last = p.ndim - 1;
+    const bool direct_x = std::find(sch.axes.begin(), sch.axes.end(), last) == sch.axes.end() &&
+                          std::find(dch.axes.begin(), dch.axes.end(), last) == dch.axes.end();
+    if (!p.has_ext && direct_x) {
+      launch_tile(p, *src, sch, extras, n_extra, *dst, dch, with_dst, step, n, fuse, mode, sum_out, max_out, min_out, sc,
+                  s, fixed);
+      return;
+    }
+  }
   // direct
   const bool zero = boundary == NCA_BOUNDARY_ZERO;
-  std::string key = "d|" + std::to_string(zero) + "|" + std::to_string(fuse) + "|" + std::to_string(mode);
-  hipFunction_t fn = stencil_kernel(p, key, [&]() { return gen_direct(p, dst->dtype, zero, fuse, mode); }, "nca_stencil_direct",
-                                    fixed != nullptr);
+  const Chain sch = chain_of(*src);
+  const Chain dch = with_dst ? chain_of(*dst) : Chain{};
+  std::string key = "d|" + std::to_string(zero) + "|" + std::to_string(fuse) + "|" + std::to_string(mode) + "|s";
+  for (int a : sch.axes) key += std::to_string(a) + ",";
+  key += "|d";
+  for (int a : dch.axes) key += std::to_string(a) + ",";
+  hipFunction_t fn = stencil_kernel(
+      p, key, [&]() { return gen_direct(p, dst->dtype, zero, fuse, mode, sch.axes, dch.axes); }, "nca_stencil_direct",
+      fixed != nullptr);
   int blocks;
   if (p.has_ext) {
     NCA_REQUIRE((n + 255) / 256 < (int64_t(1) << 31), "grid too large");
@@ -1607,19 +2058,26 @@ static void stencil_apply_impl(uint64_t handle, const nca_view* src, const nca_v
     dims.push_back(a);
     params.push_back(dims.back().data());
   };
-  ptrs.push_back(src->data);
+  auto subs_of = [&](const Chain& c) {
+    int64_t v[NCA_MAX_NDIM] = {};
+    for (size_t i = 0; i < c.axes.size(); ++i) v[c.axes[i]] = c.subs[i];
+    push_dims(v);
+  };
+  ptrs.push_back(reinterpret_cast<const void*>(sch.base));
   params.push_back(&ptrs.back());
   push_dims(src->shape);
-  push_dims(src->strides);
+  push_dims(sch.strides.data());
+  if (!sch.axes.empty()) subs_of(sch);
   for (int j = 0; j < n_extra; ++j) {
     ptrs.push_back(extras[j].data);
     params.push_back(&ptrs.back());
     push_dims(extras[j].strides);
   }
-  ptrs.push_back(with_dst ? dst->data : nullptr);
+  ptrs.push_back(with_dst ? reinterpret_cast<const void*>(dch.base) : nullptr);
   params.push_back(&ptrs.back());
   push_dims(dst->shape);
-  push_dims(dst->strides);
+  push_dims(with_dst ? dch.strides.data() : dst->strides);
+  if (!dch.axes.empty()) subs_of(dch);
   push_dims(step);
   int64_t nn = n;
   params.push_back(&nn);
@@ -1686,6 +2144,18 @@ int nca_hip_stencil_create(const nca_stencil_desc* desc, uint64_t* handle) {
     p->prologue = desc->prologue ? desc->prologue : "";
     p->epilogue = desc->epilogue ? desc->epilogue : "";
     p->has_ext = !p->extra_params.empty() || !p->prologue.empty() || !p->epilogue.empty();
+    if (desc->is_pointer_axis) {
+      p->layout_fixed = true;
+      int n_ptr = 0;
+      for (int d = 0; d < desc->ndim; ++d) {
+        p->pointer_axes[d] = desc->is_pointer_axis[d] ? 1 : 0;
+        n_ptr += p->pointer_axes[d];
+      }
+      p->is_soarr = desc->is_soarr != 0;
+      NCA_REQUIRE(p->is_soarr || n_ptr <= 1, "an NC source has at most one pointer axis (use is_soarr for more)");
+    } else {
+      NCA_REQUIRE(desc->is_soarr == 0, "is_soarr needs is_pointer_axis");
+    }
     *handle = stencil_register(std::move(p));
   });
 }
@@ -2285,6 +2755,27 @@ int nca_hip_stencil_source(uint64_t handle, int32_t variant, char* buf, size_t n
       if (variant == 4) fz.mode = NCA_ACCUM_COMPENSATED;
       const MarchPlan mp = march_plan(p);
       src = mp.queue ? gen_march_q(p, fz, p.tape.dest_dtype, mp) : gen_march(p, fz, p.tape.dest_dtype);
+    } else if (variant == 5 || variant == 6) {
+      // the tile kernel at unit steps: 5 plain; 6 with fused sum/max/min (compensated) and the
+      // first axis read through a table of pointers in the source and the destination
+      const int lift = p.ndim == 1 ? 1 : 0;
+      TileSpec ts;
+      ts.nd = p.ndim + lift;
+      for (auto& o : p.offsets) {
+        std::array<int64_t, NCA_MAX_NDIM> q{};
+        for (int d = 0; d < p.ndim; ++d) q[d + lift] = o[d];
+        ts.offsets.push_back(q);
+      }
+      ts.dst_dtype = p.tape.dest_dtype;
+      ts.vw = static_cast<int>(16 / itemsize(p.src_dtype));
+      if (variant == 6) {
+        ts.fuse = NCA_FUSE_SUM | NCA_FUSE_MAX | NCA_FUSE_MIN;
+        ts.mode = NCA_ACCUM_COMPENSATED;
+        ts.src_ind = {0};
+        ts.dst_ind = {0};
+      }
+      ts.derive();
+      src = gen_tile(p, ts);
     } else {
       src = gen_direct(p, p.tape.dest_dtype, variant == 3, 0, 0);
     }

@@ -968,7 +968,8 @@ void HIPEngine::execute_scatter_add(const NCDevArray& dest, const NCDevArray& id
 // ---- stencils ------------------------------------------------------------------------------
 namespace detail {
 uint64_t stencil_create(int ndim, const std::vector<int64_t>& offsets, DType src, const Expr& e,
-                        const StencilJITExtensions& ext) {
+                        const StencilJITExtensions& ext, const std::vector<uint8_t>& is_pointer_axis,
+                        bool is_soarr) {
   ensure_init();
   const int n_off = static_cast<int>(offsets.size()) / ndim;
   const int n_loads = static_cast<int>(e.views.size());
@@ -997,6 +998,8 @@ uint64_t stencil_create(int ndim, const std::vector<int64_t>& offsets, DType src
   sd.extra_params = ext.extra_params.empty() ? nullptr : ext.extra_params.c_str();
   sd.prologue = ext.prologue.empty() ? nullptr : ext.prologue.c_str();
   sd.epilogue = ext.epilogue.empty() ? nullptr : ext.epilogue.c_str();
+  sd.is_pointer_axis = is_pointer_axis.empty() ? nullptr : is_pointer_axis.data();
+  sd.is_soarr = is_soarr ? 1 : 0;
   uint64_t h = 0;
   check(nca_hip_stencil_create(&sd, &h), "stencil create");
   return h;
@@ -1006,15 +1009,12 @@ void stencil_apply(uint64_t h, const std::vector<Scalar>& scalars, const NCDevAr
                    void* stream, Boundary b, const std::vector<void*>& ext_args) {
   ensure_init();
   std::vector<nca_scalar> sc = pack_scalars(scalars, nullptr);
-  // pointer-axis arrays go through staged dense copies
-  const NCDevArray sd = src.dense();
-  const NCDevArray dd = dest.pointer_axis() >= 0 ? NCDevArray(dest.shape_vec(), dest.dtype()) : dest;
-  const nca_view s = sd.cview(), d = dd.cview();
+  // pointer-axis sources and destinations are read and written in place by the kernels
+  const nca_view s = src.cview(), d = dest.cview();
   check(nca_hip_stencil_apply(h, &s, nullptr, 0, sc.data(), static_cast<int32_t>(scalars.size()), &d,
                               static_cast<int32_t>(b), ext_args.empty() ? nullptr : ext_args.data(),
                               static_cast<int32_t>(ext_args.size()), stream),
         "stencil apply");
-  if (dest.pointer_axis() >= 0) HIPEngine::execute_assign(dest, dd, stream);
 }
 
 uint64_t stencil_adjoint(uint64_t h) {

@@ -415,18 +415,21 @@ class Stencil {
 
   // op_func receives one placeholder array per offset (in offset order) and returns the
   // expression over them; the k-th load in tape order reads offsets[k]
-  // (rtcompiler.hh:211-215).  Pointer axes are not supported (is_pointer_axis must be 0).
+  // (rtcompiler.hh:211-215).  is_pointer_axis (NDIM flags) and is_soarr declare the sources the
+  // stencil serves, as in the reference (stencil.hh:76-81): apply then takes only sources whose
+  // pointer axes are the flagged ones; an empty vector accepts any layout.  The kernels read
+  // pointer-axis sources and destinations in place (one pointer read per row).
   template <typename SrcT, typename Func>
   static Stencil create(const std::vector<Offset>& offsets, const std::vector<uint8_t>& is_pointer_axis,
-                        Func&& op_func, const StencilJITExtensions& ext = {}) {
-    return create(dtype_traits<SrcT>::value, offsets, is_pointer_axis, std::forward<Func>(op_func), ext);
+                        Func&& op_func, const StencilJITExtensions& ext = {}, bool is_soarr = false) {
+    return create(dtype_traits<SrcT>::value, offsets, is_pointer_axis, std::forward<Func>(op_func), ext, is_soarr);
   }
   template <typename Func>
   static Stencil create(DType src_dtype, const std::vector<Offset>& offsets,
                         const std::vector<uint8_t>& is_pointer_axis, Func&& op_func,
-                        const StencilJITExtensions& ext = {}) {
-    for (uint8_t p : is_pointer_axis)
-      if (p) throw Error(NCA_ERR_UNSUPPORTED, "stencils over pointer axes are not supported");
+                        const StencilJITExtensions& ext = {}, bool is_soarr = false) {
+    if (!is_pointer_axis.empty() && is_pointer_axis.size() != static_cast<size_t>(NDIM))
+      throw Error(NCA_ERR_INVALID, "is_pointer_axis needs one flag per axis");
     std::vector<NCDevArray> ph;
     const NCDevArray::Shape unit(NDIM, 1);
     for (size_t k = 0; k < offsets.size(); ++k) ph.push_back(NCDevArray::wrap(nullptr, unit, unit, src_dtype));
@@ -434,7 +437,7 @@ class Stencil {
     std::vector<int64_t> flat;
     for (const Offset& o : offsets) flat.insert(flat.end(), o.begin(), o.end());
     Stencil s;
-    s.handle_ = create_handle(NDIM, flat, src_dtype, e, ext);
+    s.handle_ = create_handle(NDIM, flat, src_dtype, e, ext, is_pointer_axis, is_soarr);
     s.scalars_ = e.scalars;
     s.dest_dtype_ = e.expr_dtype;
     s.has_ext_ = !ext.extra_params.empty();
@@ -459,7 +462,8 @@ class Stencil {
 
  private:
   static uint64_t create_handle(int ndim, const std::vector<int64_t>& offsets, DType src, const Expr& e,
-                                const StencilJITExtensions& ext);
+                                const StencilJITExtensions& ext, const std::vector<uint8_t>& is_pointer_axis,
+                                bool is_soarr);
   static void apply_handle(uint64_t h, const std::vector<Scalar>& scalars, const NCDevArray& src,
                            const NCDevArray& dest, void* stream, Boundary b, const std::vector<void*>& ext_args);
   static uint64_t adjoint_handle(uint64_t h);
@@ -472,7 +476,8 @@ class Stencil {
 
 namespace detail {
 uint64_t stencil_create(int ndim, const std::vector<int64_t>& offsets, DType src, const Expr& e,
-                        const StencilJITExtensions& ext);
+                        const StencilJITExtensions& ext, const std::vector<uint8_t>& is_pointer_axis = {},
+                        bool is_soarr = false);
 void stencil_apply(uint64_t h, const std::vector<Scalar>& scalars, const NCDevArray& src, const NCDevArray& dest,
                    void* stream, Boundary b, const std::vector<void*>& ext_args);
 uint64_t stencil_adjoint(uint64_t h);
@@ -481,8 +486,9 @@ std::vector<nca_scalar> pack_scalars(const std::vector<Scalar>& s, std::vector<i
 
 template <int NDIM>
 uint64_t Stencil<NDIM>::create_handle(int ndim, const std::vector<int64_t>& offsets, DType src, const Expr& e,
-                                      const StencilJITExtensions& ext) {
-  return detail::stencil_create(ndim, offsets, src, e, ext);
+                                      const StencilJITExtensions& ext, const std::vector<uint8_t>& is_pointer_axis,
+                                      bool is_soarr) {
+  return detail::stencil_create(ndim, offsets, src, e, ext, is_pointer_axis, is_soarr);
 }
 template <int NDIM>
 void Stencil<NDIM>::apply_handle(uint64_t h, const std::vector<Scalar>& scalars, const NCDevArray& src,

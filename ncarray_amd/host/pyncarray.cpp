@@ -761,7 +761,8 @@ PYBIND11_MODULE(_ncarray, m) {
   // ---- stencils -------------------------------------------------------------------------------
   py::class_<PyStencil>(m, "Stencil", "Stencil<NDIM> (expression/stencil.hh:76-243) with NDIM = len(offsets[0])")
       .def(py::init([](const std::vector<std::vector<int64_t>>& offsets, const py::function& func, DType src_dtype,
-                       const std::string& extra_params, const std::string& prologue, const std::string& epilogue) {
+                       const std::string& extra_params, const std::string& prologue, const std::string& epilogue,
+                       const std::vector<uint8_t>& is_pointer_axis, bool is_soarr) {
         if (offsets.empty()) throw Error(NCA_ERR_INVALID, "a stencil needs at least one offset");
         const int ndim = static_cast<int>(offsets[0].size());
         std::vector<int64_t> flat;
@@ -785,13 +786,16 @@ PYBIND11_MODULE(_ncarray, m) {
         Expr e = py::isinstance<Expr>(r) ? r.cast<Expr>() : Expr(r.cast<NCDevArray>());
         StencilJITExtensions ext{extra_params, prologue, epilogue};
         PyStencil s;
-        s.handle = detail::stencil_create(ndim, flat, src_dtype, e, ext);
+        if (!is_pointer_axis.empty() && static_cast<int>(is_pointer_axis.size()) != ndim)
+          throw Error(NCA_ERR_INVALID, "is_pointer_axis needs one flag per axis");
+        s.handle = detail::stencil_create(ndim, flat, src_dtype, e, ext, is_pointer_axis, is_soarr);
         s.scalars = e.scalars;
         s.ndim = ndim;
         s.dest_dtype = e.expr_dtype;
         return s;
       }), py::arg("offsets"), py::arg("func"), py::arg("src_dtype") = DType::float32, py::arg("extra_params") = "",
-          py::arg("prologue") = "", py::arg("epilogue") = "")
+          py::arg("prologue") = "", py::arg("epilogue") = "", py::arg("is_pointer_axis") = std::vector<uint8_t>{},
+          py::arg("is_soarr") = false)
       .def_property_readonly("dest_dtype", [](const PyStencil& s) { return s.dest_dtype; })
       .def_property_readonly("handle", [](const PyStencil& s) { return s.handle; })
       .def("apply", [](const PyStencil& s, const NCDevArray& src, const NCDevArray& dest, Boundary b) {

@@ -13,6 +13,8 @@
 #include <string>
 #include <vector>
 
+#include <hip/hip_runtime_api.h>
+
 #include "ncarray_amd.hh"
 
 namespace nca = ncarray_amd;
@@ -406,6 +408,54 @@ TEST(NCPointerAxis, RowsIndexingAndExpressions) {
   nca::NCDevArray twice(ref * 2.0f);
   EXPECT_TRUE(twice.is_contiguous() && twice.pointer_axis() == -1);
   EXPECT_FLOAT_EQ(nca::scalar_to_double(twice(2, 4).item()), 48.0);
+}
+
+// Stencil<3>::create with is_pointer_axis = {1, 0, 0} (stencil.hh:76-81) applied to a stack of
+// separately allocated modules and written into modules reached the same way, on a
+// non-blocking stream: the kernel reads and writes the rows in place (no staged copies), and
+// the result equals the same stencil over the dense stack.
+TEST(NCPointerAxis, StencilOverModulesOnAStream) {
+  const int M = 6, Y = 24, X = 40;
+  std::mt19937 rng(5);
+  std::normal_distribution<float> nd(0.f, 1.f);
+  std::vector<float> h(M * Y * X);
+  for (float& v : h) v = nd(rng);
+  std::vector<nca::NCDevArray> mods, outs;
+  for (int m = 0; m < M; ++m) {
+    nca::NCDevArray a(Shape{Y, X});
+    a.assign_from_host(h.data() + m * Y * X, nca::DType::float32, Shape{Y, X});
+    mods.push_back(a);
+    outs.push_back(nca::NCDevArray(Shape{Y, X}));
+  }
+  nca::NCDevArray src = nca::NCDevArray::from_rows(mods), dst = nca::NCDevArray::from_rows(outs);
+  nca::NCDevArray dense(Shape{M, Y, X}), want(Shape{M, Y, X});
+  dense.assign_from_host(h.data(), nca::DType::float32, Shape{M, Y, X});
+  using Off = nca::Stencil<3>::Offset;
+  std::vector<Off> offs = {Off{-1, 0, 0}, Off{1, 0, 0}, Off{0, -1, 0}, Off{0, 1, 0}, Off{0, 0, -1}, Off{0, 0, 1}, Off{0, 0, 0}};
+  auto f = [](std::vector<nca::NCDevArray>& v) { return v[0] + v[1] + v[2] + v[3] + v[4] + v[5] - 6.0f * v[6]; };
+  auto on_modules = nca::Stencil<3>::create<float>(offs, {1, 0, 0}, f);
+  auto on_dense = nca::Stencil<3>::create<float>(offs, {0, 0, 0}, f);
+  on_dense.apply(dense, want, std::nullopt, nca::Boundary::zero);
+  hipStream_t s = nullptr;
+  EXPECT_TRUE(hipStreamCreateWithFlags(&s, hipStreamNonBlocking) == hipSuccess);
+  on_modules.apply(src, dst, std::optional<void*>(static_cast<void*>(s)), nca::Boundary::zero);
+  EXPECT_TRUE(hipStreamSynchronize(s) == hipSuccess);
+  EXPECT_TRUE(hipStreamDestroy(s) == hipSuccess);
+  std::vector<float> w = host_f32(want);
+  int bad = 0;
+  for (int m = 0; m < M; ++m) {
+    std::vector<float> o = host_f32(outs[m]);
+    for (int i = 0; i < Y * X; ++i)
+      if (o[i] != w[m * Y * X + i]) ++bad;
+  }
+  EXPECT_EQ(bad, 0);
+  bool refused = false;
+  try {
+    on_dense.apply(src, dst, std::nullopt, nca::Boundary::zero);  // created for dense sources
+  } catch (const std::exception&) {
+    refused = true;
+  }
+  EXPECT_TRUE(refused);
 }
 
 int main(int argc, char** argv) {

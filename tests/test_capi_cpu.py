@@ -110,7 +110,7 @@ def test_linearizable_vs_vm_evaluation_dtype():
 @pytest.mark.parametrize("dt", [capi.FLOAT32, capi.BFLOAT16])
 def test_stencil_kernels_compile(dt):
     st = capi.Stencil(tapes.LAPLACIAN_OFFSETS, tapes.laplacian(), dt, capi.FLOAT32, dt, [capi.FLOAT32])
-    for variant in (0, 1, 2, 3, 4):
+    for variant in (0, 1, 2, 3, 4, 5, 6):
         _compiles(st.source(variant))
     # compensated column sums: one Kahan step per plane and column
     assert "nca_kahan_add(cs[0][0], cc[0][0]" in st.source(4) and "cc[" not in st.source(1)
@@ -162,6 +162,25 @@ def test_adjoint_source_negates_offsets():
                       capi.FLOAT32, capi.FLOAT32, capi.FLOAT32)
     src = st.adjoint().source(2)
     assert "+ (-1)" in src
+
+
+@pytest.mark.parametrize("dt", [capi.FLOAT32, capi.BFLOAT16])
+def test_tile_kernels_compile(dt):
+    """The tile kernel for a 2-D 5-point Laplacian, a 1-D three-point stencil (launched as 2-D)
+    and the 2x2x2 pixel-binning window, plain and with pointer-read rows and fused reductions."""
+    lap2 = [[0, 0], [-1, 0], [1, 0], [0, -1], [0, 1]]
+    five = [P(capi.OP_LOAD_NCARR, 1), P(capi.OP_LOAD_NCARR, 2), P(capi.OP_ADD), P(capi.OP_LOAD_NCARR, 3),
+            P(capi.OP_ADD), P(capi.OP_LOAD_NCARR, 4), P(capi.OP_ADD), P(capi.OP_LOAD_CONST, 0),
+            P(capi.OP_LOAD_NCARR, 0), P(capi.OP_MUL), P(capi.OP_SUB)]
+    win = [[a, b, c] for a in (0, 1) for b in (0, 1) for c in (0, 1)]
+    for st in (capi.Stencil(lap2, five, dt, capi.FLOAT32, dt, [capi.FLOAT32]),
+               capi.Stencil([[-1], [0], [1]], _sum_tape(3), dt, capi.FLOAT32, capi.FLOAT32),
+               capi.Stencil(win, _sum_tape(8), dt, capi.FLOAT32, capi.FLOAT32)):
+        plain, fused = st.source(5), st.source(6)
+        _compiles(plain)
+        _compiles(fused)
+        assert "nca_stencil_tile" in plain and "rp = nca_chase" not in plain
+        assert "rp = nca_chase(rp, ssub.v[0])" in fused and "rd = nca_chase(rd, dsub.v[0])" in fused
 
 
 def test_stencil_rejects_bad_descriptions():

@@ -1,7 +1,7 @@
 """max/min/argmax/argmin on data dense in NaN and signed zeros, on every kernel that computes
 them: the full reduction, the three axis-reduction kernels (column, row and generic layouts),
 the expression kernel's fused max/min, the ragged batch, and the stencils' fused max/min (march
-and direct forms).  The rule (the same in the oracle): value max/min never select NaN and order
+and tile forms).  The rule (the same in the oracle): value max/min never select NaN and order
 -0 below +0, so the bits do not depend on the order of combination; argmax/argmin are the host
 fold's first occurrence with zeros equal (host/reductions.hh:130-212).  All results bit-exact
 against the oracle."""
@@ -165,7 +165,7 @@ def test_batch_extremes(dev):
     rb.close()
 
 
-@pytest.mark.parametrize("shape,form", [((16, 8, 64), "march"), ((9, 8, 62), "direct")])
+@pytest.mark.parametrize("shape,form", [((16, 8, 64), "march"), ((9, 8, 62), "tile")])
 def test_stencil_fused_extremes(dev, shape, form):
     """Fused max/min of an identity stencil (the stored values are the input's, so the data
     decides every tie) against the oracle's fold of the stored output."""

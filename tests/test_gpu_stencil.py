@@ -38,7 +38,7 @@ def test_laplacian_bit_exact(dev, shape):
 
 
 def test_laplacian_direct_path_matches(dev):
-    """A shape the march kernel does not take (last extent % 4 != 0) runs the direct kernel."""
+    """A shape the march kernel does not take (last extent % 4 != 0) runs the tile kernel."""
     capi, tapes, ti = _c()
     shape = (9, 10, 11)
     x = np.random.default_rng(0).standard_normal(shape).astype(np.float32)
@@ -257,7 +257,7 @@ def test_march_general_offsets(dev, offs, src_code, shape):
 
 
 @pytest.mark.parametrize("dt", ["float32", "bfloat16"])
-@pytest.mark.parametrize("march", [True, False], ids=["march", "direct"])
+@pytest.mark.parametrize("march", [True, False], ids=["march", "tile"])
 def test_fused_max_min_skip_nan(dev, dt, march):
     """Fused max/min of a stencil output that holds NaN: NaN is never selected and every other
     value is still considered (maximumNumber/minimumNumber), also when the NaN and the extreme value

@@ -1,7 +1,7 @@
 """The allocation-free fused stencil entry point (nca_hip_stencil_apply_reduce_noalloc) against
 the ordinary one (nca_hip_stencil_apply_reduce), through the C ABI: bit-identical stencil
 values, axis-0 column sums and max/sum, on consecutive calls whose second input is smaller (the
-fixed scratch array keeps the previous call's contents), on the march and the direct kernels,
+fixed scratch array keeps the previous call's contents), on the march and the tile kernels,
 at the full 512^3 size, and the error for a kernel that was never loaded."""
 import ctypes
 
@@ -53,8 +53,8 @@ def _same(a, b):
 
 
 @pytest.mark.parametrize("shapes,axis0", [([(64, 96, 128), (20, 40, 64)], True),    # march kernel
-                                          ([(48, 70, 62), (9, 11, 14)], False)],    # direct kernel
-                         ids=["march", "direct"])
+                                          ([(48, 70, 62), (9, 11, 14)], False)],    # tile kernel
+                         ids=["march", "tile"])
 def test_noalloc_matches_ordinary_entry(dev, shapes, axis0):
     capi, tapes, ti = _c()
     st = _lap(capi, tapes)
