@@ -115,8 +115,10 @@ enum { NCA_COUNT_ALL = 0, NCA_COUNT_NONZERO = 1 };
  *     p += i_d * strides[d];  if (suboffsets[d] >= 0) p = *(char**)p + suboffsets[d];
  * so any number of axes may be indirect.  suboffsets is read only for this layout; an SO
  * view with no suboffset >= 0 is a plain strided view and is accepted everywhere.
- * The expression entry points (expr_launch, expr_launch_reduce, expr_vjp) and the stencil
- * apply entry points accept views with an NC pointer axis or an SO suboffset >= 0. */
+ * Views with an NC pointer axis or an SO suboffset >= 0 are read (and written) in place by the
+ * expression entry points, stencil apply, full and axis reductions (input), scatter_add and
+ * binning (frames through a table of frame pointers); the other entry points (the training
+ * kernels, ragged batches) take strided views. */
 typedef struct nca_view {
   void* data;
   int32_t dtype;
@@ -414,7 +416,8 @@ int nca_hip_stencil_source(uint64_t handle, int32_t variant, char* buf, size_t n
 
 /* dest[idx[i]] += src[i] for 0 <= idx[i] < dest.size, other indices skipped
  * (scatter_add_kernel, kernels.cuh:660-680; host::scatter_add_recursive,
- * host/elementwise.hh:482-505).  dest: 1-D contiguous, src/idx: same shape. */
+ * host/elementwise.hh:482-505).  i runs over src and idx (same size) in C order; idx[i] is a
+ * C-order linear index into dest.  Any layouts, including NC pointer-axis and SO views. */
 int nca_hip_scatter_add(const nca_view* dest, const nca_view* idx, const nca_view* src,
                         void* stream);
 /* Per-frame binning by an index map: frames is [F, P...] (P = map size), binmap is an
@@ -422,7 +425,9 @@ int nca_hip_scatter_add(const nca_view* dest, const nca_view* idx, const nca_vie
  * of pixels per bin (pred NCA_COUNT_NONZERO: only pixels with value != 0), sums
  * [F, n_bins] (float32 or float64) the sum of pixel values per bin.  Either output may
  * be NULL.  Out-of-range bins are skipped.  Counts are exact; sums are accumulated in
- * float64 inside each block and merged across blocks in float64. */
+ * float64 inside each block (atomics in hardware order) and merged across blocks in float64 in
+ * a fixed order.  frames may be C-contiguous or reached through a table of frame pointers (an
+ * NC pointer axis 0, or an SO view whose only indirect axis is 0) with contiguous pixels. */
 int nca_hip_bin(const nca_view* frames, const nca_view* binmap, int32_t n_bins,
                 int32_t count_pred, const nca_view* counts, const nca_view* sums,
                 void* stream);

@@ -7,9 +7,11 @@
 // (non-temporal) and bins, and adds every pixel to its bin's sum with one LDS atomic.  The
 // valid pixels of each bin are counted from the map alone, once per call, spread over the
 // blocks of the first frame groups.  Blocks write their tables to partial slots; a combine
-// kernel adds the chunk partials of each (frame, bin) in chunk order, so counts are exact and
-// the sums do not depend on block scheduling (float64 in LDS makes the in-block order
-// immaterial at float32 output precision).
+// kernel adds the chunk partials of each (frame, bin) in chunk order.  Counts are exact.  The
+// float64 LDS atomics of a block run in hardware order, so a float64 sum can differ in its last
+// bits from run to run (exact for integer-valued pixels below 2^53); a float32 sum is that
+// float64 value rounded once, within one ulp of the FP64 sum and equal between runs unless the
+// float64 values of two runs straddle a float32 rounding boundary.
 //
 // nca_hip_bin_backward: grad_frames[f, p] = grad_sums[f, map[p]] (0 out of range): the
 // gradient row of each frame is staged in LDS and gathered.
@@ -27,6 +29,22 @@ namespace nca {
 // Counting modes of the histogram kernel.
 enum { BIN_CNT_NONE = 0, BIN_CNT_ALL = 1, BIN_CNT_NONZERO = 2 };
 
+// Where frame f starts: dense frames at base + f * fstride; frames reached through a table of
+// pointers (an NC pointer axis 0 or an SO view indirect at axis 0, each frame's pixels
+// contiguous) at the pointer stored at base + f * fstride, plus sub bytes.
+struct FrameAddr {
+  const char* base;
+  nca_i64 fstride;
+  nca_i64 sub;
+  int ind;
+};
+template <typename T>
+__device__ __forceinline__ T* frame_ptr(const FrameAddr& a, nca_i64 f) {
+  const char* p = a.base + f * a.fstride;
+  if (a.ind) p = nca_chase(p, a.sub);
+  return reinterpret_cast<T*>(const_cast<char*>(p));
+}
+
 // One block: a chunk of pixels of FB consecutive frames (the map is read once for all FB).
 // Every pixel makes one float64 LDS atomic and no branch: a nonzero pixel adds its value to
 // its bin's sum, a zero pixel adds 1.0 to its bin's zero count (a float64 table: exact up to
@@ -40,7 +58,7 @@ enum { BIN_CNT_NONE = 0, BIN_CNT_ALL = 1, BIN_CNT_NONZERO = 2 };
 // equal bins does not pay; the atomics run at LDS rate while the frame loads (non-temporal,
 // read once) stay in flight.
 template <typename T, typename I, int FB, bool SUMS, int CNT>
-__global__ __launch_bounds__(1024) void k_bin(const T* __restrict__ frames, const I* __restrict__ map, nca_i64 P,
+__global__ __launch_bounds__(1024) void k_bin(FrameAddr fa, const I* __restrict__ map, nca_i64 P,
                                               int nb, nca_i64 chunk, nca_i64 F, int S, nca_i64 slice, int R,
                                               double* __restrict__ psum, unsigned* __restrict__ pzero,
                                               unsigned* __restrict__ pvalid) {
@@ -83,10 +101,19 @@ __global__ __launch_bounds__(1024) void k_bin(const T* __restrict__ frames, cons
     }
   };
   if (SUMS || CNT == BIN_CNT_NONZERO) {
-    // vector body: 4 pixels per vector (host guarantees p0 % 4 == 0 and aligned rows); U vectors
-    // per frame are loaded before any is folded in, so U * FB loads stay in flight
+    // the block's frames; a frame reached through a pointer that is not vector-aligned sends the
+    // block's pixels through the element loop
+    const T* fp[FB];
+    bool aligned = true;
+#pragma unroll
+    for (int fb = 0; fb < FB; ++fb) {
+      fp[fb] = frame_ptr<const T>(fa, f0 + (fb < nf ? fb : 0));
+      aligned = aligned && (reinterpret_cast<unsigned long long>(fp[fb]) % (4 * sizeof(T))) == 0;
+    }
+    // vector body: 4 pixels per vector (host guarantees p0 % 4 == 0 and aligned dense frames); U
+    // vectors per frame are loaded before any is folded in, so U * FB loads stay in flight
     constexpr int U = (FB >= 4) ? 1 : (4 / FB);
-    const nca_i64 nvec = (p1 - p0) / 4;
+    const nca_i64 nvec = aligned ? (p1 - p0) / 4 : 0;
     for (nca_i64 v0 = threadIdx.x; v0 < nvec; v0 += (nca_i64)U * blockDim.x) {
       nca_vec<I, 4> m[U];
       nca_vec<T, 4> x[U][FB];
@@ -97,7 +124,7 @@ __global__ __launch_bounds__(1024) void k_bin(const T* __restrict__ frames, cons
         m[u] = *reinterpret_cast<const nca_vec<I, 4>*>(map + p);
 #pragma unroll
         for (int fb = 0; fb < FB; ++fb)
-          x[u][fb] = nca_load_nt(reinterpret_cast<const nca_vec<T, 4>*>(frames + (f0 + (fb < nf ? fb : 0)) * P + p));
+          x[u][fb] = nca_load_nt(reinterpret_cast<const nca_vec<T, 4>*>(fp[fb] + p));
       }
 #pragma unroll
       for (int u = 0; u < U; ++u) {
@@ -113,7 +140,7 @@ __global__ __launch_bounds__(1024) void k_bin(const T* __restrict__ frames, cons
     }
     for (nca_i64 p = p0 + 4 * nvec + threadIdx.x; p < p1; p += blockDim.x) {
       const int b = slot(map[p]);
-      for (int fb = 0; fb < nf; ++fb) add(fb, b, nca_cvt<double, T>::from(frames[(f0 + fb) * P + p]));
+      for (int fb = 0; fb < nf; ++fb) add(fb, b, nca_cvt<double, T>::from(fp[fb][p]));
     }
   }
   if (count_valid) {
@@ -209,7 +236,7 @@ __global__ __launch_bounds__(256) void k_bin_combine(const double* __restrict__ 
 template <typename G, typename I, typename O, int FB>
 __global__ __launch_bounds__(256) void k_bin_backward(const G* __restrict__ g, nca_i64 gst0, nca_i64 gst1,
                                                       const I* __restrict__ map, nca_i64 P, int nb,
-                                                      O* __restrict__ out, nca_i64 chunk, nca_i64 F) {
+                                                      FrameAddr oa, nca_i64 chunk, nca_i64 F) {
   // one block: a chunk of pixels of FB consecutive frames; each map vector serves FB rows
   extern __shared__ unsigned char lds_raw[];
   float* gl = reinterpret_cast<float*>(lds_raw);  // [FB][nb]
@@ -222,7 +249,14 @@ __global__ __launch_bounds__(256) void k_bin_backward(const G* __restrict__ g, n
   __syncthreads();
   const nca_i64 p0 = (nca_i64)blockIdx.x * chunk;
   const nca_i64 p1 = (p0 + chunk < P) ? p0 + chunk : P;
-  const nca_i64 nvec = (p1 - p0) / 4;
+  O* op[FB];
+  bool aligned = true;
+#pragma unroll
+  for (int fb = 0; fb < FB; ++fb) {
+    op[fb] = frame_ptr<O>(oa, f0 + (fb < nf ? fb : 0));
+    aligned = aligned && (reinterpret_cast<unsigned long long>(op[fb]) % (4 * sizeof(O))) == 0;
+  }
+  const nca_i64 nvec = aligned ? (p1 - p0) / 4 : 0;
   for (nca_i64 v = threadIdx.x; v < nvec; v += blockDim.x) {
     const nca_i64 p = p0 + 4 * v;
     const nca_vec<I, 4> m = *reinterpret_cast<const nca_vec<I, 4>*>(map + p);
@@ -235,18 +269,19 @@ __global__ __launch_bounds__(256) void k_bin_backward(const G* __restrict__ g, n
       nca_vec<O, 4> r;
 #pragma unroll
       for (int j = 0; j < 4; ++j) r.v[j] = nca_store_cvt<O, float>::to(bins[j] >= 0 ? gl[fb * nb + bins[j]] : 0.0f);
-      nca_store_nt(reinterpret_cast<nca_vec<O, 4>*>(out + (f0 + fb) * P + p), r);
+      nca_store_nt(reinterpret_cast<nca_vec<O, 4>*>(op[fb] + p), r);
     }
   }
   for (nca_i64 p = p0 + 4 * nvec + threadIdx.x; p < p1; p += blockDim.x) {
     const I bi = map[p];
     for (int fb = 0; fb < nf; ++fb)
-      nca_st(out + (f0 + fb) * P + p, (bi >= 0 && bi < nb) ? gl[fb * nb + static_cast<int>(bi)] : 0.0f);
+      nca_st(op[fb] + p, (bi >= 0 && bi < nb) ? gl[fb * nb + static_cast<int>(bi)] : 0.0f);
   }
 }
 
-// scatter_add with the reference's semantics; runs of equal indices in the 4 elements a
-// thread owns are added with one global atomic.
+// scatter_add with the reference's semantics (scatter_add_kernel, kernels.cuh:660-680): every
+// element i adds src[i] to dest[idx[i]] with one global atomic when 0 <= idx[i] < dest.size.
+// Contiguous operands.
 template <typename D, typename I, typename S>
 __global__ __launch_bounds__(256) void k_scatter_add(D* __restrict__ dest, nca_i64 nd, const I* __restrict__ idx,
                                                      const S* __restrict__ src, nca_i64 n) {
@@ -257,7 +292,35 @@ __global__ __launch_bounds__(256) void k_scatter_add(D* __restrict__ dest, nca_i
   }
 }
 
+// The same over any layouts: idx and src by C-order linear index i, dest by the linear index
+// idx[i] (static_index, array_impl.hh:2101-2179), each through its strides and, for indirect
+// views, its pointer chain.
+template <typename D, typename I, typename S>
+__global__ __launch_bounds__(256) void k_scatter_add_nd(nca_ndview dest, nca_i64 nd, nca_ndview idx, nca_ndview src,
+                                                        nca_i64 n) {
+  const nca_i64 gs = (nca_i64)gridDim.x * blockDim.x;
+  for (nca_i64 i = (nca_i64)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gs) {
+    const I k = *reinterpret_cast<const I*>(nca_nd_addr(idx, i));
+    if (k >= 0 && static_cast<nca_i64>(k) < nd) {
+      D* dp = reinterpret_cast<D*>(const_cast<char*>(nca_nd_addr(dest, static_cast<nca_i64>(k))));
+      atomicAdd(dp, nca_cvt<D, S>::from(*reinterpret_cast<const S*>(nca_nd_addr(src, i))));
+    }
+  }
+}
+
 // ---- host side ---------------------------------------------------------------------------------------------
+static nca_ndview ndview_of(const nca_view& v) {
+  nca_ndview r{};
+  const Chain ch = chain_of(v);
+  r.base = reinterpret_cast<const char*>(ch.base);
+  r.ndim = v.ndim;
+  for (int d = 0; d < v.ndim; ++d) { r.shape[d] = v.shape[d]; r.strides[d] = ch.strides[d]; r.subs[d] = -1; }
+  for (size_t i = 0; i < ch.axes.size(); ++i) r.subs[ch.axes[i]] = ch.subs[i];
+  r.ind = ch.axes.empty() ? 0 : 1;
+  if (r.ndim == 0) { r.ndim = 1; r.shape[0] = 1; r.strides[0] = 0; r.subs[0] = -1; }
+  return r;
+}
+
 // Frames per block of the binning kernels: the largest power of two <= cap whose LDS tables fit
 // in 64 KiB (and the device's per-block limit), at most the number of frames.  Measured on
 // MI355X (256 frames of 1024^2, 1000 bins): the gather backward runs 1.4x faster with 4 frames
@@ -297,7 +360,10 @@ static void dispatch_map_type(int32_t dt, F&& f) {
   }
 }
 
-static int64_t frames_layout(const nca_view& frames, const nca_view& map, int64_t& F) {
+// Frames [F, pixels...]: C-contiguous, or reached through a table of frame pointers (an NC
+// pointer axis 0, or an SO view whose only indirect axis is 0) with each frame's pixels
+// contiguous.  Sets F and the frame addressing; returns the pixels per frame.
+static int64_t frames_layout(const nca_view& frames, const nca_view& map, int64_t& F, FrameAddr& fa) {
   const int64_t P = view_size(map);
   NCA_REQUIRE(frames.ndim >= 1, "frames must have a frame axis");
   F = frames.shape[0];
@@ -305,7 +371,22 @@ static int64_t frames_layout(const nca_view& frames, const nca_view& map, int64_
   for (int d = 1; d < frames.ndim; ++d) pf *= frames.shape[d];
   if (frames.ndim == 1) { F = 1; pf = frames.shape[0]; }
   NCA_REQUIRE(pf == P, "each frame must have as many pixels as the bin map");
-  NCA_REQUIRE(is_c_contiguous(frames) && is_c_contiguous(map), "frames and bin map must be C-contiguous");
+  NCA_REQUIRE(is_c_contiguous(map), "the bin map must be C-contiguous");
+  const int64_t item = itemsize(frames.dtype);
+  if (view_indirect(frames)) {
+    const Chain ch = chain_of(frames);
+    NCA_REQUIRE(frames.ndim >= 2 && ch.axes.size() == 1 && ch.axes[0] == 0,
+                "frames reached through pointers must have their pointers on the frame axis only");
+    int64_t expect = item;
+    for (int d = frames.ndim - 1; d >= 1; --d) {
+      NCA_REQUIRE(frames.shape[d] == 1 || ch.strides[d] == expect, "the pixels of each frame must be contiguous");
+      expect *= frames.shape[d];
+    }
+    fa = FrameAddr{reinterpret_cast<const char*>(ch.base), ch.strides[0], ch.subs[0], 1};
+  } else {
+    NCA_REQUIRE(is_c_contiguous(frames), "frames must be C-contiguous (or a table of contiguous frames)");
+    fa = FrameAddr{static_cast<const char*>(frames.data), P * item, 0, 0};
+  }
   return P;
 }
 
@@ -338,10 +419,11 @@ int nca_hip_bin(const nca_view* frames, const nca_view* binmap, int32_t n_bins, 
     NCA_REQUIRE(counts || sums, "nothing to compute");
     NCA_REQUIRE(n_bins > 0, "n_bins must be positive");
     NCA_REQUIRE(count_pred == NCA_COUNT_ALL || count_pred == NCA_COUNT_NONZERO, "bad count predicate");
-    validate_view(*frames, "frames");
+    validate_view_nc(*frames, "frames");
     validate_view(*binmap, "binmap");
     int64_t F = 0;
-    const int64_t P = frames_layout(*frames, *binmap, F);
+    FrameAddr fa{};
+    const int64_t P = frames_layout(*frames, *binmap, F, fa);
     int64_t cst0 = 0, cst1 = 0, sst0 = 0, sst1 = 0;
     out2d(counts, F, n_bins, "counts", cst0, cst1);
     out2d(sums, F, n_bins, "sums", sst0, sst1);
@@ -389,8 +471,9 @@ int nca_hip_bin(const nca_view* frames, const nca_view* binmap, int32_t n_bins, 
       typedef std::remove_pointer_t<decltype(tp)> T;
       dispatch_map_type(binmap->dtype, [&](auto* ip) {
         typedef std::remove_pointer_t<decltype(ip)> I;
-        NCA_REQUIRE(reinterpret_cast<uintptr_t>(frames->data) % (4 * sizeof(T)) == 0 &&
-                        reinterpret_cast<uintptr_t>(binmap->data) % (4 * sizeof(I)) == 0 && (P % 4 == 0 || F == 1),
+        NCA_REQUIRE((fa.ind || reinterpret_cast<uintptr_t>(frames->data) % (4 * sizeof(T)) == 0) &&
+                        reinterpret_cast<uintptr_t>(binmap->data) % (4 * sizeof(I)) == 0 &&
+                        (P % 4 == 0 || F == 1 || fa.ind),
                     "binning needs 4-element aligned frames and map (and pixels per frame % 4 == 0)");
         if (!work) return;
         const dim3 grid(static_cast<unsigned>(chunks), static_cast<unsigned>(groups));
@@ -399,7 +482,7 @@ int nca_hip_bin(const nca_view* frames, const nca_view* binmap, int32_t n_bins, 
           constexpr bool SC = decltype(sc)::value;
           constexpr int CC = decltype(cc)::value;
           ProfScope ps("nca_bin", s);
-          hipLaunchKernelGGL((k_bin<T, I, FBC, SC, CC>), grid, dim3(1024), lds, s, (const T*)frames->data,
+          hipLaunchKernelGGL((k_bin<T, I, FBC, SC, CC>), grid, dim3(1024), lds, s, fa,
                              (const I*)binmap->data, (nca_i64)P, n_bins, (nca_i64)chunk, (nca_i64)F, S, (nca_i64)slice, R,
                              psum, pzero, pvalid);
         };
@@ -452,9 +535,10 @@ int nca_hip_bin_backward(const nca_view* grad_sums, const nca_view* binmap, cons
     NCA_REQUIRE(grad_sums && binmap && grad_frames, "null argument");
     validate_view(*grad_sums, "grad_sums");
     validate_view(*binmap, "binmap");
-    validate_view(*grad_frames, "grad_frames");
+    validate_view_nc(*grad_frames, "grad_frames");
     int64_t F = 0;
-    const int64_t P = frames_layout(*grad_frames, *binmap, F);
+    FrameAddr oa{};
+    const int64_t P = frames_layout(*grad_frames, *binmap, F, oa);
     NCA_REQUIRE(grad_sums->ndim >= 1, "grad_sums must be [frames, n_bins]");
     const int nb = static_cast<int>(grad_sums->shape[grad_sums->ndim - 1]);
     int64_t gst0 = 0, gst1 = 0;
@@ -476,15 +560,16 @@ int nca_hip_bin_backward(const nca_view* grad_sums, const nca_view* binmap, cons
       typedef std::remove_pointer_t<decltype(gp)> G;
       typedef std::remove_pointer_t<decltype(ip)> I;
       typedef std::remove_pointer_t<decltype(op)> O;
-      NCA_REQUIRE(reinterpret_cast<uintptr_t>(grad_frames->data) % (4 * sizeof(O)) == 0 &&
-                      reinterpret_cast<uintptr_t>(binmap->data) % (4 * sizeof(I)) == 0 && (P % 4 == 0 || F == 1),
+      NCA_REQUIRE((oa.ind || reinterpret_cast<uintptr_t>(grad_frames->data) % (4 * sizeof(O)) == 0) &&
+                      reinterpret_cast<uintptr_t>(binmap->data) % (4 * sizeof(I)) == 0 &&
+                      (P % 4 == 0 || F == 1 || oa.ind),
                   "bin backward needs 4-element aligned outputs and map");
       const dim3 grid(static_cast<unsigned>(chunks), static_cast<unsigned>(groups));
       auto go = [&](auto fbc) {
         constexpr int FBC = decltype(fbc)::value;
         ProfScope ps("nca_bin_backward", s);
         hipLaunchKernelGGL((k_bin_backward<G, I, O, FBC>), grid, dim3(256), lds, s, (const G*)grad_sums->data,
-                           (nca_i64)gst0, (nca_i64)gst1, (const I*)binmap->data, (nca_i64)P, nb, (O*)grad_frames->data,
+                           (nca_i64)gst0, (nca_i64)gst1, (const I*)binmap->data, (nca_i64)P, nb, oa,
                            (nca_i64)chunk, (nca_i64)F);
       };
       if (fb == 8) go(std::integral_constant<int, 8>{});
@@ -517,13 +602,12 @@ int nca_hip_bin_backward(const nca_view* grad_sums, const nca_view* binmap, cons
 int nca_hip_scatter_add(const nca_view* dest, const nca_view* idx, const nca_view* src, void* stream) {
   return api_call([&] {
     NCA_REQUIRE(dest && idx && src, "null argument");
-    validate_view(*dest, "dest");
-    validate_view(*idx, "idx");
-    validate_view(*src, "src");
-    NCA_REQUIRE(dest->ndim == 1 && is_c_contiguous(*dest), "dest must be 1-D contiguous");
-    NCA_REQUIRE(is_c_contiguous(*idx) && is_c_contiguous(*src) && view_size(*idx) == view_size(*src),
-                "idx and src must be contiguous with the same size");
-    const int64_t n = view_size(*src), nd = dest->shape[0];
+    validate_view_nc(*dest, "dest");
+    validate_view_nc(*idx, "idx");
+    validate_view_nc(*src, "src");
+    NCA_REQUIRE(view_size(*idx) == view_size(*src), "idx and src must have the same size");
+    const bool contig = dest->ndim == 1 && is_c_contiguous(*dest) && is_c_contiguous(*idx) && is_c_contiguous(*src);
+    const int64_t n = view_size(*src), nd = view_size(*dest);
     hipStream_t s = as_stream(stream);
     DeviceState& dev = current_device();
     if (n == 0) return;
@@ -533,8 +617,12 @@ int nca_hip_scatter_add(const nca_view* dest, const nca_view* idx, const nca_vie
       typedef std::remove_pointer_t<decltype(ip)> I;
       typedef std::remove_pointer_t<decltype(sp)> S;
       ProfScope ps("nca_scatter_add", s);
-      hipLaunchKernelGGL((k_scatter_add<D, I, S>), dim3(blocks), dim3(256), 0, s, (D*)dest->data, (nca_i64)nd,
-                         (const I*)idx->data, (const S*)src->data, (nca_i64)n);
+      if (contig)
+        hipLaunchKernelGGL((k_scatter_add<D, I, S>), dim3(blocks), dim3(256), 0, s, (D*)dest->data, (nca_i64)nd,
+                           (const I*)idx->data, (const S*)src->data, (nca_i64)n);
+      else
+        hipLaunchKernelGGL((k_scatter_add_nd<D, I, S>), dim3(blocks), dim3(256), 0, s, ndview_of(*dest), (nca_i64)nd,
+                           ndview_of(*idx), ndview_of(*src), (nca_i64)n);
     };
     auto with_src = [&](auto* dp, auto* ip) {
       switch (src->dtype) {

@@ -84,26 +84,6 @@ ScalarSlots make_scalar_slots(const Tape& t, const nca_scalar* s, int n) {
 }
 
 // ---- operand analysis ----------------------------------------------------------------------
-Chain chain_of(const nca_view& v) {
-  Chain c;
-  c.base = reinterpret_cast<uintptr_t>(v.data);
-  for (int d = 0; d < v.ndim; ++d) c.strides[d] = v.strides[d];
-  if (v.pointer_axis >= 0) {
-    const int P = v.pointer_axis;
-    c.base += static_cast<uintptr_t>(v.ptr_offset * static_cast<int64_t>(sizeof(void*)));
-    c.strides[P] = v.strides[P] * static_cast<int64_t>(sizeof(void*));
-    c.axes.push_back(P);
-    c.subs.push_back(v.row_offset);
-  } else if (v.layout == NCA_LAYOUT_SO) {
-    for (int d = 0; d < v.ndim; ++d)
-      if (v.suboffsets[d] >= 0) {
-        c.axes.push_back(d);
-        c.subs.push_back(v.suboffsets[d]);
-      }
-  }
-  return c;
-}
-
 static bool views_alias(const nca_view& a, const nca_view& b) {
   if (a.dtype != b.dtype || a.ndim != b.ndim) return false;
   const Chain ca = chain_of(a), cb = chain_of(b);

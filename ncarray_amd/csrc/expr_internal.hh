@@ -21,19 +21,6 @@ struct ExprProgram {
 ExprProgram& expr_program(uint64_t h);
 uint64_t expr_register(Tape&& t);
 
-// The dereference chain of a view: start address, byte strides, indirect axes and the byte
-// offset added after each pointer read.  An NC pointer axis P (NCOffsetsPolicy::advance,
-// layout.hh:468-480) is one step: the start moves by ptr_offset table entries, strides[P]
-// (table entries) becomes bytes and row_offset is the offset after the read.  An SO view
-// (SOArrayPolicy::advance, layout.hh:677-700) has a step at each axis with suboffset >= 0.
-struct Chain {
-  uintptr_t base = 0;
-  std::array<int64_t, NCA_MAX_NDIM> strides{};
-  std::vector<int> axes;
-  std::vector<int64_t> subs;
-};
-Chain chain_of(const nca_view& v);
-
 // Scalars converted to the compute type, one 8-byte slot each (kernel parameters).
 struct ScalarSlots {
   std::vector<uint64_t> slots;

@@ -76,6 +76,31 @@ __device__ __forceinline__ char* nca_chase(char* p, nca_i64 sub) {
   return *reinterpret_cast<char* const*>(p) + sub;
 }
 
+// A view addressed by C-order linear index: strides in bytes and, for indirect views, a pointer
+// read after the step of each axis whose sub is >= 0 (SOArrayPolicy::advance, layout.hh:677-700;
+// an NC pointer axis is one such step).  Up to 10 axes (NCARRAY_MAX_NDIM).
+struct nca_ndview {
+  const char* base;
+  nca_i64 shape[10];
+  nca_i64 strides[10];
+  nca_i64 subs[10];
+  int ndim;
+  int ind;
+};
+__device__ __forceinline__ const char* nca_nd_addr(const nca_ndview& v, nca_i64 lin) {
+  nca_i64 c[10];
+  for (int k = v.ndim - 1; k >= 0; --k) {
+    c[k] = lin % v.shape[k];
+    lin /= v.shape[k];
+  }
+  const char* p = v.base;
+  for (int k = 0; k < v.ndim; ++k) {
+    p += c[k] * v.strides[k];
+    if (v.ind && v.subs[k] >= 0) p = nca_chase(p, v.subs[k]);
+  }
+  return p;
+}
+
 // ---- numeric limits ------------------------------------------------------------------
 template <typename T> struct nca_lim;
 template <> struct nca_lim<bool> { __device__ static bool lowest() { return false; } __device__ static bool max() { return true; } };

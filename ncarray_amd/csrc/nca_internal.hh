@@ -120,6 +120,18 @@ void validate_view_nc(const nca_view& v, const char* what, bool allow_null_data 
 // True when reaching an element reads a stored pointer: an NC pointer axis, or an SO axis
 // whose suboffset is >= 0.
 bool view_indirect(const nca_view& v);
+// The dereference chain of a view: start address, byte strides, indirect axes and the byte
+// offset added after each pointer read.  An NC pointer axis P (NCOffsetsPolicy::advance,
+// layout.hh:468-480) is one step: the start moves by ptr_offset table entries, strides[P]
+// (table entries) becomes bytes and row_offset is the offset after the read.  An SO view
+// (SOArrayPolicy::advance, layout.hh:677-700) has a step at each axis with suboffset >= 0.
+struct Chain {
+  uintptr_t base = 0;
+  std::array<int64_t, NCA_MAX_NDIM> strides{};
+  std::vector<int> axes;
+  std::vector<int64_t> subs;
+};
+Chain chain_of(const nca_view& v);
 bool is_c_contiguous(const nca_view& v);
 bool same_shape(const nca_view& a, const nca_view& b);
 // True when every stride is zero (a broadcast scalar).

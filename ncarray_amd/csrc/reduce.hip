@@ -251,12 +251,29 @@ __device__ void combine_slots_and_store(const Slot* slots, int n, nca_i64 count,
 // ---- full reduction kernel ---------------------------------------------------------------------------
 struct NdDesc {
   nca_i64 shape[NCA_MAX_NDIM];
-  nca_i64 strides[NCA_MAX_NDIM];  // bytes
+  nca_i64 strides[NCA_MAX_NDIM];  // bytes (the dereference chain's strides for an indirect view)
+  nca_i64 subs[NCA_MAX_NDIM];     // indirect views: >= 0 after an axis whose step ends in a pointer read
   int ndim;
+  int ind;                        // an indirect view (NC pointer axis or SO suboffsets)
 };
 
+// Address of the element with C-order linear index idx.  Indirect views walk their chain
+// (SOArrayPolicy::advance, layout.hh:677-700; an NC pointer axis is one such step).
 template <int DT>
 __device__ __forceinline__ const typename RT<DT>::T* nd_ptr(const char* base, const NdDesc& d, nca_i64 idx) {
+  if (d.ind) {
+    nca_i64 c[NCA_MAX_NDIM];
+    for (int k = d.ndim - 1; k >= 0; --k) {
+      c[k] = idx % d.shape[k];
+      idx /= d.shape[k];
+    }
+    const char* p = base;
+    for (int k = 0; k < d.ndim; ++k) {
+      p += c[k] * d.strides[k];
+      if (d.subs[k] >= 0) p = nca_chase(p, d.subs[k]);
+    }
+    return reinterpret_cast<const typename RT<DT>::T*>(p);
+  }
   nca_i64 off = 0;
   for (int k = d.ndim - 1; k >= 0; --k) {
     nca_i64 c = idx % d.shape[k];
@@ -479,6 +496,9 @@ struct AxesDesc {
   int nk, nr;
 };
 
+template <int DT, int OP>
+__device__ void store_axis_result(const Slot& sl, char* po, nca_i64 nred);
+
 template <int DT, int OP, bool COMP>
 __global__ __launch_bounds__(256) void k_reduce_generic(const char* __restrict__ base, AxesDesc d, nca_i64 nout,
                                                         nca_i64 nred, char* __restrict__ out) {
@@ -508,7 +528,14 @@ __global__ __launch_bounds__(256) void k_reduce_generic(const char* __restrict__
   }
   Slot sl;
   a.to_slot(sl);
-  char* po = out + ooff;
+  store_axis_result<DT, OP>(sl, out + ooff, nred);
+}
+
+// Store one axis-reduction result (a folded accumulator in slot form) at po.
+template <int DT, int OP>
+__device__ void store_axis_result(const Slot& sl, char* po, nca_i64 nred) {
+  typedef typename RT<DT>::T T;
+  typedef typename RT<DT>::V V;
   if constexpr (OP == NCA_REDUCE_SUM) {
     if constexpr (RT<DT>::fp) *reinterpret_cast<typename RT<DT>::SO*>(po) = static_cast<typename RT<DT>::SO>(sl.a.d);
     else *reinterpret_cast<typename RT<DT>::SO*>(po) = static_cast<typename RT<DT>::SO>(static_cast<typename RT<DT>::S>(sl.a.u));
@@ -523,6 +550,76 @@ __global__ __launch_bounds__(256) void k_reduce_generic(const char* __restrict__
     *reinterpret_cast<long long*>(po) = sl.idx < 0 ? 0 : sl.idx;
   } else {
     *reinterpret_cast<bool*>(po) = sl.a.i != 0;
+  }
+}
+
+// Axis reductions of an indirect view (NC pointer axis or SO suboffsets): the element at full
+// coordinate c is reached through the view's dereference chain.  G threads per output (1, or a
+// block of 256 for few, long reductions); each folds reduced elements r = t, t + G, ... in C
+// order, so a fold meets the linear input indices in increasing order (argmax/argmin keep the
+// first occurrence), then the block combines its threads' accumulators.
+struct ChainAxesDesc {
+  nca_i64 shape[NCA_MAX_NDIM];
+  nca_i64 strides[NCA_MAX_NDIM];  // chain byte strides
+  nca_i64 subs[NCA_MAX_NDIM];     // >= 0: pointer read after this axis's step
+  nca_i64 lin[NCA_MAX_NDIM];      // C-order linear stride of each axis
+  nca_i64 ostride[NCA_MAX_NDIM];  // output byte stride of each kept axis
+  int red[NCA_MAX_NDIM];          // 1: reduced axis
+  int ndim;
+};
+
+template <int DT, int OP, bool COMP, int G>
+__global__ __launch_bounds__(256) void k_reduce_axes_chain(const char* __restrict__ base, ChainAxesDesc d, nca_i64 nout,
+                                                           nca_i64 nred, char* __restrict__ out) {
+  typedef typename RT<DT>::T T;
+  typedef typename RT<DT>::V V;
+  const nca_i64 o = G == 1 ? (nca_i64)blockIdx.x * blockDim.x + threadIdx.x : (nca_i64)blockIdx.x;
+  const int t = G == 1 ? 0 : static_cast<int>(threadIdx.x);
+  if (o >= nout) return;  // G == 1: whole threads; G == 256: whole blocks
+  nca_i64 kc[NCA_MAX_NDIM];
+  nca_i64 ooff = 0;
+  {
+    nca_i64 rem = o;
+    for (int k = d.ndim - 1; k >= 0; --k) {
+      if (d.red[k]) continue;
+      kc[k] = rem % d.shape[k];
+      rem /= d.shape[k];
+      ooff += kc[k] * d.ostride[k];
+    }
+  }
+  Acc<DT, OP, COMP> a;
+  a.init();
+  for (nca_i64 r = t; r < nred; r += G) {
+    nca_i64 c[NCA_MAX_NDIM];
+    nca_i64 rem = r;
+    for (int k = d.ndim - 1; k >= 0; --k) {
+      if (d.red[k]) {
+        c[k] = rem % d.shape[k];
+        rem /= d.shape[k];
+      } else {
+        c[k] = kc[k];
+      }
+    }
+    const char* p = base;
+    nca_i64 lin = 0;
+    for (int k = 0; k < d.ndim; ++k) {
+      p += c[k] * d.strides[k];
+      if (d.subs[k] >= 0) p = nca_chase(p, d.subs[k]);
+      lin += c[k] * d.lin[k];
+    }
+    a.fold(nca_ld<V>(reinterpret_cast<const T*>(p)), lin);
+  }
+  Slot sl;
+  if constexpr (G == 1) {
+    a.to_slot(sl);
+    store_axis_result<DT, OP>(sl, out + ooff, nred);
+  } else {
+    __shared__ Acc<DT, OP, COMP> lds[16];
+    block_combine(a, lds);
+    if (threadIdx.x == 0) {
+      a.to_slot(sl);
+      store_axis_result<DT, OP>(sl, out + ooff, nred);
+    }
   }
 }
 
@@ -675,12 +772,24 @@ template <int DT> struct RunFull {
     typedef typename RT<DT>::T T;
     DeviceState& dev = current_device();
     NdDesc d{};
-    int64_t cs[NCA_MAX_NDIM], cst[NCA_MAX_NDIM];
-    d.ndim = coalesce(a.in->shape, a.in->strides, a.in->ndim, cs, cst);
-    for (int k = 0; k < d.ndim; ++k) { d.shape[k] = cs[k]; d.strides[k] = cst[k]; }
     const int64_t n = view_size(*a.in);
-    bool contig = (d.ndim == 0) || (d.ndim == 1 && d.strides[0] == (int64_t)sizeof(T));
-    if (d.ndim == 0) { d.ndim = 1; d.shape[0] = n; d.strides[0] = sizeof(T); }
+    const char* base = static_cast<const char*>(a.in->data);
+    bool contig = false;
+    if (view_indirect(*a.in)) {
+      // the view's dereference chain, axis by axis (no axes merged across a pointer read)
+      const Chain ch = chain_of(*a.in);
+      d.ind = 1;
+      d.ndim = a.in->ndim;
+      for (int k = 0; k < d.ndim; ++k) { d.shape[k] = a.in->shape[k]; d.strides[k] = ch.strides[k]; d.subs[k] = -1; }
+      for (size_t i = 0; i < ch.axes.size(); ++i) d.subs[ch.axes[i]] = ch.subs[i];
+      base = reinterpret_cast<const char*>(ch.base);
+    } else {
+      int64_t cs[NCA_MAX_NDIM], cst[NCA_MAX_NDIM];
+      d.ndim = coalesce(a.in->shape, a.in->strides, a.in->ndim, cs, cst);
+      for (int k = 0; k < d.ndim; ++k) { d.shape[k] = cs[k]; d.strides[k] = cst[k]; }
+      contig = (d.ndim == 0) || (d.ndim == 1 && d.strides[0] == (int64_t)sizeof(T));
+      if (d.ndim == 0) { d.ndim = 1; d.shape[0] = n; d.strides[0] = sizeof(T); }
+    }
     contig = contig && (reinterpret_cast<uintptr_t>(a.in->data) % 16 == 0);
     constexpr int E = (16 / sizeof(T)) > 0 ? (16 / sizeof(T)) : 1;
     int64_t items = contig ? std::max<int64_t>(1, n / E / 2) : std::max<int64_t>(1, n);
@@ -695,10 +804,10 @@ template <int DT> struct RunFull {
           ProfScope ps("nca_reduce_full", a.s);
           if (contig)
             hipLaunchKernelGGL((k_reduce_full<DT, OP, COMP, true>), dim3(blocks), dim3(256), 0, a.s,
-                               (const char*)a.in->data, d, (nca_i64)n, (Slot*)w.partials, w.counters, a.out);
+                               base, d, (nca_i64)n, (Slot*)w.partials, w.counters, a.out);
           else
             hipLaunchKernelGGL((k_reduce_full<DT, OP, COMP, false>), dim3(blocks), dim3(256), 0, a.s,
-                               (const char*)a.in->data, d, (nca_i64)n, (Slot*)w.partials, w.counters, a.out);
+                               base, d, (nca_i64)n, (Slot*)w.partials, w.counters, a.out);
         }
         ProfScope ps2("nca_reduce_full_combine", a.s);
         hipLaunchKernelGGL((k_reduce_full_combine<DT, OP>), dim3(1), dim3(256), 0, a.s, (const Slot*)w.partials,
@@ -717,6 +826,44 @@ struct AxesArgs {
 };
 
 template <int DT> struct RunAxes {
+  // an indirect input: k_reduce_axes_chain, a block per output when outputs are few
+  static void run_chain(const AxesArgs& a, int64_t nout, int64_t nred) {
+    DeviceState& dev = current_device();
+    const nca_view& in = *a.in;
+    const nca_view& out = *a.out;
+    const Chain ch = chain_of(in);
+    ChainAxesDesc d{};
+    d.ndim = in.ndim;
+    int64_t lin = 1;
+    for (int k = in.ndim - 1; k >= 0; --k) { d.lin[k] = lin; lin *= in.shape[k]; }
+    int oi = 0;
+    for (int k = 0; k < in.ndim; ++k) {
+      d.shape[k] = in.shape[k];
+      d.strides[k] = ch.strides[k];
+      d.subs[k] = -1;
+      d.red[k] = a.reduced[k] ? 1 : 0;
+      d.ostride[k] = a.reduced[k] ? 0 : out.strides[oi++];
+    }
+    for (size_t i = 0; i < ch.axes.size(); ++i) d.subs[ch.axes[i]] = ch.subs[i];
+    const char* base = reinterpret_cast<const char*>(ch.base);
+    dispatch_op(a.op, [&](auto opc) {
+      constexpr int OP = decltype(opc)::value;
+      with_comp<DT, OP>(a.mode, [&](auto cc) {
+        constexpr bool COMP = decltype(cc)::value;
+        if (nout == 0) return;
+        ProfScope ps("nca_reduce_axes", a.s);
+        if (nout < static_cast<int64_t>(dev.n_cus) * 256 && nred >= 256) {
+          NCA_REQUIRE(nout < (int64_t(1) << 31), "too many outputs");
+          hipLaunchKernelGGL((k_reduce_axes_chain<DT, OP, COMP, 256>), dim3(static_cast<unsigned>(nout)), dim3(256), 0, a.s,
+                             base, d, (nca_i64)nout, (nca_i64)nred, (char*)out.data);
+        } else {
+          hipLaunchKernelGGL((k_reduce_axes_chain<DT, OP, COMP, 1>), dim3(static_cast<unsigned>((nout + 255) / 256)),
+                             dim3(256), 0, a.s, base, d, (nca_i64)nout, (nca_i64)nred, (char*)out.data);
+        }
+      });
+    });
+  }
+
   static void run(const AxesArgs& a) {
     typedef typename RT<DT>::T T;
     DeviceState& dev = current_device();
@@ -725,6 +872,10 @@ template <int DT> struct RunAxes {
     const int nd = in.ndim;
     int64_t nout = 1, nred = 1;
     for (int d = 0; d < nd; ++d) (a.reduced[d] ? nred : nout) *= in.shape[d];
+    if (view_indirect(in)) {
+      run_chain(a, nout, nred);
+      return;
+    }
     // Pattern [KO][R][KI] on a contiguous input and output?
     bool fast = is_c_contiguous(in) && is_c_contiguous(out);
     int first_red = -1, last_red = -1;
@@ -841,7 +992,7 @@ extern "C" {
 int nca_hip_reduce_full(const nca_view* in, int32_t op, int32_t accum_mode, void* out_dev, void* stream) {
   return api_call([&] {
     NCA_REQUIRE(in != nullptr && out_dev != nullptr, "null argument");
-    validate_view(*in, "in");
+    validate_view_nc(*in, "in");
     (void)reduce_result_dtype(op, in->dtype);
     FullArgs a{in, op, accum_mode, out_dev, as_stream(stream)};
     dispatch_dtype<RunFull>(in->dtype, a);
@@ -855,7 +1006,7 @@ int nca_hip_reduce_axes(const nca_view* in, const int64_t* axes, int32_t n_axes,
   return api_call([&] {
     NCA_REQUIRE(in != nullptr && out != nullptr, "null argument");
     NCA_REQUIRE(n_axes == 0 || axes != nullptr, "null axes");
-    validate_view(*in, "in");
+    validate_view_nc(*in, "in");
     validate_view(*out, "out");
     AxesArgs a{in, out, std::vector<char>(in->ndim, 0), op, accum_mode, as_stream(stream)};
     for (int i = 0; i < n_axes; ++i)

@@ -222,6 +222,26 @@ void validate_view(const nca_view& v, const char* what, bool allow_null_data) {
     NCA_REQUIRE(v.data != nullptr, std::string(what) + ": null data pointer");
 }
 
+Chain chain_of(const nca_view& v) {
+  Chain c;
+  c.base = reinterpret_cast<uintptr_t>(v.data);
+  for (int d = 0; d < v.ndim; ++d) c.strides[d] = v.strides[d];
+  if (v.pointer_axis >= 0) {
+    const int P = v.pointer_axis;
+    c.base += static_cast<uintptr_t>(v.ptr_offset * static_cast<int64_t>(sizeof(void*)));
+    c.strides[P] = v.strides[P] * static_cast<int64_t>(sizeof(void*));
+    c.axes.push_back(P);
+    c.subs.push_back(v.row_offset);
+  } else if (v.layout == NCA_LAYOUT_SO) {
+    for (int d = 0; d < v.ndim; ++d)
+      if (v.suboffsets[d] >= 0) {
+        c.axes.push_back(d);
+        c.subs.push_back(v.suboffsets[d]);
+      }
+  }
+  return c;
+}
+
 bool is_c_contiguous(const nca_view& v) {
   if (view_indirect(v)) return false;
   int64_t expect = itemsize(v.dtype);

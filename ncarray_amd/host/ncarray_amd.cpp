@@ -801,12 +801,11 @@ Scalar NCDevArray::item() const {
 
 Scalar NCDevArray::full(ReduceOp op, Accum mode) const {
   if (pax_ >= 0) {
-    // sum/max/min read the rows in place through the fused expression reduction; the
-    // others reduce a staged dense copy
+    // sum/max/min read the rows through the fused expression reduction (one pointer read per
+    // row); the other reductions follow the row pointers element by element
     if (op == ReduceOp::sum) return Expr(*this).sum(mode);
     if (op == ReduceOp::max) return Expr(*this).max();
     if (op == ReduceOp::min) return Expr(*this).min();
-    return HIPEngine::execute_full_reduce(dense(), op, mode);
   }
   return HIPEngine::execute_full_reduce(*this, op, mode);
 }
@@ -816,7 +815,7 @@ NCDevArray NCDevArray::axes_reduce(ReduceOp op, const std::vector<int64_t>& axes
   for (int d = 0; d < nd_; ++d)
     if (std::find(axes.begin(), axes.end(), static_cast<int64_t>(d)) == axes.end()) kept.push_back(sh_[d]);
   NCDevArray r(kept, reduce_result_dtype(op, dt_));
-  HIPEngine::execute_reduce_axes(dense(), axes, op, r, mode);
+  HIPEngine::execute_reduce_axes(*this, axes, op, r, mode);
   return r;
 }
 
@@ -958,11 +957,9 @@ void HIPEngine::execute_copy_into(const NCDevArray& src, void* host, DType host_
 
 void HIPEngine::execute_scatter_add(const NCDevArray& dest, const NCDevArray& idx, const NCDevArray& src, void* stream) {
   ensure_init();
-  // pointer-axis operands go through staged dense copies
-  const NCDevArray dd = dest.dense(), id = idx.dense(), sd = src.dense();
-  const nca_view d = dd.cview(), i = id.cview(), s = sd.cview();
+  // pointer-axis operands are read and written in place (the kernel follows their row pointers)
+  const nca_view d = dest.cview(), i = idx.cview(), s = src.cview();
   check(nca_hip_scatter_add(&d, &i, &s, stream), "scatter_add");
-  if (dest.pointer_axis() >= 0) execute_assign(dest, dd, stream);
 }
 
 // ---- stencils ------------------------------------------------------------------------------

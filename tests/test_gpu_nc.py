@@ -142,15 +142,25 @@ def test_capi_bf16_rows():
     assert torch.equal(out, dense)
 
 
-def test_capi_other_entry_points_reject_pointer_views():
-    rows = [torch.zeros(8, device="cuda") for _ in range(2)]
+def test_capi_pointer_views_in_reductions_and_binning_layouts():
+    """Reductions read pointer views in place; binning takes frames behind pointers only when the
+    pointer axis is the frame axis (a pointer per row inside a frame is refused, not misread)."""
+    rows = [torch.full((8,), float(r + 1), device="cuda") for r in range(2)]
     tbl = _table(rows)
     v = capi.make_nc_view(tbl.data_ptr(), capi.FLOAT32, (2, 8), [1, 4], 0)
-    slot = torch.empty((), dtype=torch.float64, device="cuda")
+    slot = torch.empty((), dtype=torch.float32, device="cuda")
+    capi.call("nca_hip_reduce_full", ctypes.byref(v), capi.REDUCE_SUM, capi.ACCUM_PLAIN,
+              ctypes.c_void_p(slot.data_ptr()), None)
+    assert slot.item() == 24.0
+    frows = [torch.ones(16, device="cuda") for _ in range(4)]
+    ftbl = _table(frows)
+    fv = capi.make_nc_view(ftbl.data_ptr(), capi.FLOAT32, (1, 4, 16), [0, 1, 4], 1)   # rows of one frame
+    bm = torch.zeros(64, dtype=torch.int32, device="cuda")
+    counts = torch.empty((1, 2), dtype=torch.int32, device="cuda")
     with pytest.raises(capi.NcaError) as e:
-        capi.call("nca_hip_reduce_full", ctypes.byref(v), capi.REDUCE_SUM, capi.ACCUM_PLAIN,
-                  ctypes.c_void_p(slot.data_ptr()), None)
-    assert e.value.code == capi.ERR_UNSUPPORTED
+        capi.call("nca_hip_bin", ctypes.byref(fv), ctypes.byref(ti.view(bm)), 2, capi.COUNT_ALL,
+                  ctypes.byref(ti.view(counts)), None, None)
+    assert e.value.code == capi.ERR_INVALID
 
 
 # ---- Python NCDevArrayRef ------------------------------------------------------------------------
