@@ -223,6 +223,14 @@ int nca_hip_synchronize(void* stream);
 int nca_hip_profile_enable(int32_t on);
 int nca_hip_profile_read(char* buf, size_t n);
 
+/* Counters of the kernel cache since the process started: sources compiled with hipRTC,
+ * code objects loaded from the on-disk cache, and lookups served from memory.  The disk cache
+ * (NCA_HIP_CACHE_DIR, else $XDG_CACHE_HOME/ncarray_amd, else ~/.cache/ncarray_amd) stores
+ * each code object under the SHA-256 of its source and compiler options with that key, the
+ * code's length and its SHA-256 in a header; a truncated, damaged or foreign file is compiled
+ * again and rewritten (the reference's cubin cache, rtcompiler.cc:74-86). */
+int nca_hip_jit_stats(int64_t* compiles, int64_t* disk_loads, int64_t* memory_hits);
+
 /* Compile a complete generated source (as returned by nca_hip_expr_source or
  * nca_hip_stencil_source) with hipRTC for `arch` (NULL: gfx950) without loading it.
  * Needs no GPU.  On failure the compiler log is copied into log. */

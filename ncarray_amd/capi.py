@@ -112,6 +112,7 @@ _SIGS = {
     "nca_hip_memcpy_d2d": [_P, _P, _SZ, _P],
     "nca_hip_synchronize": [_P],
     "nca_hip_rtc_compile": [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p, _SZ],
+    "nca_hip_jit_stats": [ctypes.POINTER(_I64), ctypes.POINTER(_I64), ctypes.POINTER(_I64)],
     "nca_hip_profile_enable": [_I32],
     "nca_hip_profile_read": [ctypes.c_char_p, _SZ],
     "nca_hip_reduce_result_dtype": [_I32, _I32, ctypes.POINTER(_I32)],
@@ -452,6 +453,13 @@ def init(device=0):
     if not _initialised.get(device):
         call("nca_hip_init", device)
         _initialised[device] = True
+
+
+def jit_stats():
+    """(compiles, disk_loads, memory_hits) of the kernel cache in this process."""
+    c, d, m = _I64(), _I64(), _I64()
+    call("nca_hip_jit_stats", ctypes.byref(c), ctypes.byref(d), ctypes.byref(m))
+    return c.value, d.value, m.value
 
 
 def profile_enable(on=True):

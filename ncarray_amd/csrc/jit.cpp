@@ -13,6 +13,7 @@
 
 #include <cstdio>
 #include <cstdlib>
+#include <atomic>
 #include <fstream>
 #include <sstream>
 #include <unordered_map>
@@ -119,15 +120,20 @@ static std::string cache_dir() {
   return dir;
 }
 
-static const char kMagic[8] = {'N', 'C', 'A', 'H', 'S', 'C', 'O', '1'};
+// A cached code object: magic, the SHA-256 key of the source it was compiled from (so a file
+// copied in under another name is refused), the code length, the SHA-256 of the code (a
+// truncated or damaged file is refused), the code.
+static const char kMagic[8] = {'N', 'C', 'A', 'H', 'S', 'C', 'O', '2'};
 
-static bool read_cached(const std::string& path, std::string& code) {
+static bool read_cached(const std::string& path, const std::string& key, std::string& code) {
   std::ifstream f(path, std::ios::binary);
   if (!f) return false;
   char magic[8];
+  char fkey[64];
   uint64_t len = 0;
   char digest[64];
   if (!f.read(magic, 8) || std::memcmp(magic, kMagic, 8) != 0) return false;
+  if (!f.read(fkey, 64) || key.size() != 64 || std::memcmp(fkey, key.data(), 64) != 0) return false;
   if (!f.read(reinterpret_cast<char*>(&len), 8) || len == 0 || len > (uint64_t(1) << 30)) return false;
   if (!f.read(digest, 64)) return false;
   code.resize(len);
@@ -135,7 +141,7 @@ static bool read_cached(const std::string& path, std::string& code) {
   return sha256_hex(code) == std::string(digest, 64);
 }
 
-static void write_cached(const std::string& path, const std::string& code) {
+static void write_cached(const std::string& path, const std::string& key, const std::string& code) {
   std::string tmp = path + ".tmp" + std::to_string(getpid());
   {
     std::ofstream f(tmp, std::ios::binary);
@@ -143,6 +149,7 @@ static void write_cached(const std::string& path, const std::string& code) {
     uint64_t len = code.size();
     std::string digest = sha256_hex(code);
     f.write(kMagic, 8);
+    f.write(key.data(), 64);
     f.write(reinterpret_cast<const char*>(&len), 8);
     f.write(digest.data(), 64);
     f.write(code.data(), static_cast<std::streamsize>(code.size()));
@@ -150,6 +157,10 @@ static void write_cached(const std::string& path, const std::string& code) {
   }
   if (std::rename(tmp.c_str(), path.c_str()) != 0) std::remove(tmp.c_str());
 }
+
+namespace {
+std::atomic<int64_t> g_stat_compiles{0}, g_stat_disk{0}, g_stat_memory{0};
+}  // namespace
 
 #define RTC_CHECK(expr)                                                                   \
   do {                                                                                    \
@@ -197,25 +208,39 @@ hipFunction_t jit_get(const std::string& source, const std::string& name) {
   std::string mkey = std::to_string(dev.id) + ":" + key;
   std::lock_guard<std::mutex> lk(g_jit_mu);
   auto fit = g_funcs.find(mkey + ":" + name);
-  if (fit != g_funcs.end()) return fit->second;
+  if (fit != g_funcs.end()) {
+    ++g_stat_memory;
+    return fit->second;
+  }
   hipModule_t mod = nullptr;
   auto mit = g_modules.find(mkey);
   if (mit != g_modules.end()) {
     mod = mit->second;
+    ++g_stat_memory;
   } else {
     std::string code;
     std::string dir = cache_dir();
     std::string path = dir + "/" + key + ".hsaco";
-    bool from_disk = read_cached(path, code);
-    if (!from_disk) {
-      code = compile_code_object(full, dev.arch);
-      write_cached(path, code);
+    bool from_disk = read_cached(path, key, code);
+    hipError_t e = hipErrorInvalidImage;
+    hipFunction_t probe = nullptr;
+    if (from_disk) {
+      e = hipModuleLoadData(&mod, code.data());
+      if (e == hipSuccess && hipModuleGetFunction(&probe, mod, name.c_str()) != hipSuccess) {
+        // a file that loads but lacks the kernel: not this source's code
+        (void)hipModuleUnload(mod);
+        mod = nullptr;
+        e = hipErrorNotFound;
+      }
+      (void)hipGetLastError();
     }
-    hipError_t e = hipModuleLoadData(&mod, code.data());
-    if (e != hipSuccess && from_disk) {
-      // a stale or foreign file: rebuild from source once
+    if (e == hipSuccess) {
+      ++g_stat_disk;
+    } else {
+      // no cached file, or one that is truncated, damaged, foreign or stale: compile from source
       code = compile_code_object(full, dev.arch);
-      write_cached(path, code);
+      ++g_stat_compiles;
+      write_cached(path, key, code);
       e = hipModuleLoadData(&mod, code.data());
     }
     if (e != hipSuccess) NCA_THROW(NCA_ERR_HIP, std::string("hipModuleLoadData: ") + hipGetErrorString(e));
@@ -235,6 +260,15 @@ void jit_launch(hipFunction_t fn, dim3 grid, dim3 block, size_t shmem, hipStream
 }
 
 }  // namespace nca
+
+extern "C" int nca_hip_jit_stats(int64_t* compiles, int64_t* disk_loads, int64_t* memory_hits) {
+  return nca::api_call([&] {
+    NCA_REQUIRE(compiles && disk_loads && memory_hits, "null argument");
+    *compiles = nca::g_stat_compiles.load();
+    *disk_loads = nca::g_stat_disk.load();
+    *memory_hits = nca::g_stat_memory.load();
+  });
+}
 
 extern "C" int nca_hip_rtc_compile(const char* source, const char* arch, char* log, size_t n) {
   std::string msg;

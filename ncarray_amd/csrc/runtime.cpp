@@ -88,9 +88,16 @@ Workspace& workspace(hipStream_t s, size_t partial_bytes, size_t big_bytes) {
   auto& slot = dev.workspaces[s];
   if (!slot) slot = std::make_unique<Workspace>();
   Workspace& w = *slot;
+  // Growing allocates and synchronises, which a stream capture cannot hold: refuse it when this
+  // stream is capturing, and for the legacy stream also when the query itself fails (the legacy
+  // stream cannot be used while another stream captures in global mode).
   auto capturing = [&]() {
     hipStreamCaptureStatus st = hipStreamCaptureStatusNone;
-    if (s != nullptr) HIP_CHECK(hipStreamIsCapturing(s, &st));
+    const hipError_t e = hipStreamIsCapturing(s, &st);
+    if (e != hipSuccess) {
+      (void)hipGetLastError();
+      return true;
+    }
     return st != hipStreamCaptureStatusNone;
   };
   const size_t min_partials = size_t(1) << 20;
