@@ -30,10 +30,29 @@ HBM_PEAK_GBPS = 8000.0
 SHAPE = (512, 512, 512)
 # the fused step's kernel: float32 march stencil with the axis-0 sum and the max in its epilogue
 MARCH_KERNEL = "nca_stencil_march_float32_axis0_max"
-# HBM bytes per launch of that kernel from rocprofv3 PMC runs of this command (FETCH_SIZE
-# scaled by 2.0, the factor that makes a 512 MiB device copy read 512 MiB on this hardware;
-# WRITE_SIZE exact), recorded in profiles/nca_stencil_march_laplacian_512cubed.md.
-MARCH_TRAFFIC_BYTES = 1109746802
+# The rocprofv3 PMC summary of that kernel (FETCH_SIZE and WRITE_SIZE per dispatch, calibrated
+# against a copy of known size): its `traffic_bytes_per_launch` line is the kernel's HBM traffic,
+# valid for the kernel source whose SHA-256 its `kernel_source_sha256` line records.
+MARCH_PROFILE = os.path.join(ROOT, "profiles", "nca_stencil_march_laplacian_512cubed.md")
+
+
+def march_traffic(st):
+    """(traffic bytes per launch or None, note): the recorded PMC traffic of the headline kernel
+    when the summary was measured on the kernel this build generates (same source SHA-256)."""
+    import hashlib
+    sha = hashlib.sha256(st.source(4).encode()).hexdigest()
+    rec = {}
+    try:
+        with open(MARCH_PROFILE) as f:
+            for line in f:
+                if ":" in line and line.split(":", 1)[0].strip() in ("traffic_bytes_per_launch", "kernel_source_sha256"):
+                    k, v = line.split(":", 1)
+                    rec[k.strip()] = v.strip().strip("`")
+    except OSError:
+        return None, "no PMC summary found", sha
+    if rec.get("kernel_source_sha256") != sha:
+        return None, "the PMC summary was measured on another build of the kernel (source SHA-256 differs)", sha
+    return int(rec["traffic_bytes_per_launch"]), os.path.relpath(MARCH_PROFILE, ROOT), sha
 
 
 def parse():
@@ -47,10 +66,11 @@ def parse():
                     help="also run a plain 512 MiB device copy (for PMC byte-counter calibration)")
     ap.add_argument("--only-headline", action="store_true",
                     help="skip the other workloads (expression, binning, autograd, ragged batch)")
-    ap.add_argument("--workloads", default="c0,c2,c3,c4,cp",
+    ap.add_argument("--workloads", default="c0,c2,c3,c4,cp,st",
                     help="comma list of the other workloads to run: c0 (fused expression + sum), c2 (binning), "
                          "c3 (autograd pipeline), c4 (ragged batch), cp (casting, strided, permuted and "
-                         "pointer-table copies)")
+                         "pointer-table copies, and the runtime's device copy), st (tile stencils: 2-D "
+                         "Laplacian, pixel binning, pointer-axis planes)")
     return ap.parse_args()
 
 
@@ -128,6 +148,7 @@ def main():
     for name, kms in prof:
         per_kernel.setdefault(name, []).append(kms)
     march_ms = float(np.mean(per_kernel[MARCH_KERNEL]))
+    traffic, traffic_note, kernel_sha = march_traffic(st)
     kernel_bytes = 8 * n  # dominant kernel: reads x once, writes the Laplacian once
     achieved = kernel_bytes / (march_ms * 1e-3) / 1e9
 
@@ -190,7 +211,7 @@ def main():
         chosen = {w.strip() for w in args.workloads.split(",") if w.strip()}
         for name, fn in (("c0_fused_expression_sum", bench_c0), ("c2_binning", bench_c2),
                          ("c3_autograd_pipeline", bench_c3), ("c4_ragged_batch", bench_c4),
-                         ("cp_layout_copies", bench_cp)):
+                         ("cp_layout_copies", bench_cp), ("st_tile_stencils", bench_st)):
             if name.split("_")[0] not in chosen:
                 continue
             try:
@@ -202,7 +223,7 @@ def main():
     # ---- CPU baseline (oracle restatement of the reference host path) ----
     cpu = None
     if not args.no_cpu_baseline:
-        cpu = cpu_baseline(x.cpu().numpy(), bytes_fused)
+        cpu = cpu_baseline(x.cpu().numpy(), bytes_fused, extra)
 
     out = {
         "metric": "HBM GB/s (% of 8 TB/s) fwd vs fwd+bwd, err vs FP64; CPU host baseline",
@@ -231,8 +252,10 @@ def main():
             "peak": HBM_PEAK_GBPS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBPS, 4),
-            "traffic": MARCH_TRAFFIC_BYTES,
-            "traffic_over_algorithmic": round(MARCH_TRAFFIC_BYTES / kernel_bytes, 4),
+            "traffic": traffic,
+            "traffic_over_algorithmic": round(traffic / kernel_bytes, 4) if traffic else None,
+            "traffic_source": traffic_note,
+            "kernel_source_sha256": kernel_sha,
             "kernel": MARCH_KERNEL + " (Laplacian + axis-0 sum + max epilogue)",
             "bytes_per_launch": kernel_bytes,
             "avg_launch_ms": round(march_ms, 4),
@@ -557,6 +580,76 @@ def bench_cp(torch, capi, tapes, ti, dev, sptr, time_loop):
         ms = float(np.median([k for _, k in recs]))
         res[name] = {"kernel": recs[0][0], "kernel_ms": round(ms, 4), "GBps": gbps(nbytes, ms),
                      "frac_of_8TBps": round(nbytes / (ms * 1e-3) / 1e9 / HBM_PEAK_GBPS, 4)}
+    # the runtime's device-to-device copy of the same 512 MiB (hipMemcpyAsync), timed with events on
+    # torch's stream: the copy ceiling the library's plain copy is compared with
+    for _ in range(3):
+        dense.copy_(x)
+    torch.cuda.synchronize()
+    evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(10)]
+    for a, b in evs:
+        a.record()
+        dense.copy_(x)
+        b.record()
+    torch.cuda.synchronize()
+    ms = float(np.median([a.elapsed_time(b) for a, b in evs]))
+    res["runtime_d2d_copy_f32"] = {"kernel": "hipMemcpyAsync device to device (runtime)", "kernel_ms": round(ms, 4),
+                                   "GBps": gbps(n ** 3 * 8, ms),
+                                   "frac_of_8TBps": round(n ** 3 * 8 / (ms * 1e-3) / 1e9 / HBM_PEAK_GBPS, 4)}
+    return res
+
+
+def bench_st(torch, capi, tapes, ti, dev, sptr, time_loop):
+    """The tile stencil kernel: a 5-point 2-D Laplacian on a 16384^2 float32 grid (zero boundary),
+    2x2 pixel binning (step 2) of 256 frames of 1024^2 float32 into 512^2, and the 3-D Laplacian
+    of a 512^3 float32 grid whose 512 planes are reached through a table of plane pointers (an NC
+    pointer axis 0).  Kernel time from the library's HIP events; algorithmic bytes = source read
+    once + destination written once."""
+    P = capi.pack
+    five = [P(capi.OP_LOAD_NCARR, 1), P(capi.OP_LOAD_NCARR, 2), P(capi.OP_ADD), P(capi.OP_LOAD_NCARR, 3),
+            P(capi.OP_ADD), P(capi.OP_LOAD_NCARR, 4), P(capi.OP_ADD), P(capi.OP_LOAD_CONST, 0),
+            P(capi.OP_LOAD_NCARR, 0), P(capi.OP_MUL), P(capi.OP_SUB)]
+    lap2 = capi.Stencil([[0, 0], [-1, 0], [1, 0], [0, -1], [0, 1]], five, capi.FLOAT32, capi.FLOAT32, capi.FLOAT32,
+                        [capi.FLOAT32])
+    win = [[0, a, b] for a in (0, 1) for b in (0, 1)]
+    sum4 = [P(capi.OP_LOAD_NCARR, 0), P(capi.OP_LOAD_NCARR, 1), P(capi.OP_ADD), P(capi.OP_LOAD_NCARR, 2),
+            P(capi.OP_ADD), P(capi.OP_LOAD_NCARR, 3), P(capi.OP_ADD)]
+    binst = capi.Stencil(win, sum4, capi.FLOAT32, capi.FLOAT32, capi.FLOAT32)
+    lap3 = capi.Stencil(tapes.LAPLACIAN_OFFSETS, tapes.laplacian(), capi.FLOAT32, capi.FLOAT32, capi.FLOAT32,
+                        [capi.FLOAT32], is_pointer_axis=[1, 0, 0])
+    g = torch.Generator(device=dev).manual_seed(0)
+    S = capi.STREAM_DEFAULT_ASYNC
+    res = {}
+
+    def measure(name, fn, nbytes):
+        for _ in range(3):
+            fn()
+        torch.cuda.synchronize()
+        capi.profile_enable(True)
+        for _ in range(10):
+            fn()
+        torch.cuda.synchronize()
+        recs = [k for k in capi.profile_read() if k[0] == "nca_stencil_tile"]
+        capi.profile_enable(False)
+        ms = float(np.median([k for _, k in recs]))
+        res[name] = {"kernel": "nca_stencil_tile", "kernel_ms": round(ms, 4), "GBps": gbps(nbytes, ms),
+                     "frac_of_8TBps": round(nbytes / (ms * 1e-3) / 1e9 / HBM_PEAK_GBPS, 4)}
+
+    x = torch.randn((16384, 16384), generator=g, device=dev)
+    o = torch.empty_like(x)
+    measure("laplacian_2d_16384sq_f32", lambda: lap2.apply(ti.view(x), ti.view(o), capi.BOUNDARY_ZERO, scalars=(4.0,),
+                                                         stream=S), 8 * x.numel())
+    del x, o
+    fr = torch.rand((256, 1024, 1024), generator=g, device=dev)
+    ob = torch.empty((256, 512, 512), device=dev)
+    measure("pixel_binning_2x2_256x1024sq_f32", lambda: binst.apply(ti.view(fr), ti.view(ob), capi.BOUNDARY_NONE,
+                                                                    stream=S), 4 * fr.numel() + 4 * ob.numel())
+    del fr, ob
+    x3 = torch.randn((512, 512, 512), generator=g, device=dev)
+    o3 = torch.empty_like(x3)
+    tbl = torch.tensor([x3[i].data_ptr() for i in range(512)], dtype=torch.int64, device=dev)
+    v3 = capi.make_nc_view(tbl.data_ptr(), capi.FLOAT32, (512, 512, 512), [1, 2048, 4], 0)
+    measure("laplacian_3d_512cubed_pointer_planes_f32", lambda: lap3.apply(v3, ti.view(o3), capi.BOUNDARY_ZERO,
+                                                                           scalars=(6.0,), stream=S), 8 * x3.numel())
     return res
 
 
@@ -653,40 +746,110 @@ def fp64_errors(x, lap, col, mx, xb, lapb, colb, mxb):
     return res
 
 
-def cpu_baseline(xn, bytes_per_step):
-    """The oracle's C restatement of the host path (Laplacian -> axis-0 fold -> max),
-    timed on this machine's cores: 1 thread and all threads (OpenMP over z planes)."""
-    from oracle import oracle as O  # the checker/baseline, never the measured GPU path
-    import multiprocessing
-    ncores = min(int(os.environ.get("OMP_NUM_THREADS", multiprocessing.cpu_count())), multiprocessing.cpu_count())
-    res = {}
-    for threads in (1, ncores):
-        reps = []
-        for _ in range(2):
-            t0 = time.perf_counter()
-            O.laplacian_pipeline(xn, threads)
-            reps.append(time.perf_counter() - t0)
-        res[threads] = min(reps)
-    cpu_model = ""
+def cpu_info():
+    """CPU model, core counts and the threads this process may use (lscpu and /proc/cpuinfo)."""
+    import subprocess
+    info = {"logical_cpus": os.cpu_count()}
+    try:
+        info["usable_cpus"] = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        info["usable_cpus"] = os.cpu_count()
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        for line in out.splitlines():
+            k, _, v = line.partition(":")
+            k = k.strip()
+            if k in ("Model name", "Socket(s)", "Core(s) per socket", "Thread(s) per core", "CPU family", "Model",
+                     "Stepping", "CPU max MHz"):
+                info["lscpu_" + k.lower().replace(" ", "_").replace("(s)", "s")] = v.strip()
+    except (OSError, subprocess.SubprocessError):
+        pass
     try:
         with open("/proc/cpuinfo") as f:
             for line in f:
                 if line.startswith("model name"):
-                    cpu_model = line.split(":", 1)[1].strip()
+                    info["cpu_model"] = line.split(":", 1)[1].strip()
                     break
     except OSError:
         pass
-    best_threads = ncores
+    return info
+
+
+def cpu_baseline(xn, bytes_per_step, gpu_extra=None):
+    """The oracle's C restatement of the reference host path, timed on this machine's cores at
+    1 thread and at all usable threads (OpenMP, as the reference VM path splits its outer loop,
+    hostengine.hh:97-118): the headline C1 pipeline on the full 512^3 grid, and bounded samples
+    of the other workloads (C0 in full, 16 of C2's 256 frames, C3 at 256^3 forward and
+    forward+backward, C4's first 64 arrays).  GB/s use the same algorithmic bytes as the GPU
+    figures; best of 2 runs each."""
+    from oracle import oracle as O  # the checker/baseline, never the measured GPU path
+    from ncarray_amd import batch, tapes
+    info = cpu_info()
+    ncores = max(1, min(int(os.environ.get("OMP_NUM_THREADS", info["usable_cpus"])), info["usable_cpus"]))
+    thread_counts = sorted({1, ncores})
+
+    def best(fn, reps=2):
+        ts = []
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            fn()
+            ts.append(time.perf_counter() - t0)
+        return min(ts)
+
+    def rates(fn, nbytes, sample):
+        out = {"sample": sample, "algorithmic_bytes": int(nbytes)}
+        for th in thread_counts:
+            sec = best(lambda: fn(th))
+            out[f"threads_{th}"] = {"seconds": round(sec, 4), "GBps": round(nbytes / sec / 1e9, 3)}
+        return out
+
+    res = {th: best(lambda: O.laplacian_pipeline(xn, th)) for th in thread_counts}
+    per = {}
+    rng = np.random.default_rng(0)
+    # C0: the expression through the oracle's tape VM, then the float32 fold (12 B/element)
+    n0 = 1 << 22
+    a = rng.uniform(-1, 1, n0).astype(np.float32)
+    b = rng.uniform(-1, 1, n0).astype(np.float32)
+    per["c0_fused_expression_sum"] = rates(
+        lambda th: O.expr_sum_mt(tapes.fused_c0(), [a, b, a, a, b], [0.5], 10, th), 12 * n0,
+        "the full 2^22-element workload")
+    # C2: per-frame bin counts and sums, 16 frames of 1024^2 with the radial 1000-bin map
+    F2, nb = 16, 1000
+    fr = rng.exponential(50.0, (F2, 1024, 1024)).astype(np.float32)
+    fr[rng.random(fr.shape) < 0.01] = 0
+    bm = O.radial_binmap()
+    per["c2_binning"] = rates(lambda th: O.bin_f32_mt(fr, bm, nb, True, th),
+                              4 * fr.size + 4 * bm.size + 2 * F2 * nb * 4, "16 of the 256 frames of 1024^2")
+    del fr
+    # C3: Laplacian + C0 loss at 256^3, forward and forward+backward (8 and 16 B/element)
+    x3 = rng.standard_normal((256, 256, 256)).astype(np.float32)
+    y3 = rng.standard_normal((256, 256, 256)).astype(np.float32)
+    per["c3_forward"] = rates(lambda th: O.laplacian_c0(x3, y3, False, th), 8 * x3.size, "the full 256^3 float32 grid")
+    per["c3_forward_backward"] = rates(lambda th: O.laplacian_c0(x3, y3, True, th), 16 * x3.size,
+                                       "the full 256^3 float32 grid")
+    del x3, y3
+    # C4: the per-array loop over the first 64 arrays of the seeded ragged sizes (12 B/element)
+    sizes = batch.ragged_sizes(4096, 1e3, 4e6, seed=0)[:64]
+    items = [(rng.uniform(-1, 1, int(k)).astype(np.float32), rng.uniform(-1, 1, int(k)).astype(np.float32))
+             for k in sizes]
+
+    def c4(th):
+        for ia, ib in items:
+            O.expr_sum_mt(tapes.fused_c0(), [ia, ib, ia, ia, ib], [0.5], 10, th)
+
+    per["c4_ragged_batch"] = rates(c4, 12 * int(sizes.sum()), f"the first 64 of the 4096 arrays ({int(sizes.sum())} elements)")
+    del items
     return {
-        "value": round(bytes_per_step / res[best_threads] / 1e9, 3),
+        "value": round(bytes_per_step / res[ncores] / 1e9, 3),
         "unit": "GB/s",
-        "cores": best_threads,
+        "cores": ncores,
         "kind": "port",
         "sample": "the full 512^3 float32 workload (Laplacian, axis-0 sum, max), best of 2 runs",
-        "seconds_per_step": round(res[best_threads], 4),
+        "seconds_per_step": round(res[ncores], 4),
         "single_thread_GBps": round(bytes_per_step / res[1] / 1e9, 3),
         "single_thread_seconds": round(res[1], 4),
-        "cpu_model": cpu_model,
+        "cpu": info,
+        "per_config": per,
     }
 
 

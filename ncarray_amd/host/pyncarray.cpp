@@ -659,9 +659,17 @@ PYBIND11_MODULE(_ncarray, m) {
       .def_static("from_dlpack", [](const py::object& obj) {
         if (py::isinstance<py::capsule>(obj)) return dl::from_capsule(obj.cast<py::capsule>());
         if (!py::hasattr(obj, "__dlpack__")) throw py::type_error("object has no __dlpack__ and is not a DLPack capsule");
-        return dl::from_capsule(obj.attr("__dlpack__")().cast<py::capsule>());
+        // The consumer stream of the DLPack protocol: this library's calls without a stream run on
+        // the legacy default stream, which is 0 for a ROCm producer (1 for a CUDA one); the
+        // producer then orders its pending work on its current stream before that stream.
+        int dev_type = 0;
+        if (py::hasattr(obj, "__dlpack_device__")) dev_type = obj.attr("__dlpack_device__")().cast<py::tuple>()[0].cast<int>();
+        py::object cap = dev_type == static_cast<int>(dl::kROCM) ? obj.attr("__dlpack__")(py::arg("stream") = 0)
+                                                                 : obj.attr("__dlpack__")();
+        return dl::from_capsule(cap.cast<py::capsule>());
       }, py::arg("obj"), "Zero-copy view of a DLPack producer (an object with __dlpack__, e.g. a torch tensor, or a "
-                         "capsule); the producer's memory stays alive while the view does.")
+                         "capsule); the producer's memory stays alive while the view does, and a ROCm producer's "
+                         "pending work on its current stream is ordered before the library's default stream.")
       .def_property_readonly("__cuda_array_interface__", [](const NCDevArray& a) {
         if (a.pointer_axis() >= 0)
           throw Error(NCA_ERR_UNSUPPORTED, "a pointer-axis array has no __cuda_array_interface__ (use dense())");

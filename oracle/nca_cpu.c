@@ -429,3 +429,94 @@ int nca_cpu_max_threads(void) {
   return 1;
 #endif
 }
+
+/* ---- CPU baselines of the other workloads (timed by bench.py) ------------------------------
+ * The same restatements, with threads > 1 splitting the work over OpenMP threads as the
+ * reference VM path does (engines/hostengine.hh:97-118); per-output arithmetic is unchanged. */
+
+/* Tape evaluation (nca_cpu_expr's VM) into a float32 out, then the float32 fold of out
+ * (host/reductions.hh:58-73); threads > 1: contiguous chunks per thread, each folded in order,
+ * the chunk sums added in chunk order.  Returns the sum. */
+float nca_cpu_expr_sum_mt(const uint32_t* ins, int n_ins, const void* const* arrays, const int* arr_dtypes,
+                          int n_arrays, const double* scalars, int eval_dtype, float* out, int64_t n, int threads) {
+  if (threads < 1) threads = 1;
+  float part[256];
+  if (threads > 256) threads = 256;
+  const int64_t chunk = (n + threads - 1) / threads;
+#ifdef _OPENMP
+#pragma omp parallel for num_threads(threads) schedule(static)
+#endif
+  for (int t = 0; t < threads; ++t) {
+    double loads[64];
+    float acc = 0.0f;
+    const int64_t i0 = t * chunk, i1 = (t + 1) * chunk < n ? (t + 1) * chunk : n;
+    for (int64_t i = i0; i < i1; ++i) {
+      for (int k = 0; k < n_arrays; ++k) loads[k] = load_d(arrays[k], arr_dtypes[k], i);
+      const float v = (float)eval_one(ins, n_ins, loads, scalars, (double)i, eval_dtype, 0);
+      out[i] = v;
+      acc += v;
+    }
+    part[t] = acc;
+  }
+  float s = 0.0f;
+  for (int t = 0; t < threads; ++t) s += part[t];
+  return s;
+}
+
+/* nca_cpu_bin_f32 with the frames split over threads. */
+void nca_cpu_bin_f32_mt(const float* frames, int64_t F, int64_t P, const int32_t* map, int nb, int nonzero,
+                        int64_t* counts, double* sums, int threads) {
+  if (threads < 1) threads = 1;
+#ifdef _OPENMP
+#pragma omp parallel for num_threads(threads) schedule(static)
+#endif
+  for (int64_t f = 0; f < F; ++f) nca_cpu_bin_f32(frames + f * P, 1, P, map, nb, nonzero, counts + f * nb, sums + f * nb);
+}
+
+/* 7-point Laplacian with zero Dirichlet in the reference op order (as the pipeline above). */
+static void lap3d(const float* x, float* out, int64_t n0, int64_t n1, int64_t n2, int threads) {
+  const int64_t plane = n1 * n2;
+#ifdef _OPENMP
+#pragma omp parallel for num_threads(threads) schedule(static)
+#endif
+  for (int64_t z = 0; z < n0; ++z)
+    for (int64_t y = 0; y < n1; ++y) {
+      const float* row = x + z * plane + y * n2;
+      float* o = out + z * plane + y * n2;
+      for (int64_t xx = 0; xx < n2; ++xx) {
+        float v = (z > 0 ? row[xx - plane] : 0.0f) + (z + 1 < n0 ? row[xx + plane] : 0.0f);
+        v = v + (y > 0 ? row[xx - n2] : 0.0f);
+        v = v + (y + 1 < n1 ? row[xx + n2] : 0.0f);
+        v = v + (xx > 0 ? row[xx - 1] : 0.0f);
+        v = v + (xx + 1 < n2 ? row[xx + 1] : 0.0f);
+        o[xx] = v - 6.0f * row[xx];
+      }
+    }
+}
+
+/* The differentiable C3 pipeline: a = Laplacian(x), loss = sum(a*y + exp(-a*a) - 0.5*y) (float32
+ * per point in tape order, float64 sum); with with_grad, dy = a - 0.5 and dx = Laplacian(w) with
+ * w = y - 2*a*exp(-a*a) (the Laplacian with zero boundary is its own transpose).  Returns the
+ * loss. */
+double nca_cpu_laplacian_c0(const float* x, const float* y, int64_t n0, int64_t n1, int64_t n2, float* a, float* dx,
+                            float* dy, int with_grad, int threads) {
+  if (threads < 1) threads = 1;
+  const int64_t n = n0 * n1 * n2;
+  lap3d(x, a, n0, n1, n2, threads);
+  double loss = 0.0;
+#ifdef _OPENMP
+#pragma omp parallel for num_threads(threads) schedule(static) reduction(+ : loss)
+#endif
+  for (int64_t i = 0; i < n; ++i) {
+    const float av = a[i], yv = y[i];
+    const float e = expf(-av * av);
+    const float v = av * yv + e - 0.5f * yv;
+    loss += (double)v;
+    if (with_grad) {
+      dy[i] = av - 0.5f;
+      a[i] = yv - 2.0f * av * e;   /* w, in place */
+    }
+  }
+  if (with_grad) lap3d(a, dx, n0, n1, n2, threads);
+  return loss;
+}

@@ -59,6 +59,13 @@ lib.nca_cpu_max_threads.restype = ctypes.c_int
 lib.nca_cpu_f32_to_bf16.argtypes = [ctypes.c_float]
 lib.nca_cpu_f32_to_bf16.restype = ctypes.c_uint16
 
+lib.nca_cpu_expr_sum_mt.argtypes = [ctypes.POINTER(ctypes.c_uint32), ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
+                                    ctypes.c_int, ctypes.POINTER(ctypes.c_double), ctypes.c_int,
+                                    ctypes.POINTER(ctypes.c_float), ctypes.c_int64, ctypes.c_int]
+lib.nca_cpu_expr_sum_mt.restype = ctypes.c_float
+lib.nca_cpu_bin_f32_mt.restype = None
+lib.nca_cpu_laplacian_c0.restype = ctypes.c_double
+
 # numpy dtype -> reference DType code
 CODE = {np.dtype(np.bool_): 0, np.dtype(np.uint8): 2, np.dtype(np.uint16): 3, np.dtype(np.uint32): 4,
         np.dtype(np.uint64): 5, np.dtype(np.int8): 6, np.dtype(np.int16): 7, np.dtype(np.int32): 8,
@@ -299,3 +306,45 @@ class ByteImage:
         """A table of 8-byte pointers to image offsets (plus `deltas` bytes each); returns its offset."""
         vals = np.asarray(offsets, np.int64) + np.asarray(deltas, np.int64) + self.addr
         return self.put(vals.astype("<u8"), align=8)
+
+
+# ---- CPU baselines (bench.py times these; threads > 1 uses OpenMP) ------------------------------
+def expr_sum_mt(instrs, arrays, scalars, eval_dtype, threads=1):
+    """(out float32, float32 sum) of a tape over contiguous same-size arrays."""
+    arrays = [np.ascontiguousarray(a) for a in arrays]
+    n = arrays[0].size
+    ptrs = (ctypes.c_void_p * len(arrays))(*[a.ctypes.data for a in arrays])
+    dts = (ctypes.c_int * len(arrays))(*[CODE[a.dtype] for a in arrays])
+    out = np.empty(n, dtype=np.float32)
+    ins = np.array(instrs, dtype=np.uint32)
+    sc = np.array(list(scalars) or [0.0], dtype=np.float64)
+    s = lib.nca_cpu_expr_sum_mt(ins.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)), len(instrs), ptrs, dts,
+                                len(arrays), sc.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), eval_dtype,
+                                out.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), n, threads)
+    return out, np.float32(s)
+
+
+def bin_f32_mt(frames, binmap, nb, nonzero=True, threads=1):
+    frames = np.ascontiguousarray(frames, dtype=np.float32)
+    F = frames.shape[0]
+    P = binmap.size
+    counts = np.zeros((F, nb), dtype=np.int64)
+    sums = np.zeros((F, nb), dtype=np.float64)
+    lib.nca_cpu_bin_f32_mt(frames.ctypes.data_as(ctypes.c_void_p), ctypes.c_int64(F), ctypes.c_int64(P),
+                           np.ascontiguousarray(binmap, dtype=np.int32).ctypes.data_as(ctypes.c_void_p), nb,
+                           1 if nonzero else 0, counts.ctypes.data_as(ctypes.c_void_p),
+                           sums.ctypes.data_as(ctypes.c_void_p), threads)
+    return counts, sums
+
+
+def laplacian_c0(x, y, with_grad=True, threads=1):
+    """(loss, dx, dy) of sum(C0(Laplacian(x), y)) (dx, dy None without with_grad)."""
+    x = np.ascontiguousarray(x, dtype=np.float32)
+    y = np.ascontiguousarray(y, dtype=np.float32)
+    a = np.empty_like(x)
+    dx = np.empty_like(x) if with_grad else a
+    dy = np.empty_like(x) if with_grad else a
+    f = lambda t: t.ctypes.data_as(ctypes.c_void_p)  # noqa: E731
+    loss = lib.nca_cpu_laplacian_c0(f(x), f(y), ctypes.c_int64(x.shape[0]), ctypes.c_int64(x.shape[1]),
+                                    ctypes.c_int64(x.shape[2]), f(a), f(dx), f(dy), 1 if with_grad else 0, threads)
+    return (loss, dx, dy) if with_grad else (loss, None, None)

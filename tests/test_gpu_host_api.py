@@ -268,6 +268,23 @@ def test_dlpack_from_torch_is_zero_copy():
     assert b.strides == (8, 32) and np.array_equal(np.array(b), tt.cpu().numpy())
 
 
+def test_dlpack_import_orders_the_producer_stream():
+    """A tensor still being written on a non-default torch stream: from_dlpack passes the
+    library's consumer stream (0, ROCm's default) to __dlpack__, so torch orders its pending work
+    before the library's next call on the default stream (a 256 MiB fill and multiply, then the
+    library's synchronous sum)."""
+    side = torch.cuda.Stream()
+    n = 1 << 26
+    with torch.cuda.stream(side):
+        t = torch.empty(n, device="cuda")
+        t.fill_(1.0)
+        t.mul_(3.0)
+        a = nca.from_dlpack(t)
+        s = a.sum()
+    assert float(s) == 3.0 * n
+    torch.cuda.synchronize()
+
+
 def test_dlpack_to_torch_is_zero_copy():
     a = nca.NCDevArray.from_host(np.arange(24, dtype=np.int32).reshape(2, 3, 4))
     t = torch.from_dlpack(a)

@@ -320,3 +320,50 @@ def test_so_scatter_round_trip_touches_only_the_view():
     for r in range(R):
         assert np.array_equal(out[rows[r]:rows[r] + 4].view(np.float32), [-9.0])
         assert np.array_equal(out[rows[r] + 4 * (W + 1):rows[r] + 4 * (W + 2)].view(np.float32), [-9.0])
+
+
+def test_cpu_baselines_match_the_oracle():
+    """The multi-threaded baseline forms give the oracle's values: the tape output bit-exact and
+    the 1-thread sum equal to the sequential fold; binning equal to bin_f32; the C3 loss and
+    gradients against an independent float64 numpy evaluation, whose gradient is pinned by
+    central finite differences."""
+    from ncarray_amd import tapes
+    rng = np.random.default_rng(21)
+    a = rng.uniform(-1, 1, 10001).astype(np.float32)
+    b = rng.uniform(-1, 1, 10001).astype(np.float32)
+    ref = O.expr(tapes.fused_c0(), [a, b, a, a, b], [0.5], 10, np.float32)
+    for threads in (1, 4):
+        out, s = O.expr_sum_mt(tapes.fused_c0(), [a, b, a, a, b], [0.5], 10, threads)
+        assert np.array_equal(out.view(np.uint32), ref.view(np.uint32))
+        if threads == 1:
+            assert s == O.sum_fold(ref)
+        # recursive summation's bound: (n - 1) u sum|x|
+        bound = (ref.size - 1) * float(np.finfo(np.float32).eps) / 2 * float(np.abs(ref).astype(np.float64).sum())
+        assert abs(float(s) - ref.astype(np.float64).sum()) <= bound
+    fr = rng.exponential(50.0, (6, 40, 40)).astype(np.float32)
+    fr[rng.random(fr.shape) < 0.1] = 0
+    bm = O.radial_binmap(40, 40, 20, 20, 30)
+    c1, s1 = O.bin_f32(fr, bm, 30)
+    c2, s2 = O.bin_f32_mt(fr, bm, 30, threads=3)
+    assert np.array_equal(c1, c2) and np.array_equal(s1, s2)
+
+    def loss64(x, y):
+        A = O.laplacian_fp64(x)
+        return float((A * y + np.exp(-A * A) - 0.5 * y).sum())
+
+    x = rng.standard_normal((6, 7, 8)).astype(np.float32)
+    y = rng.standard_normal((6, 7, 8)).astype(np.float32)
+    loss, dx, dy = O.laplacian_c0(x, y, True, threads=2)
+    A = O.laplacian_fp64(x)
+    y64 = y.astype(np.float64)
+    dx64 = O.laplacian_fp64(y64 - 2 * A * np.exp(-A * A))
+    assert abs(loss - loss64(x, y)) <= 1e-5 * np.abs(A * y64 + np.exp(-A * A) - 0.5 * y64).sum()
+    assert np.allclose(dy, A - 0.5, atol=2e-5) and np.allclose(dx, dx64, atol=2e-4)
+    h = 1e-3
+    x64 = x.astype(np.float64)
+    for idx in [(0, 0, 0), (3, 4, 5), (5, 6, 7), (2, 0, 7)]:
+        e = np.zeros_like(x64)
+        e[idx] = h
+        fd = (loss64(x64 + e, y64) - loss64(x64 - e, y64)) / (2 * h)
+        assert abs(fd - dx64[idx]) <= 1e-5 * (1 + abs(fd)), idx
+    assert O.laplacian_c0(x, y, False)[0] == loss
