@@ -610,7 +610,8 @@ NCDevArray NCDevArray::view_from_indices(const std::vector<IndexItem>& items) co
   NCDevArray r = *this;
   r.nd_ = 0;
   r.pax_ = -1;
-  char* base = static_cast<char*>(data_);
+  // address arithmetic in integers: a geometry-only array has no data pointer
+  uintptr_t base = reinterpret_cast<uintptr_t>(data_);
   // pointer axis (out_from_axes_ptr, array_impl.hh:964-1031): axes before it move the table
   // position, an integer index on it dereferences the row, a slice moves the table offset;
   // axes after it move the row offset while the pointer axis is kept
@@ -629,7 +630,7 @@ NCDevArray NCDevArray::view_from_indices(const std::vector<IndexItem>& items) co
   };
   auto move = [&](int a, int64_t bytes) {
     if (live_ptr && a > pax_) roff += bytes;
-    else base += bytes;
+    else base += static_cast<uintptr_t>(bytes);
   };
   for (const IndexItem& it : items) {
     if (std::holds_alternative<Ellipsis>(it)) {
@@ -643,11 +644,10 @@ NCDevArray NCDevArray::view_from_indices(const std::vector<IndexItem>& items) co
       if (i < 0) i += sh_[src];
       if (live_ptr && src == pax_) {
         NCDevArray probe = *this;
-        probe.data_ = base;
+        probe.data_ = reinterpret_cast<void*>(base);
         probe.poff_ = poff;
         probe.roff_ = 0;
-        base = static_cast<char*>(probe.row(i));
-        base += roff;
+        base = reinterpret_cast<uintptr_t>(probe.row(i)) + static_cast<uintptr_t>(roff);
         roff = 0;
         live_ptr = false;
       } else {
@@ -683,7 +683,7 @@ NCDevArray NCDevArray::view_from_indices(const std::vector<IndexItem>& items) co
     }
   }
   for (; src < nd_; ++src) keep_axis(src);
-  r.data_ = base;
+  r.data_ = reinterpret_cast<void*>(base);
   if (live_ptr) {
     r.poff_ = poff;
     r.roff_ = roff;

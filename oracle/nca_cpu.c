@@ -180,15 +180,36 @@ static void store_d(void* p, int dt, int64_t i, double v) {
   }
 }
 
-/* out[i] = tape(arrays[k][i], scalars) for contiguous same-size arrays. */
-void nca_cpu_expr(const uint32_t* ins, int n_ins, const void* const* arrays, const int* arr_dtypes, int n_arrays,
-                  const double* scalars, int eval_dtype, void* out, int out_dtype, int64_t n) {
+static void expr_eval(const uint32_t* ins, int n_ins, const void* const* arrays, const int* arr_dtypes, int n_arrays,
+                      const double* scalars, int eval_dtype, void* out, int out_dtype, int64_t n, int raw_div) {
   double loads[64];
   for (int64_t i = 0; i < n; ++i) {
     for (int k = 0; k < n_arrays; ++k) loads[k] = load_d(arrays[k], arr_dtypes[k], i);
-    double v = eval_one(ins, n_ins, loads, scalars, (double)i, eval_dtype, 0);
+    double v = eval_one(ins, n_ins, loads, scalars, (double)i, eval_dtype, raw_div);
     store_d(out, out_dtype, i, v);
   }
+}
+
+/* out[i] = tape(arrays[k][i], scalars) for contiguous same-size arrays. */
+void nca_cpu_expr(const uint32_t* ins, int n_ins, const void* const* arrays, const int* arr_dtypes, int n_arrays,
+                  const double* scalars, int eval_dtype, void* out, int out_dtype, int64_t n) {
+  expr_eval(ins, n_ins, arrays, arr_dtypes, n_arrays, scalars, eval_dtype, out, out_dtype, n, 0);
+}
+
+/* The reference's host x86 JIT for linearizable tapes (jit/host/rtcompiler.cc:285-610): a
+ * left fold per element with one scalar SSE instruction per operation in the work type,
+ * DIV as divss/divsd (jit/host/x86.cc:157-169), i.e. IEEE division: x/0 is +-inf for a
+ * finite nonzero x and NaN for 0/0, where the VM and the device kernels give NaN for any
+ * finite x, and +-inf/-0 flips the sign, where they return the numerator unchanged
+ * (interface.hh:150-156).  For a linearizable tape (RPN depth at most 2) the left
+ * fold and the stack evaluation coincide.  Float work types only: the JIT's integer DIV is
+ * idiv, which traps on a zero divisor; returns -1 (nothing written) for an integer work
+ * type. */
+int nca_cpu_expr_host_jit(const uint32_t* ins, int n_ins, const void* const* arrays, const int* arr_dtypes,
+                          int n_arrays, const double* scalars, int eval_dtype, void* out, int out_dtype, int64_t n) {
+  if (eval_dtype != D_F32 && eval_dtype != D_F64) return -1;
+  expr_eval(ins, n_ins, arrays, arr_dtypes, n_arrays, scalars, eval_dtype, out, out_dtype, n, 1);
+  return 0;
 }
 
 /* ---- full reductions (host/reductions.hh) -------------------------------------------- */

@@ -40,6 +40,8 @@ lib.nca_cpu_min.restype = ctypes.c_double
 lib.nca_cpu_expr.argtypes = [_u32p, ctypes.c_int, ctypes.POINTER(ctypes.c_void_p), ctypes.POINTER(ctypes.c_int),
                              ctypes.c_int, _f64p, ctypes.c_int, ctypes.c_void_p, ctypes.c_int, ctypes.c_int64]
 lib.nca_cpu_expr.restype = None
+lib.nca_cpu_expr_host_jit.argtypes = lib.nca_cpu_expr.argtypes
+lib.nca_cpu_expr_host_jit.restype = ctypes.c_int
 lib.nca_cpu_reduce_axes_f32.argtypes = [_f32p, ctypes.c_int, _i64p, np.ctypeslib.ndpointer(np.intc), ctypes.c_int,
                                         ctypes.c_void_p]
 lib.nca_cpu_reduce_axes_f32.restype = None
@@ -137,17 +139,24 @@ def reduce_axes_f32(x, axes, op):
 
 
 # ---- expressions ---------------------------------------------------------------------------------------
-def expr(instrs, arrays, scalars, eval_dtype, out_dtype):
+def expr(instrs, arrays, scalars, eval_dtype, out_dtype, host_jit=False):
     """Tape evaluation with the VM/apply_op semantics (dynamicmvnode.hh:146-251,
-    interface.hh:129-181); eval_dtype 10 (float32) or 11 (float64)."""
+    interface.hh:129-181); eval_dtype 10 (float32) or 11 (float64).  host_jit=True restates
+    the reference's host x86 JIT for linearizable tapes instead (jit/host/rtcompiler.cc:285-610,
+    x86.cc:157-169): the same fold with IEEE division (x/0 = +-inf, 0/0 = NaN)."""
     arrays = [np.ascontiguousarray(a) for a in arrays]
     n = arrays[0].size if arrays else 0
     ptrs = (ctypes.c_void_p * max(1, len(arrays)))(*[a.ctypes.data for a in arrays])
     dts = (ctypes.c_int * max(1, len(arrays)))(*[CODE[a.dtype] for a in arrays])
     out = np.empty(n, dtype=out_dtype)
     sc = np.array(list(scalars) or [0.0], dtype=np.float64)
-    lib.nca_cpu_expr(np.array(instrs, dtype=np.uint32), len(instrs), ptrs, dts, len(arrays), sc, eval_dtype,
-                     out.ctypes.data, CODE[np.dtype(out_dtype)], n)
+    args = (np.array(instrs, dtype=np.uint32), len(instrs), ptrs, dts, len(arrays), sc, eval_dtype,
+            out.ctypes.data, CODE[np.dtype(out_dtype)], n)
+    if host_jit:
+        if lib.nca_cpu_expr_host_jit(*args) != 0:
+            raise ValueError("the host JIT restatement evaluates float work types only (integer DIV is idiv)")
+    else:
+        lib.nca_cpu_expr(*args)
     return out.reshape(arrays[0].shape) if arrays else out
 
 

@@ -127,14 +127,18 @@ def test_div_by_zero_semantics(dev):
     capi, tapes, ti = _capi()
     P = capi.pack
     div = [P(capi.OP_LOAD_NCARR, 0), P(capi.OP_LOAD_NCARR, 1), P(capi.OP_DIV)]
-    x = torch.tensor([1.0, -2.0, 0.0, float("inf"), -float("inf"), 6.0], dtype=torch.float32, device=dev)
-    y = torch.tensor([0.0, 0.0, 0.0, 0.0, 0.0, 3.0], dtype=torch.float32, device=dev)
-    o = torch.empty(6, dtype=torch.float32, device=dev)
+    inf = float("inf")
+    x = torch.tensor([1.0, -2.0, 0.0, inf, -inf, 6.0, -inf, inf, -0.0], dtype=torch.float32, device=dev)
+    y = torch.tensor([0.0, 0.0, 0.0, 0.0, 0.0, 3.0, -0.0, -0.0, -0.0], dtype=torch.float32, device=dev)
+    o = torch.empty(9, dtype=torch.float32, device=dev)
     # evaluate in float32 (destination float32, linearizable: work dtype float32)
     capi.Tape(div, [capi.FLOAT32] * 2, [], capi.FLOAT32, capi.FLOAT32).launch([ti.view(x), ti.view(y)], ti.view(o))
     r = o.cpu().numpy()
-    assert np.isnan(r[0]) and np.isnan(r[1]) and np.isnan(r[2])
+    assert np.isnan(r[0]) and np.isnan(r[1]) and np.isnan(r[2]) and np.isnan(r[8])
     assert r[3] == np.inf and r[4] == -np.inf and r[5] == 2.0
+    assert r[6] == -np.inf and r[7] == np.inf       # an infinite numerator is kept, sign included
+    ref = O.expr(div, [x.cpu().numpy(), y.cpu().numpy()], [], 10, np.float32)
+    assert np.array_equal(r, ref, equal_nan=True)
 
 
 def test_strided_and_broadcast(dev):

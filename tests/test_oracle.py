@@ -164,6 +164,34 @@ def test_expression_semantics():
     assert i.tolist() == [2 * k + 1 for k in range(10)]
 
 
+def test_host_jit_division_is_ieee():
+    """The host x86 JIT's divss/divsd (x86.cc:157-169) against numpy's IEEE division, in both
+    float widths, and where it departs from the VM rule (finite / 0 only)."""
+    x = np.array([1.0, -2.5, 0.0, -0.0, np.inf, -np.inf, np.nan, 6.0, 1e-30, 3e38], dtype=np.float32)
+    y = np.array([0.0, 0.0, 0.0, -0.0, 0.0, -0.0, 0.0, 3.0, 1e30, 1e-3], dtype=np.float32)
+    tape = [P(LOAD, 0), P(LOAD, 1), P(DIV)]
+    with np.errstate(all="ignore"):
+        for ev, dt in ((10, np.float32), (11, np.float64)):
+            xs, ys = x.astype(dt), y.astype(dt)
+            jit = O.expr(tape, [xs, ys], [], ev, dt, host_jit=True)
+            assert np.array_equal(jit, xs / ys, equal_nan=True)
+            assert np.array_equal(np.signbit(jit), np.signbit(xs / ys))
+            vm = O.expr(tape, [xs, ys], [], ev, dt)
+            differ = ~((jit == vm) | (np.isnan(jit) & np.isnan(vm)))
+            # a finite nonzero numerator over a zero divisor (+-inf vs NaN), and an infinite one
+            # over -0 (IEEE flips the sign, the VM returns the numerator unchanged)
+            assert differ.tolist() == [True, True, False, False, False, True, False, False, False, False]
+            assert np.isinf(jit[:2]).all() and np.isnan(vm[:2]).all()
+            assert jit[5] == np.inf and vm[5] == -np.inf
+    # a scalar divisor in a left fold: (a - s0) / s1 with s1 = 0
+    a = np.array([1.0, 2.0, 3.0], dtype=np.float32)
+    got = O.expr([P(LOAD, 0), P(CONST, 0), P(SUB), P(CONST, 1), P(DIV)], [a], [2.0, 0.0], 10, np.float32,
+                 host_jit=True)
+    assert got[0] == -np.inf and np.isnan(got[1]) and got[2] == np.inf
+    with pytest.raises(ValueError):
+        O.expr(tape, [np.array([1], np.int32), np.array([0], np.int32)], [], 8, np.int32, host_jit=True)
+
+
 def test_c0_expression_vs_fp64():
     rng = np.random.default_rng(3)
     a = rng.uniform(-1, 1, 10000).astype(np.float32)
