@@ -253,6 +253,13 @@ __device__ __forceinline__ void nca_wave_arg(T& v, nca_i64& i) {
   }
 }
 
+// Zeros for loads that fall outside an array: a kernel reads them instead of zeroing a loaded
+// value afterwards (a select on the loaded data would make the load's wave wait for it at once).
+[[maybe_unused]] static __device__ __attribute__((aligned(16))) unsigned int nca_zero_words[16];
+// Where lanes outside an array store a vector (every wave then issues the same stores): written,
+// never read.
+[[maybe_unused]] static __device__ __attribute__((aligned(16))) unsigned int nca_sink_words[16];
+
 // 128-bit vector of N elements of T (N * sizeof(T) == 16 or smaller power of two).
 template <typename T, int N>
 struct __attribute__((aligned(sizeof(T) * N))) nca_vec { T v[N]; };

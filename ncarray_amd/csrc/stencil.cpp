@@ -374,6 +374,8 @@ struct MarchPlan {
   int rpt = 1;
   int txt = 16;  // threads across x (256 / txt thread rows)
   int pd = 1;    // register-queue march: planes loaded ahead of the one being stored
+  bool defer = false;  // register-queue march: loads kept raw (zeros read from a zero buffer) and
+                       // converted at first use; every wave issues the same loads and stores
 };
 
 // NCA_HIP_MARCH_TILE="rows,vx[,tx]" overrides the tile shape of the register-queue march (rows
@@ -431,12 +433,21 @@ static MarchPlan march_plan(const StencilProgram& p, int64_t nx = int64_t(1) << 
     m.vx = march_vx(p.src_dtype);
     m.txt = 16;
   }
-  // Two planes in flight for 4-byte sources when the z runs are long (measured 2-3% faster at
+  // 2-byte sources use deferred loads: the plane loads stay raw until first used and every
+  // wave issues the same loads and stores (zero buffer, sink), so the compiler's wait counts are
+  // exact and the prefetched planes stay in flight.  Measured on MI355X, 512^3 BF16 Laplacian
+  // with the axis-0 sum and max: 126.6 -> 108.0 us with two planes ahead (the kernel's immediate
+  // conversions waited for the newest loads in the halo waves, and the per-plane barrier made
+  // the whole block wait with them).  4-byte sources keep immediate conversions: with deferred
+  // loads the float32 kernel read HBM with more requests in flight and ran slower (194.3 ->
+  // 196.3 us with one plane ahead, 215.4 with two).
+  m.defer = m.queue && itemsize(p.src_dtype) == 2;
+  // Two planes in flight when the z runs are long (4-byte sources: measured 2-3% faster at
   // 512^3 float32, whose runs are 64 planes; at 256^3 the runs are 8 planes and the extra
-  // planes read past each run's end cost more than the deeper prefetch gains); 2-byte sources
-  // keep one (their extra queue registers cost more than it gains).  The run length is
+  // planes read past each run's end cost more than the deeper prefetch gains; 2-byte sources
+  // with deferred loads: as fast as one plane ahead with the same registers).  The run length is
   // estimated with 4 resident blocks per CU.  NCA_HIP_MARCH_PREFETCH = 1 or 2 overrides it.
-  if (m.queue && itemsize(p.src_dtype) == 4 && p.ndim == 3) {
+  if (m.queue && (itemsize(p.src_dtype) == 4 || m.defer) && p.ndim == 3) {
     const int64_t tx = static_cast<int64_t>(m.txt) * m.vx, ty = static_cast<int64_t>(256 / m.txt) * m.rpt;
     const double tiles = static_cast<double>((std::min<int64_t>(nx, int64_t(1) << 20) + tx - 1) / tx) *
                          static_cast<double>((std::min<int64_t>(ny, int64_t(1) << 20) + ty - 1) / ty);
@@ -512,6 +523,12 @@ static std::string gen_march_q(const StencilProgram& p, const StencilFuse& fz, i
   const int TXT = mp.txt, TYT = 256 / mp.txt;
   const int TX = TXT * VX, TY = TYT * RPT, LH = TY + 2 * RY, PADL = 4, LW = PADL + TX + 4;
   auto I = [](long long v) { return std::to_string(v); };
+  // deferred loads: every global load of a plane lands in a raw queue slot (out-of-range rows and
+  // columns read zeros from a zero buffer instead of being zeroed after the load) and is
+  // converted where it is first used, steps later, so no instruction waits for a load in the step
+  // that issued it (a use right after a load, in a branch of its own, waits for every load in
+  // flight, the prefetched planes included)
+  const bool dfr = mp.defer;
   std::ostringstream os;
   os << "#define VX " << VX << "\n#define TX " << TX << "\n#define TY " << TY << "\n#define LH " << LH << "\n#define LW " << LW
      << "\n#define PADL " << PADL << "\n#define RY " << RY << "\n#define RX " << RX << "\n";
@@ -574,8 +591,12 @@ static std::string gen_march_q(const StencilProgram& p, const StencilFuse& fz, i
        << "    hcrow = hrow;\n    hccol = (hc < RX) ? (PADL - RX + hc) : (PADL + TX + hc - RX);\n"
        << "  }\n";
   os << "  " << C << " cq[" << Q << "][" << RPT << "][VX];\n";
-  if (nhr > 0) os << "  " << C << " hq[" << Q << "][VX];\n";
-  if (nhc > 0) os << "  " << C << " hcq[" << Q << "];\n";
+  if (dfr) os << "  nca_vec<" << S << ", VX> rq[" << Q << "][" << RPT << "];\n";
+  if (nhr > 0) {
+    if (dfr) os << "  nca_vec<" << S << ", VX> hq[" << Q << "];\n";
+    else os << "  " << C << " hq[" << Q << "][VX];\n";
+  }
+  if (nhc > 0) os << "  " << (dfr ? S : C) << " hcq[" << Q << "];\n";
   // accumulators
   if (fz.axis0) {
     os << "  float cs[" << RPT << "][VX] = {};\n";
@@ -614,9 +635,45 @@ static std::string gen_march_q(const StencilProgram& p, const StencilFuse& fz, i
            << zero << ";\n";
     }
   };
+  // a raw row vector (nca_vec<S, VX>) `v` converted into dst[0..VX) (compute type)
+  auto conv_row = [&](const std::string& v, const std::string& dst) {
+    if (two_byte) {
+      os << "        { nca_u32x" << VX / 2 << " rw;\n        __builtin_memcpy(&rw, &" << v << ", sizeof(rw));\n";
+      for (int i = 0; i < VX / 2; ++i) {
+        if (p.src_dtype == NCA_BFLOAT16)
+          os << "        " << dst << "[" << 2 * i << "] = (" << C << ")__uint_as_float(rw[" << i << "] << 16);\n"
+             << "        " << dst << "[" << 2 * i + 1 << "] = (" << C << ")__uint_as_float(rw[" << i << "] & 0xffff0000u);\n";
+        else
+          os << "        " << dst << "[" << 2 * i << "] = (" << C << ")nca_f16_to_f32((unsigned short)(rw[" << i << "] & 0xffffu));\n"
+             << "        " << dst << "[" << 2 * i + 1 << "] = (" << C << ")nca_f16_to_f32((unsigned short)(rw[" << i << "] >> 16));\n";
+      }
+      os << "        }\n";
+    } else {
+      for (int j = 0; j < VX; ++j)
+        os << "        " << dst << "[" << j << "] = nca_cvt<" << C << ", " << S << ">::from(" << v << ".v[" << j << "]);\n";
+    }
+  };
+  // raw queue slot q -> converted slot q (deferred loads)
+  auto unpack = [&](int q) {
+    for (int r = 0; r < RPT; ++r) conv_row("rq[" + I(q) + "][" + I(r) + "]", "cq[" + I(q) + "][" + I(r) + "]");
+  };
+  const std::string zsrc = "reinterpret_cast<const " + S + "*>(nca_zero_words)";
   auto gload = [&](int q, const std::string& pz, const std::string& off) {
     os << "    {\n      const bool zin = (unsigned)(" << pz << ") < (unsigned)NZi;\n"
        << "      const " << S << "* __restrict__ pl = src + (zin ? (" << off << ") : (size_t)0);\n";
+    if (dfr) {
+      const std::string VT = "nca_vec<" + S + ", VX>";
+      for (int r = 0; r < RPT; ++r)
+        os << "      rq[" << q << "][" << r << "] = *reinterpret_cast<const " << VT << "*>((zin && valid" << r << ") ? pl + ioff"
+           << r << " : " << zsrc << ");\n";
+      // every wave issues the same loads (lanes without halo work read zeros): the compiler's
+      // wait counts are then the same on every path, so no wave waits for a newer load
+      if (nhr > 0)
+        os << "      hq[" << q << "] = *reinterpret_cast<const " << VT << "*>((zin && hrok) ? pl + hroff : " << zsrc << ");\n";
+      if (nhc > 0) os << "      hcq[" << q << "] = *((zin && hcok) ? pl + hcoff : " << zsrc << ");\n";
+      os << "    }\n";
+      return;
+    }
     for (int r = 0; r < RPT; ++r) {
       os << "      { const bool ok = zin && valid" << r << ";\n";
       load_row("ok", "pl + ioff" + I(r), "cq[" + I(q) + "][" + I(r) + "]");
@@ -641,11 +698,16 @@ static std::string gen_march_q(const StencilProgram& p, const StencilFuse& fz, i
     }
     if (nhr > 0) {
       os << "    if (hid < " << nhr << ") { lvec w;\n";
-      for (int j = 0; j < VX; ++j) os << "      w.v[" << j << "] = hq[" << q << "][" << j << "];\n";
+      if (dfr) {
+        conv_row("hq[" + I(q) + "]", "w.v");
+      } else {
+        for (int j = 0; j < VX; ++j) os << "      w.v[" << j << "] = hq[" << q << "][" << j << "];\n";
+      }
       os << "      *reinterpret_cast<lvec*>(&lds[" << buf << "][hrrow][hrcol]) = w; }\n";
     }
     if (nhc > 0)
-      os << "    if (hid >= " << nhr << " && hid < " << nhr + nhc << ") lds[" << buf << "][hcrow][hccol] = hcq[" << q << "];\n";
+      os << "    if (hid >= " << nhr << " && hid < " << nhr + nhc << ") lds[" << buf << "][hcrow][hccol] = "
+         << (dfr ? "nca_cvt<" + C + ", " + S + ">::from(hcq[" + I(q) + "])" : "hcq[" + I(q) + "]") << ";\n";
   };
   // rows (dy) read from LDS, and how far x-offsets reach left/right of the VX columns
   std::map<int64_t, std::pair<int, int>> reach;  // dy -> (left, right) element counts
@@ -676,9 +738,13 @@ static std::string gen_march_q(const StencilProgram& p, const StencilFuse& fz, i
       }
       for (size_t jx = 0; jx < xdt.size(); ++jx) {
         const std::string E = ctype_of(xdt[jx]);
-        os << "      nca_vec<" << E << ", VX> X" << jx << " = {};\n"
-           << "      if (valid" << r << ") X" << jx << " = *reinterpret_cast<const nca_vec<" << E << ", VX>*>(e" << jx
-           << " + soff + ioff" << r << ");\n";
+        if (dfr)
+          os << "      const nca_vec<" << E << ", VX> X" << jx << " = *reinterpret_cast<const nca_vec<" << E << ", VX>*>(valid" << r
+             << " ? e" << jx << " + soff + ioff" << r << " : reinterpret_cast<const " << E << "*>(nca_zero_words));\n";
+        else
+          os << "      nca_vec<" << E << ", VX> X" << jx << " = {};\n"
+             << "      if (valid" << r << ") X" << jx << " = *reinterpret_cast<const nca_vec<" << E << ", VX>*>(e" << jx
+             << " + soff + ioff" << r << ");\n";
       }
       if (vj) {
         for (size_t m = 0; m < vj->out_dtypes.size(); ++m)
@@ -754,7 +820,11 @@ static std::string gen_march_q(const StencilProgram& p, const StencilFuse& fz, i
         if (fz.fuse & NCA_FUSE_MIN) os << "        acc_mn = nca_minop(acc_mn, " << tree("nca_minnum") << ");\n";
         os << "      }\n";
       }
-      if (fz.with_dst)
+      // deferred mode: lanes outside the array store into a sink (every wave issues the same stores)
+      if (fz.with_dst && dfr)
+        os << "      nca_store_nt(reinterpret_cast<nca_vec<" << D << ", VX>*>(valid" << r << " ? dst + soff + ioff" << r
+           << " : reinterpret_cast<" << D << "*>(nca_sink_words)), res);\n";
+      else if (fz.with_dst)
         os << "      if (valid" << r << ") nca_store_nt(reinterpret_cast<nca_vec<" << D << ", VX>*>(dst + soff + ioff" << r
            << "), res);\n";
       os << "    }\n";
@@ -766,6 +836,9 @@ static std::string gen_march_q(const StencilProgram& p, const StencilFuse& fz, i
     const std::string pz = "zb - " + I(RZ) + " + " + I(m);
     gload(m, pz, "(size_t)(" + pz + ") * plane");
   }
+  // deferred loads: planes zb-RZ .. zb+RZ-1 converted now, plane z+RZ at the start of step z
+  if (dfr)
+    for (int m = 0; m < 2 * RZ; ++m) unpack(m);
   // steady state: step k of an unrolled group handles plane z = z' + k; it stores plane z
   // (slot (k + RZ) % Q) into the LDS buffer of its parity, loads plane z + RZ + PD into the
   // slot just freed ((k - 1) mod Q) and computes plane z.  With Q even the parity is k's.
@@ -775,6 +848,7 @@ static std::string gen_march_q(const StencilProgram& p, const StencilFuse& fz, i
   for (int k = 0; k < Q; ++k) {
     if (k > 0) os << "    if (z + " << k << " >= ze) break;\n";
     const std::string pb = Q % 2 == 0 ? I(k & 1) : "((z - zb + " + I(k) + ") & 1)";
+    if (dfr) unpack((k + 2 * RZ) % Q);
     lstore((k + RZ) % Q, pb);
     os << "    __syncthreads();\n";
     gload((k + Q - 1) % Q, "z + " + I(k + RZ + PD), "loff");
@@ -2316,7 +2390,13 @@ static void stencil_expr_vjp_impl(uint64_t stencil_handle, uint64_t expr_handle,
       outs.push_back(grad_extras[j].data);
     }
     const int64_t NZ = w->shape[0], NY = w->shape[1], NX = w->shape[2];
-    const MarchPlan mp = march_plan(p, NX, NY, NZ, current_device().n_cus);
+    MarchPlan mp = march_plan(p, NX, NY, NZ, current_device().n_cus);
+    if (mp.defer) {
+      // the backward march keeps immediate conversions and one plane ahead (at 256^3 BF16 the
+      // deferred form made the loss-and-gradient step 5% slower)
+      mp.defer = false;
+      mp.pd = 1;
+    }
     const MarchGeom g = geom_of(p);
     NCA_REQUIRE(mp.queue && g.rz <= 3 && g.ry <= 3 && g.rx <= 3,
                 "the stencil-expression vjp needs a stencil whose z offsets have no x or y part (radius <= 3)");
