@@ -344,6 +344,18 @@ def cold_device_ms(torch, capi, fn, reps=10):
     return float(np.median(per))
 
 
+def f32_ulp(torch, ref64):
+    """The float32 ulp at each value of a float64 tensor (spacing above |fl32(ref)|)."""
+    r = ref64.float().abs()
+    return (torch.nextafter(r, torch.full_like(r, float("inf"))) - r).double()
+
+
+def sum_errors(torch, S, S64, abs_sum):
+    """|S - S64| / |S64| and |S - S64| / sum|x| for a float32 (or float64) sum S."""
+    d = abs(float(S) - float(S64))
+    return {"rel_err_vs_fp64": d / abs(float(S64)) if S64 != 0 else d, "err_over_sum_abs": d / float(abs_sum)}
+
+
 def bench_c0(torch, capi, tapes, ti, dev, sptr, time_loop):
     """out = a*b + exp(-a*a) - 0.5*b on 2^22 float32, then sum(out): fused in one kernel
     (out materialised), and the expression alone.  The 50 MB working set fits the 256 MB
@@ -387,6 +399,26 @@ def bench_c0(torch, capi, tapes, ti, dev, sptr, time_loop):
             e = A * B + nca.exp(-(A * A)) - B * 0.5
         return e.sum()  # a host scalar: waits for the result
 
+    # errors against an FP64 evaluation of the same inputs (the fused call's out and sum)
+    fused()
+    torch.cuda.synchronize()
+    a64, b64 = a.double(), b.double()
+    ref64 = a64 * b64 + torch.exp(-a64 * a64) - 0.5 * b64
+    diff = (out.double() - ref64).abs()
+    # a*b + exp(-a*a) - 0.5*b cancels to near zero for some inputs, where ulp and relative error
+    # say nothing about the evaluation: errors are given relative to the terms' magnitudes (the
+    # bound the tests state is 2 eps of that), and in ulp where the result keeps at least a
+    # quarter of that magnitude
+    scale = (a64 * b64).abs() + torch.exp(-a64 * a64) + 0.5 * b64.abs()
+    keep = ref64.abs() >= 0.25 * scale
+    eps = float(np.finfo(np.float32).eps)
+    errors = {"expr_max_err_in_eps_of_term_magnitude": round(float((diff / scale).max()) / eps, 4),
+              "expr_max_ulp_without_cancellation": round(float((diff[keep] / f32_ulp(torch, ref64[keep])).max()), 3),
+              "expr_max_rel_err_without_cancellation": float((diff[keep] / ref64[keep].abs()).max()),
+              "sum_vs_fp64_of_exact_values": sum_errors(torch, s.item(), ref64.sum().item(), ref64.abs().sum().item()),
+              "sum_vs_fp64_of_stored_values": sum_errors(torch, s.item(), out.double().sum().item(),
+                                                         out.double().abs().sum().item())}
+    del a64, b64, ref64, diff, scale, keep
     cold_f = cold_device_ms(torch, capi, fused)
     cold_o = cold_device_ms(torch, capi, only)
     pe, _ = time_loop(py_eager, 30, 5)
@@ -402,6 +434,7 @@ def bench_c0(torch, capi, tapes, ti, dev, sptr, time_loop):
             "graph_fused_ms": round(gfm, 4), "graph_fused_GBps": gbps(12 * n, gfm),
             "graph_expr_only_ms": round(gom, 4), "graph_expr_only_GBps": gbps(12 * n, gom),
             "graph_sum_equals_eager": bool(float(s.item()) == eager_sum),
+            "errors_vs_fp64": errors,
             "note": "cache-resident (50 MB < 256 MB MALL); bytes = read a, b + write out; graph_* = "
                     "20 steps per hipGraph replay (no host launch overhead); hbm_cold_* = device time of one "
                     "call after reading 512 MiB flushed the Infinity Cache; python_* = the expression "
@@ -444,6 +477,22 @@ def bench_c2(torch, capi, tapes, ti, dev, sptr, time_loop):
     for name, kms in recs:
         per.setdefault(name, []).append(kms)
     n = F * ny * nx
+    # counts and sums against a float64 scatter of the same frames by the same map
+    fwd()
+    torch.cuda.synchronize()
+    idx = bm.view(-1).long()
+    valid = (idx >= 0) & (idx < nb)
+    s64 = torch.zeros((F, nb), dtype=torch.float64, device=dev)
+    c64 = torch.zeros((F, nb), dtype=torch.int64, device=dev)
+    for f0 in range(0, F, 32):
+        blk = fr[f0:f0 + 32].reshape(fr[f0:f0 + 32].shape[0], -1)[:, valid]
+        s64[f0:f0 + 32].index_add_(1, idx[valid], blk.double())
+        c64[f0:f0 + 32].index_add_(1, idx[valid], (blk != 0).long())
+    err = (sums.double() - s64).abs()
+    c2_errors = {"counts_bit_exact_vs_fp64_scatter": bool(torch.equal(counts.long(), c64)),
+                 "sums_max_rel_err_vs_fp64": float((err / s64.abs().clamp_min(1e-300)).max()),
+                 "sums_max_ulp": round(float((err / f32_ulp(torch, s64)).max()), 3)}
+    del s64, c64, err
     fwd_bytes = 4 * n + 4 * ny * nx + 2 * 4 * F * nb
     bwd_bytes = 4 * n + 4 * ny * nx + 4 * F * nb
     return {"fwd_ms": round(float(np.mean(fm)), 4), "fwd_GBps": gbps(fwd_bytes, np.mean(fm)),
@@ -453,7 +502,8 @@ def bench_c2(torch, capi, tapes, ti, dev, sptr, time_loop):
             "histogram_kernel_frac_of_8TBps": round((4 * n + 4 * ny * nx) / (np.mean(per["nca_bin"]) * 1e-3) / 1e9
                                                     / HBM_PEAK_GBPS, 4),
             "fwd_bytes": fwd_bytes, "bwd_bytes": bwd_bytes,
-            "counts_total_equals_nonzero_pixels": bool(int(counts.sum()) == int((fr != 0).sum()))}
+            "counts_total_equals_nonzero_pixels": bool(int(counts.sum()) == int((fr != 0).sum())),
+            "errors_vs_fp64": c2_errors}
 
 
 def bench_c3(torch, capi, tapes, ti, dev, sptr, time_loop):
@@ -505,6 +555,7 @@ def bench_c3(torch, capi, tapes, ti, dev, sptr, time_loop):
         cold_vg = cold_device_ms(torch, capi, lambda: A.laplacian_c0_value_and_grad(xd, yd))
         with torch.no_grad():
             loss_eager = A.laplacian_c0_loss(x, y).clone()
+        errs = c3_errors(torch, xd, yd, loss_eager, vg)
         gfm, _ = graph_loop(torch, fwd, 10, 10)  # forward only (launched from this thread)
         # forward + backward without torch's autograd (the same kernels, launched from this
         # thread), 10 steps per hipGraph replay
@@ -529,7 +580,8 @@ def bench_c3(torch, capi, tapes, ti, dev, sptr, time_loop):
                     "hbm_cold_fwd_ms": round(cold_fwd, 4), "hbm_cold_fwd_frac_of_8TBps": frac(fwd_bytes, cold_fwd),
                     "hbm_cold_value_and_grad_ms": round(cold_vg, 4),
                     "hbm_cold_value_and_grad_frac_of_8TBps": frac(fb_bytes, cold_vg),
-                    "loss": float(loss_eager), "loss_value_and_grad": float(vg[0])}
+                    "loss": float(loss_eager), "loss_value_and_grad": float(vg[0]),
+                    "errors_vs_fp64": errs}
     res["note"] = ("bytes are algorithmic (each input read once, each output written once); the backward is "
                    "one march pass (the Laplacian recomputed, both gradients of the C0 expression written) and "
                    "the transposed stencil; fwd_bwd_ms is host-bound (torch autograd and Python per step); "
@@ -541,6 +593,32 @@ def bench_c3(torch, capi, tapes, ti, dev, sptr, time_loop):
                    "cache-resident, except hbm_cold_* (device time after reading 512 MiB flushed the "
                    "Infinity Cache)")
     return res
+
+
+def c3_errors(torch, x, y, loss, vg):
+    """The C3 loss and gradients against FP64 on the same (stored, e.g. bf16-rounded) inputs:
+    a = Laplacian(x) with zero boundary, loss = sum(a*y + exp(-a*a) - 0.5*y), dy = a - 0.5,
+    dx = Laplacian(y - 2 a exp(-a*a)) (the zero-boundary Laplacian is its own adjoint)."""
+    import torch.nn.functional as F
+
+    def lap(u):
+        p = F.pad(u, (1, 1, 1, 1, 1, 1))
+        return (p[:-2, 1:-1, 1:-1] + p[2:, 1:-1, 1:-1] + p[1:-1, :-2, 1:-1] + p[1:-1, 2:, 1:-1]
+                + p[1:-1, 1:-1, :-2] + p[1:-1, 1:-1, 2:] - 6.0 * u)
+
+    x64, y64 = x.double(), y.double()
+    a = lap(x64)
+    e = torch.exp(-a * a)
+    terms = a * y64 + e - 0.5 * y64
+    loss64 = terms.sum().item()
+    dy64 = a - 0.5
+    dx64 = lap(y64 - 2.0 * a * e)
+    _, dx, dy = vg
+    out = {"loss": sum_errors(torch, float(loss), loss64, terms.abs().sum().item()),
+           "loss_value_and_grad": sum_errors(torch, float(vg[0]), loss64, terms.abs().sum().item()),
+           "dx_max_err_over_max_abs": float((dx.double() - dx64).abs().max() / dx64.abs().max()),
+           "dy_max_err_over_max_abs": float((dy.double() - dy64).abs().max() / dy64.abs().max())}
+    return out
 
 
 def bench_cp(torch, capi, tapes, ti, dev, sptr, time_loop):
@@ -599,10 +677,11 @@ def bench_cp(torch, capi, tapes, ti, dev, sptr, time_loop):
 
 
 def bench_st(torch, capi, tapes, ti, dev, sptr, time_loop):
-    """The tile stencil kernel: a 5-point 2-D Laplacian on a 16384^2 float32 grid (zero boundary),
-    2x2 pixel binning (step 2) of 256 frames of 1024^2 float32 into 512^2, and the 3-D Laplacian
-    of a 512^3 float32 grid whose 512 planes are reached through a table of plane pointers (an NC
-    pointer axis 0).  Kernel time from the library's HIP events; algorithmic bytes = source read
+    """Stencils off the dense 3-D path: a 5-point 2-D Laplacian on a 16384^2 float32 grid (zero
+    boundary) and 2x2 pixel binning (step 2) of 256 frames of 1024^2 float32 into 512^2 (the tile
+    kernel), and the 3-D Laplacian of a 512^3 float32 grid whose 512 planes are reached through a
+    table of plane pointers (an NC pointer axis 0: the register-queue march reading each plane's
+    pointer once).  Kernel time from the library's HIP events; algorithmic bytes = source read
     once + destination written once."""
     P = capi.pack
     five = [P(capi.OP_LOAD_NCARR, 1), P(capi.OP_LOAD_NCARR, 2), P(capi.OP_ADD), P(capi.OP_LOAD_NCARR, 3),
@@ -628,10 +707,10 @@ def bench_st(torch, capi, tapes, ti, dev, sptr, time_loop):
         for _ in range(10):
             fn()
         torch.cuda.synchronize()
-        recs = [k for k in capi.profile_read() if k[0] == "nca_stencil_tile"]
+        recs = capi.profile_read()   # one stencil kernel per call (no fused reductions here)
         capi.profile_enable(False)
         ms = float(np.median([k for _, k in recs]))
-        res[name] = {"kernel": "nca_stencil_tile", "kernel_ms": round(ms, 4), "GBps": gbps(nbytes, ms),
+        res[name] = {"kernel": recs[0][0], "kernel_ms": round(ms, 4), "GBps": gbps(nbytes, ms),
                      "frac_of_8TBps": round(nbytes / (ms * 1e-3) / 1e9 / HBM_PEAK_GBPS, 4)}
 
     x = torch.randn((16384, 16384), generator=g, device=dev)
@@ -702,6 +781,21 @@ def bench_c4(torch, capi, tapes, ti, dev, sptr, time_loop):
     nbytes = 12 * total
     same_max = bool(torch.equal(maxs, rb.maxs))
     rel = float(((sums.double() - rb.sums.double()).abs() / (rb.sums.double().abs() + 1.0)).max())
+    # the batch's sums against float64 sums of its stored outputs, and its max against the stored
+    # outputs' max (exact), per array
+    rb.launch(scalars=(0.5,), stream=sptr)
+    torch.cuda.synchronize()
+    worst_rel = worst_abs = 0.0
+    max_exact = True
+    for i in range(len(sizes)):
+        oi = o[i].double()
+        s64, sa = oi.sum().item(), oi.abs().sum().item()
+        d = abs(float(rb.sums[i]) - s64)
+        worst_rel = max(worst_rel, d / abs(s64) if s64 != 0 else d)
+        worst_abs = max(worst_abs, d / sa if sa else d)
+        max_exact = max_exact and float(rb.maxs[i]) == float(o[i].max())
+    c4_errors = {"sum_max_rel_err_vs_fp64": worst_rel, "sum_max_err_over_sum_abs": worst_abs,
+                 "max_equals_max_of_stored": bool(max_exact)}
     rb.close()
     return {"n_arrays": len(sizes), "total_elements": total, "total_GB_moved": round(nbytes / 1e9, 3),
             "batched_default_ms": round(float(np.mean(dm)), 3), "default_config": cfg_default,
@@ -709,6 +803,7 @@ def bench_c4(torch, capi, tapes, ti, dev, sptr, time_loop):
             "batched_graph_ms": round(float(np.mean(gm)), 3), "batched_graph_GBps": gbps(nbytes, np.mean(gm)),
             "per_array_loop_ms": round(float(np.mean(lm)), 3), "per_array_loop_GBps": gbps(nbytes, np.mean(lm)),
             "loop_vs_batch_max_equal": same_max, "loop_vs_batch_sum_max_rel_diff": rel,
+            "errors_vs_fp64": c4_errors,
             "note": ("sizes follow the stated distribution: ~1.97e9 elements (~23.6 GB moved), not 2.5 GB; "
                      "the per-array loop calls the C ABI from Python once per array (kernel + combine launch)")}
 

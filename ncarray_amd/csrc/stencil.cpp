@@ -111,6 +111,10 @@ struct StencilFuse {
   int mode = NCA_ACCUM_PLAIN;
   bool with_dst = true;
   int32_t axis0_dtype = NCA_FLOAT32;
+  // the register-queue march over plane tables: src (dst) points to a table of plane pointers
+  // (an NC pointer axis 0 or an SO view indirect at axis 0) whose planes are C-contiguous
+  bool src_tab = false;
+  bool dst_tab = false;
 };
 
 // Kernel symbol of a march variant: storage types and fused reductions are part of the name, so
@@ -124,6 +128,7 @@ static std::string march_name(const StencilProgram& p, const StencilFuse& fz, in
   if (fz.fuse & NCA_FUSE_MIN) n += "_min";
   if (!fz.with_dst) n += "_nodst";
   if (!p.extra_dtypes.empty()) n += "_x" + std::to_string(p.extra_dtypes.size());
+  if (fz.src_tab || fz.dst_tab) n += std::string("_planes") + (fz.src_tab ? "s" : "") + (fz.dst_tab ? "d" : "");
   return n;
 }
 
@@ -544,6 +549,8 @@ static std::string gen_march_q(const StencilProgram& p, const StencilFuse& fz, i
     os << ", " << D << "* __restrict__ dst, nca_i64 NZ, nca_i64 NY, nca_i64 NX, int ntx, int nty, int zlen"
        << ", double* __restrict__ colpart, void* __restrict__ colout, void* __restrict__ psum, void* __restrict__ pmax"
        << ", void* __restrict__ pmin";
+    // plane tables: plane z of src starts at *(src + z * stab) + ssub (bytes), of dst likewise
+    if (fz.src_tab || fz.dst_tab) os << ", nca_i64 stab, nca_i64 ssub, nca_i64 dtab, nca_i64 dsub";
   }
   for (size_t i = 0; i < t.scalar_dtypes.size(); ++i) os << ", " << C << " s" << i;
   if (vj) {
@@ -552,6 +559,13 @@ static std::string gen_march_q(const StencilProgram& p, const StencilFuse& fz, i
   }
   os << ") {\n";
   if (vj) os << "  const " << C << " gsc = gin ? *gin : (" << C << ")1;\n";
+  const bool stab = !vj && fz.src_tab, dtab = !vj && fz.dst_tab;
+  if (stab)
+    os << "  auto splane = [&](int pz) { return reinterpret_cast<const " << S
+       << "*>(nca_chase(reinterpret_cast<const char*>(src) + (nca_i64)pz * stab, ssub)); };\n";
+  if (dtab)
+    os << "  auto dplane = [&](int pz) { return reinterpret_cast<" << D
+       << "*>(nca_chase(reinterpret_cast<char*>(dst) + (nca_i64)pz * dtab, dsub)); };\n";
   // LDS rows are 16-byte aligned (not aligned to a whole VX vector when VX * sizeof(C) > 16)
   os << "  struct __attribute__((aligned(16))) lvec { " << C << " v[VX]; };\n";
   os << "  __shared__ __attribute__((aligned(16))) " << C << " lds[2][LH][LW];\n";
@@ -613,10 +627,13 @@ static std::string gen_march_q(const StencilProgram& p, const StencilFuse& fz, i
   // one row vector of VX source values into dst[0..VX) (compute type), zero when !ok.  2-byte
   // storage types are read as four 32-bit words (zeroed per word) and unpacked by shifts.
   const bool two_byte = itemsize(p.src_dtype) == 2;
+  // the address a lane outside the array reads (its value is discarded): the source, or the zero
+  // buffer when src is a table of plane pointers
+  const std::string fbk = stab ? "reinterpret_cast<const " + S + "*>(nca_zero_words)" : std::string("src");
   auto load_row = [&](const std::string& ok, const std::string& addr, const std::string& dst) {
     if (two_byte) {
       const bool bf = p.src_dtype == NCA_BFLOAT16;
-      os << "        nca_u32x4 w = *reinterpret_cast<const nca_u32x4*>(" << ok << " ? " << addr << " : src);\n"
+      os << "        nca_u32x4 w = *reinterpret_cast<const nca_u32x4*>(" << ok << " ? " << addr << " : " << fbk << ");\n"
          << "        w = " << ok << " ? w : (nca_u32x4)(0u);\n";
       for (int i = 0; i < VX / 2; ++i) {
         if (bf) {
@@ -629,7 +646,7 @@ static std::string gen_march_q(const StencilProgram& p, const StencilFuse& fz, i
       }
     } else {
       os << "        const nca_vec<" << S << ", VX> v = *reinterpret_cast<const nca_vec<" << S << ", VX>*>(" << ok << " ? " << addr
-         << " : src);\n";
+         << " : " << fbk << ");\n";
       for (int j = 0; j < VX; ++j)
         os << "        " << dst << "[" << j << "] = " << ok << " ? nca_cvt<" << C << ", " << S << ">::from(v.v[" << j << "]) : "
            << zero << ";\n";
@@ -659,8 +676,12 @@ static std::string gen_march_q(const StencilProgram& p, const StencilFuse& fz, i
   };
   const std::string zsrc = "reinterpret_cast<const " + S + "*>(nca_zero_words)";
   auto gload = [&](int q, const std::string& pz, const std::string& off) {
-    os << "    {\n      const bool zin = (unsigned)(" << pz << ") < (unsigned)NZi;\n"
-       << "      const " << S << "* __restrict__ pl = src + (zin ? (" << off << ") : (size_t)0);\n";
+    os << "    {\n      const bool zin = (unsigned)(" << pz << ") < (unsigned)NZi;\n";
+    if (stab)
+      os << "      const " << S << "* __restrict__ pl = zin ? splane(" << pz << ") : reinterpret_cast<const " << S
+         << "*>(nca_zero_words);\n";
+    else
+      os << "      const " << S << "* __restrict__ pl = src + (zin ? (" << off << ") : (size_t)0);\n";
     if (dfr) {
       const std::string VT = "nca_vec<" + S + ", VX>";
       for (int r = 0; r < RPT; ++r)
@@ -686,7 +707,7 @@ static std::string gen_march_q(const StencilProgram& p, const StencilFuse& fz, i
     }
     if (nhc > 0)
       os << "      if (hid >= " << nhr << " && hid < " << nhr + nhc << ") { const bool ok = zin && hcok;\n"
-         << "        const " << C << " v = nca_ld<" << C << ">(ok ? pl + hcoff : src);\n"
+         << "        const " << C << " v = nca_ld<" << C << ">(ok ? pl + hcoff : " << fbk << ");\n"
          << "        hcq[" << q << "] = ok ? v : " << zero << "; }\n";
     os << "    }\n";
   };
@@ -723,6 +744,9 @@ static std::string gen_march_q(const StencilProgram& p, const StencilFuse& fz, i
   auto compute = [&](int k, const std::string& buf) {
     // slot of plane z + k + dz
     auto slot = [&](int64_t dz) { return static_cast<int>(((k + RZ + dz) % Q + Q) % Q); };
+    // where plane z + k of the destination starts: dst + soff, or the plane table's pointer
+    const std::string dbase = dtab ? "dpl" + I(k) : std::string("dst + soff");
+    if (dtab && fz.with_dst) os << "    " << D << "* const dpl" << k << " = dplane(z + " << k << ");\n";
     for (int r = 0; r < RPT; ++r) {
       os << "    {\n";
       for (int64_t dy : vec_rows)
@@ -822,10 +846,10 @@ static std::string gen_march_q(const StencilProgram& p, const StencilFuse& fz, i
       }
       // deferred mode: lanes outside the array store into a sink (every wave issues the same stores)
       if (fz.with_dst && dfr)
-        os << "      nca_store_nt(reinterpret_cast<nca_vec<" << D << ", VX>*>(valid" << r << " ? dst + soff + ioff" << r
+        os << "      nca_store_nt(reinterpret_cast<nca_vec<" << D << ", VX>*>(valid" << r << " ? " << dbase << " + ioff" << r
            << " : reinterpret_cast<" << D << "*>(nca_sink_words)), res);\n";
       else if (fz.with_dst)
-        os << "      if (valid" << r << ") nca_store_nt(reinterpret_cast<nca_vec<" << D << ", VX>*>(dst + soff + ioff" << r
+        os << "      if (valid" << r << ") nca_store_nt(reinterpret_cast<nca_vec<" << D << ", VX>*>(" << dbase << " + ioff" << r
            << "), res);\n";
       os << "    }\n";
     }
@@ -1955,16 +1979,31 @@ static void stencil_apply_impl(uint64_t handle, const nca_view* src, const nca_v
   bool unit = true;
   for (int d = 0; d < p.ndim; ++d) unit = unit && step[d] == 1;
   MarchGeom g = p.ndim == 3 ? geom_of(p) : MarchGeom{};
-  bool march = p.ndim == 3 && unit && !p.has_ext && g.rz <= 3 && g.ry <= 3 && g.rx <= 3 && is_c_contiguous(*src) &&
-               dst->shape[1] * dst->shape[2] < (int64_t(1) << 31) &&
-               same_shape(*src, *dst) && (!with_dst || is_c_contiguous(*dst)) &&
-               (itemsize(src->dtype) == 4 || itemsize(src->dtype) == 2);
   const MarchPlan mplan = p.ndim == 3 ? march_plan(p, dst->shape[2], dst->shape[1], dst->shape[0], current_device().n_cus)
                                       : march_plan(p, 0);
   const int vx = mplan.vx;
-  march = march && dst->shape[2] % vx == 0 &&
-          reinterpret_cast<uintptr_t>(src->data) % (vx * itemsize(src->dtype)) == 0 &&
-          (!with_dst || reinterpret_cast<uintptr_t>(dst->data) % (vx * itemsize(dst->dtype)) == 0);
+  // a 3-D view the march reads (writes) plane by plane: C-contiguous with aligned vectors, or (for
+  // the register-queue march) a table of plane pointers -- axis 0 the only indirect axis, each
+  // plane C-contiguous, the byte offset after the pointer a multiple of the vector size (the
+  // plane pointers themselves are the caller's allocations; an unaligned one is still read
+  // correctly, global vector accesses need no alignment on this hardware)
+  auto plane_table = [&](const nca_view& v, Chain* c) {
+    if (v.ndim != 3 || !view_indirect(v) || !mplan.queue) return false;
+    *c = chain_of(v);
+    const int64_t it = itemsize(v.dtype);
+    return c->axes.size() == 1 && c->axes[0] == 0 && c->strides[2] == it && (v.shape[1] == 1 || c->strides[1] == v.shape[2] * it) &&
+           c->subs[0] % (vx * it) == 0;
+  };
+  auto dense_ok = [&](const nca_view& v) {
+    return is_c_contiguous(v) && reinterpret_cast<uintptr_t>(v.data) % (vx * itemsize(v.dtype)) == 0;
+  };
+  Chain stab_c, dtab_c;
+  const bool src_tab = !dense_ok(*src) && plane_table(*src, &stab_c);
+  const bool dst_tab = with_dst && !dense_ok(*dst) && plane_table(*dst, &dtab_c);
+  bool march = p.ndim == 3 && unit && !p.has_ext && g.rz <= 3 && g.ry <= 3 && g.rx <= 3 &&
+               dst->shape[1] * dst->shape[2] < (int64_t(1) << 31) && same_shape(*src, *dst) &&
+               (itemsize(src->dtype) == 4 || itemsize(src->dtype) == 2) && dst->shape[2] % vx == 0 &&
+               (dense_ok(*src) || src_tab) && (!with_dst || dense_ok(*dst) || dst_tab);
   for (int j = 0; j < n_extra && march; ++j)
     march = is_c_contiguous(extras[j]) && reinterpret_cast<uintptr_t>(extras[j].data) % (vx * itemsize(extras[j].dtype)) == 0;
   if (axis0) {
@@ -2003,9 +2042,12 @@ static void stencil_apply_impl(uint64_t handle, const nca_view* src, const nca_v
     fz.mode = mode;
     fz.with_dst = with_dst;
     fz.axis0_dtype = axis0 ? axis0->dtype : NCA_FLOAT32;
+    fz.src_tab = src_tab;
+    fz.dst_tab = dst_tab;
     std::string key = "m|" + std::to_string(fz.axis0) + std::to_string(fz.axis0_dtype) + "|" + std::to_string(fuse) + "|" +
                       std::to_string(mode) + "|" + std::to_string(with_dst) + "|" + std::to_string(mp.queue) + "," +
-                      std::to_string(mp.vx) + "," + std::to_string(mp.rpt) + "," + std::to_string(mp.txt) + "," + std::to_string(mp.pd);
+                      std::to_string(mp.vx) + "," + std::to_string(mp.rpt) + "," + std::to_string(mp.txt) + "," + std::to_string(mp.pd) +
+                      "|t" + std::to_string(src_tab) + std::to_string(dst_tab);
     const std::string kname = march_name(p, fz, dst->dtype);
     hipFunction_t fn = stencil_kernel(
         p, key, [&]() { return mp.queue ? gen_march_q(p, fz, dst->dtype, mp) : gen_march(p, fz, dst->dtype); }, kname,
@@ -2037,10 +2079,10 @@ static void stencil_apply_impl(uint64_t handle, const nca_view* src, const nca_v
     Workspace& w = fixed ? fw : workspace(s, static_cast<size_t>(blocks) * 24, colbytes);
     std::deque<const void*> ptrs;
     std::vector<void*> params;
-    ptrs.push_back(src->data);
+    ptrs.push_back(src_tab ? reinterpret_cast<const void*>(stab_c.base) : src->data);
     params.push_back(&ptrs.back());
     for (int j = 0; j < n_extra; ++j) { ptrs.push_back(extras[j].data); params.push_back(&ptrs.back()); }
-    ptrs.push_back(with_dst ? dst->data : nullptr);
+    ptrs.push_back(!with_dst ? nullptr : dst_tab ? reinterpret_cast<const void*>(dtab_c.base) : dst->data);
     params.push_back(&ptrs.back());
     int64_t dims[3] = {NZ, NY, NX};
     params.push_back(&dims[0]);
@@ -2060,6 +2102,10 @@ static void stencil_apply_impl(uint64_t handle, const nca_view* src, const nca_v
     params.push_back(&psum);
     params.push_back(&pmax);
     params.push_back(&pmin);
+    int64_t tabs[4] = {src_tab ? stab_c.strides[0] : 0, src_tab ? stab_c.subs[0] : 0, dst_tab ? dtab_c.strides[0] : 0,
+                       dst_tab ? dtab_c.subs[0] : 0};
+    if (src_tab || dst_tab)
+      for (auto& v : tabs) params.push_back(&v);
     for (auto& x : sc.slots) params.push_back(&x);
     {
       ProfScope ps(kname.c_str(), s);
