@@ -679,44 +679,59 @@ static std::string gen_transpose(const Tape& t, const Operands& ops, const Trans
      << "  const nca_i64 ntiles = (n / (nA * nL)) * tA * tL;\n"
      << (tp.pair ? "  const int sx = (threadIdx.x % (TA / 2)) * 2, sy = threadIdx.x / (TA / 2), ex = (threadIdx.x % (TL / 2)) * 2, "
                    "ey = threadIdx.x / (TL / 2);\n"
-                 : "  const int sx = threadIdx.x % TA, sy = threadIdx.x / TA, ex = threadIdx.x % TL, ey = threadIdx.x / TL;\n")
-     << "  for (nca_i64 tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {\n"
-     << "    nca_i64 rem = tile;\n"
-     << "    const nca_i64 l0 = (rem % tL) * TL;\n    rem /= tL;\n"
-     << "    const nca_i64 a0 = (rem % tA) * TA;\n    rem /= tA;\n";
+                 : "  const int sx = threadIdx.x % TA, sy = threadIdx.x / TA, ex = threadIdx.x % TL, ey = threadIdx.x / TL;\n");
+  // a tile's origin and operand offsets (locate: na0, nl0, no<k>, noo) and its staged operands'
+  // loads (fetch)
   for (int k : ops.unique)
-    if (!ops.bcast[k]) os << "    nca_i64 o" << k << " = 0;\n";
-  os << "    nca_i64 oo = 0;\n";
+    if (!ops.bcast[k]) os << "  nca_i64 no" << k << " = 0;\n";
+  os << "  nca_i64 na0 = 0, nl0 = 0, noo = 0;\n"
+     << "  auto locate = [&](nca_i64 tile) {\n"
+     << "    nca_i64 rem = tile;\n"
+     << "    nl0 = (rem % tL) * TL;\n    rem /= tL;\n"
+     << "    na0 = (rem % tA) * TA;\n    rem /= tA;\n";
+  for (int k : ops.unique)
+    if (!ops.bcast[k]) os << "    no" << k << " = 0;\n";
+  os << "    noo = 0;\n";
   os << "#pragma unroll\n    for (int d = NCA_ND - 2; d >= 0; --d) {\n      if (d == A) continue;\n"
      << "      const nca_i64 c = rem % shp.v[d];\n      rem /= shp.v[d];\n";
   for (int k : ops.unique)
-    if (!ops.bcast[k]) os << "      o" << k << " += c * st" << k << ".v[d];\n";
-  os << "      oo += c * sto.v[d];\n    }\n";
-  // phase 1: stage the operands that are unit-stride along A (tx runs along A)
-  if (tp.pair) {
-    os << "    for (int j = sy; j < TL; j += 512 / TA) {\n      const nca_i64 a = a0 + sx, l = l0 + j;\n"
-       << "      if (l < nL && a < nA) {\n";
-    for (int k : ops.unique)
-      if (!ops.bcast[k] && tp.lds[k]) {
-        const std::string Tk = ctype_of(t.arr_dtypes[k]);
-        const std::string ad = "(p" + std::to_string(k) + " + o" + std::to_string(k) + " + a * st" + std::to_string(k) +
-                               ".v[A] + l * st" + std::to_string(k) + ".v[L])";
-        os << "        if (a + 1 < nA) {\n          const nca_vec<" << Tk << ", 2> w = *reinterpret_cast<const nca_vec<" << Tk
-           << ", 2>*>" << ad << ";\n          sk" << k << "[j][sx] = w.v[0];\n          sk" << k << "[j][sx + 1] = w.v[1];\n"
-           << "        } else {\n          sk" << k << "[j][sx] = *reinterpret_cast<const " << Tk << "*>" << ad << ";\n        }\n";
-      }
-    os << "      }\n    }\n    __syncthreads();\n";
-  } else {
-    os << "    for (int j = sy; j < TL; j += 256 / TA) {\n      const nca_i64 a = a0 + sx, l = l0 + j;\n"
-       << "      if (a < nA && l < nL) {\n";
-    for (int k : ops.unique)
-      if (!ops.bcast[k] && tp.lds[k])
-        os << "        sk" << k << "[j][sx] = *reinterpret_cast<const " << ctype_of(t.arr_dtypes[k]) << "*>(p" << k
-           << " + o" << k << " + a * st" << k << ".v[A] + l * st" << k << ".v[L]);\n";
-    os << "      }\n    }\n    __syncthreads();\n";
+    if (!ops.bcast[k]) os << "      no" << k << " += c * st" << k << ".v[d];\n";
+  os << "      noo += c * sto.v[d];\n    }\n  };\n";
+  // stage the operands that are unit-stride along A (sx runs along A).  Every thread issues all
+  // its loads of a tile before any LDS store (a fixed count, unrolled), and lanes past the array's
+  // edge read the nearest element inside it (never used), so the loads are in flight together and
+  // no branch separates a load from its use.
+  const int step1 = tp.pair ? 512 / tp.ta : 256 / tp.ta, R1 = tp.tl / step1;
+  const int step2 = tp.pair ? 512 / tp.tl : 256 / tp.tl, R2 = tp.ta / step2;
+  for (int k : ops.unique) {
+    if (ops.bcast[k] || !tp.lds[k]) continue;
+    const std::string Tk = ctype_of(t.arr_dtypes[k]);
+    const std::string VT = tp.pair ? "nca_vec<" + Tk + ", 2>" : Tk;
+    os << "  " << VT << " g" << k << "[" << R1 << "]" << (tp.pair ? " = {}" : "") << ";\n";
   }
-  // phase 2: compute and store (ex runs along L).  value(dl) evaluates the tape at L offset dl.
-  auto value = [&](const std::string& dl, const std::string& dst) {
+  auto fetch = [&](const std::string& ind) {
+    os << ind << "{\n" << ind << "  const nca_i64 sa = na0 + sx < nA ? na0 + sx : nA - 1;\n";
+    for (int k : ops.unique) {
+      if (ops.bcast[k] || !tp.lds[k]) continue;
+      const std::string Tk = ctype_of(t.arr_dtypes[k]), K = std::to_string(k);
+      const std::string VT = tp.pair ? "nca_vec<" + Tk + ", 2>" : Tk;
+      os << "#pragma unroll\n" << ind << "  for (int r = 0; r < " << R1 << "; ++r) {\n"
+         << ind << "    const nca_i64 l = (nl0 + sy + r * " << step1 << " < nL) ? nl0 + sy + r * " << step1 << " : nL - 1;\n"
+         << ind << "    const char* ad = p" << K << " + no" << K << " + sa * st" << K << ".v[A] + l * st" << K << ".v[L];\n";
+      if (tp.pair)
+        os << ind << "    if (sa + 1 < nA) g" << K << "[r] = *reinterpret_cast<const " << VT << "*>(ad);\n"
+           << ind << "    else g" << K << "[r].v[0] = *reinterpret_cast<const " << Tk << "*>(ad);\n";
+      else
+        os << ind << "    g" << K << "[r] = *reinterpret_cast<const " << Tk << "*>(ad);\n";
+      os << ind << "  }\n";
+    }
+    os << ind << "}\n";
+  };
+  // compute and store one tile (ex runs along L).  value(dl) evaluates the tape at L offset dl;
+  // operands read directly (not staged) are read at the clamped position (ac, lc*) inside the
+  // array, and lanes past the edge store into a sink, so every lane issues the same loads and
+  // stores.
+  auto value = [&](const std::string& dl, const std::string& lcl, const std::string& dst) {
     os << "        {\n";
     for (int k : ops.unique) {
       if (ops.bcast[k]) continue;
@@ -726,27 +741,58 @@ static std::string gen_transpose(const Tape& t, const Operands& ops, const Trans
            << "][j]);\n";
       else
         os << "        const " << C << " u" << k << " = nca_ld<" << C << ">(reinterpret_cast<const " << Tk << "*>(p" << k
-           << " + o" << k << " + a * st" << k << ".v[A] + (l" << dl << ") * st" << k << ".v[L]));\n";
+           << " + o" << k << " + ac * st" << k << ".v[A] + " << lcl << " * st" << k << ".v[L]));\n";
     }
     os << body;
     os << "        " << dst << " = nca_store_cvt<" << D << ", " << C << ">::to(" << roots[0] << ");\n        }\n";
   };
-  if (tp.pair) {
-    os << "    for (int j = ey; j < TA; j += 512 / TL) {\n      const nca_i64 a = a0 + j, l = l0 + ex;\n"
-       << "      if (a < nA && l < nL) {\n        " << D << " r0, r1;\n";
-    value("", "r0");
-    os << "        char* q = outp + oo + a * sto.v[A] + l * sto.v[L];\n        if (l + 1 < nL) {\n";
-    value(" + 1", "r1");
-    os << "          nca_vec<" << D << ", 2> w;\n          w.v[0] = r0;\n          w.v[1] = r1;\n"
-       << "          *reinterpret_cast<nca_vec<" << D << ", 2>*>(q) = w;\n"
-       << "        } else {\n          *reinterpret_cast<" << D << "*>(q) = r0;\n        }\n";
-  } else {
-    os << "    for (int j = ey; j < TA; j += 256 / TL) {\n      const nca_i64 a = a0 + j, l = l0 + ex;\n"
-       << "      if (a < nA && l < nL) {\n        " << D << " r0;\n";
-    value("", "r0");
-    os << "        *reinterpret_cast<" << D << "*>(outp + oo + a * sto.v[A] + l * sto.v[L]) = r0;\n";
-  }
-  os << "      }\n    }\n    __syncthreads();\n  }\n}\n";
+  auto phase2 = [&]() {
+    os << "#pragma unroll\n    for (int r = 0; r < " << R2 << "; ++r) {\n"
+       << "      const int j = ey + r * " << step2 << ";\n"
+       << "      const nca_i64 a = a0 + j, l = l0 + ex;\n"
+       << "      const bool ok = a < nA && l < nL;\n"
+       << "      const nca_i64 ac = a < nA ? a : nA - 1, lc = l < nL ? l : nL - 1;\n"
+       << "      char* q = ok ? outp + oo + a * sto.v[A] + l * sto.v[L] : reinterpret_cast<char*>(nca_sink_words);\n";
+    if (tp.pair) {
+      os << "      const nca_i64 lc1 = l + 1 < nL ? l + 1 : nL - 1;\n"
+         << "      " << D << " r0, r1;\n";
+      value("", "lc", "r0");
+      value(" + 1", "lc1", "r1");
+      os << "      if (!ok || l + 1 < nL) {\n"
+         << "        nca_vec<" << D << ", 2> w;\n        w.v[0] = r0;\n        w.v[1] = r1;\n"
+         << "        *reinterpret_cast<nca_vec<" << D << ", 2>*>(q) = w;\n"
+         << "      } else {\n        *reinterpret_cast<" << D << "*>(q) = r0;\n      }\n";
+    } else {
+      os << "      " << D << " r0;\n";
+      value("", "lc", "r0");
+      os << "      *reinterpret_cast<" << D << "*>(q) = r0;\n";
+    }
+    os << "    }\n";
+  };
+  // one tile: the registers the fetch filled go to LDS, then the tile is computed and stored
+  auto tile_body = [&]() {
+    os << "    {\n    const nca_i64 a0 = na0, l0 = nl0, oo = noo;\n";
+    for (int k : ops.unique)
+      if (!ops.bcast[k]) os << "    const nca_i64 o" << k << " = no" << k << ";\n";
+    for (int k : ops.unique) {
+      if (ops.bcast[k] || !tp.lds[k]) continue;
+      const std::string K = std::to_string(k);
+      os << "#pragma unroll\n    for (int r = 0; r < " << R1 << "; ++r) {\n";
+      if (tp.pair)
+        os << "      sk" << K << "[sy + r * " << step1 << "][sx] = g" << K << "[r].v[0];\n"
+           << "      sk" << K << "[sy + r * " << step1 << "][sx + 1] = g" << K << "[r].v[1];\n";
+      else
+        os << "      sk" << K << "[sy + r * " << step1 << "][sx] = g" << K << "[r];\n";
+      os << "    }\n";
+    }
+    os << "    __syncthreads();\n";
+    phase2();
+    os << "    __syncthreads();\n    }\n";
+  };
+  os << "  for (nca_i64 tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {\n    locate(tile);\n";
+  fetch("    ");
+  tile_body();
+  os << "  }\n}\n";
   return os.str();
 }
 
