@@ -7,7 +7,8 @@
  * source tree, src/lib/ncarray/).  Built with -O3 -ffp-contract=off so float32 arithmetic
  * is evaluated one IEEE operation at a time, as the reference's scalar SSE code does.
  *
- * Parity pins (see tests/test_oracle.py): the reference's own known-answer tests
+ * Parity pins (see tests/test_oracle.py, and tests/test_oracle_ref.py against the reference's
+ * own host routines compiled into oracle/_ref): the reference's own known-answer tests
  * (test_reductions.cc, test_gpu_reductions.cu, test_binary_operations.cc,
  * test_unary_operations.cc, tests/python/test_operations.py), closed forms
  * (sum of iota, Laplacian of polynomials), numpy (argmax first occurrence, bincount,
@@ -62,6 +63,22 @@ static double load_d(const void* p, int dt, int64_t i) {
     case D_F64: return ((const double*)p)[i];
     case D_BF16: return bf16_to_f32(((const uint16_t*)p)[i]);
     default: return NAN;
+  }
+}
+
+/* value of element i of an integer-typed (or bool) array, exact */
+static int64_t load_i(const void* p, int dt, int64_t i) {
+  switch (dt) {
+    case D_BOOL: return ((const uint8_t*)p)[i] != 0;
+    case D_CHAR: case D_I8: return ((const int8_t*)p)[i];
+    case D_U8: return ((const uint8_t*)p)[i];
+    case D_U16: return ((const uint16_t*)p)[i];
+    case D_I16: return ((const int16_t*)p)[i];
+    case D_U32: return ((const uint32_t*)p)[i];
+    case D_I32: return ((const int32_t*)p)[i];
+    case D_U64: return (int64_t)((const uint64_t*)p)[i];
+    case D_I64: return ((const int64_t*)p)[i];
+    default: return 0;
   }
 }
 
@@ -228,7 +245,7 @@ double nca_cpu_sum_f64(const double* x, int64_t n) {
 int64_t nca_cpu_sum_i64(const void* x, int dt, int64_t n) {
   /* integer sums wrap modulo 2^64 (then the caller narrows to sum_type) */
   uint64_t acc = 0;
-  for (int64_t i = 0; i < n; ++i) acc += (uint64_t)(int64_t)load_d(x, dt, i);
+  for (int64_t i = 0; i < n; ++i) acc += (uint64_t)load_i(x, dt, i);
   return (int64_t)acc;
 }
 
@@ -256,7 +273,24 @@ static double min_num(double a, double b) {
  * at lowest(); the index is the C-order position of the first element strictly greater
  * than the running value (zeros compare equal, NaN never taken), 0 if none was ever taken.
  * The value follows max_num from lowest(). */
+/* 64-bit integers compare exactly (unsigned for uint64); the value returns as a double */
+static double int_extreme(const void* x, int dt, int64_t n, int want_max, int64_t* argidx) {
+  int64_t idx = 0, have = 0;
+  int64_t best = 0;
+  uint64_t ubest = 0;
+  for (int64_t i = 0; i < n; ++i) {
+    int64_t v = load_i(x, dt, i);
+    int take;
+    if (dt == D_U64) take = !have || (want_max ? (uint64_t)v > ubest : (uint64_t)v < ubest);
+    else take = !have || (want_max ? v > best : v < best);
+    if (take) { best = v; ubest = (uint64_t)v; idx = i; have = 1; }
+  }
+  if (argidx) *argidx = idx;
+  return dt == D_U64 ? (double)ubest : (double)best;
+}
+
 double nca_cpu_max(const void* x, int dt, int64_t n, double lowest, int64_t* argidx) {
+  if ((dt == D_I64 || dt == D_U64) && n > 0) return int_extreme(x, dt, n, 1, argidx);
   double acc = lowest, best = lowest;
   int64_t idx = 0;
   for (int64_t i = 0; i < n; ++i) {
@@ -269,6 +303,7 @@ double nca_cpu_max(const void* x, int dt, int64_t n, double lowest, int64_t* arg
 }
 /* min_recursive / argmin_recursive (host/reductions.hh:171-212) */
 double nca_cpu_min(const void* x, int dt, int64_t n, double highest, int64_t* argidx) {
+  if ((dt == D_I64 || dt == D_U64) && n > 0) return int_extreme(x, dt, n, 0, argidx);
   double acc = highest, best = highest;
   int64_t idx = 0;
   for (int64_t i = 0; i < n; ++i) {

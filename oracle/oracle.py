@@ -88,14 +88,22 @@ def _lim(code, lo):
 
 
 # ---- reductions -----------------------------------------------------------------------------------
+# op_traits<T>::sum_type (op_traits.hh:75, 380-410): the type itself unless promoted
+SUM_TYPE = {0: np.uint64, 2: np.uint64, 3: np.uint64, 4: np.uint32, 5: np.uint64, 6: np.int64, 7: np.int64,
+            8: np.int32, 9: np.int64}
+
+
 def sum_fold(x):
-    """host::sum_recursive (host/reductions.hh:58-73): sequential fold in sum_type."""
+    """host::sum_recursive (host/reductions.hh:58-73): sequential fold in sum_type.  Integer
+    sums wrap in sum_type (int32 and uint32 at 32 bits, the others at 64)."""
     x = np.ascontiguousarray(x).ravel()
     if x.dtype == np.float32:
         return np.float32(lib.nca_cpu_sum_f32(x, x.size))
     if x.dtype == np.float64:
         return lib.nca_cpu_sum_f64(x, x.size)
-    return int(lib.nca_cpu_sum_i64(x.ctypes.data, CODE[x.dtype], x.size))
+    c = CODE[x.dtype]
+    wide = np.array([lib.nca_cpu_sum_i64(x.ctypes.data, c, x.size)], np.int64)
+    return int(wide.astype(SUM_TYPE[c])[0])
 
 
 # largest finite bfloat16 (0x7f7f) and float16: the max/min identities of those storage types
@@ -263,6 +271,30 @@ def so_offsets(image, image_addr, base_addr, shape, strides, suboffsets):
             assert off.size == 0 or (off.min() >= 0 and off.max() + 8 <= img.size), "pointer read outside the image"
             ptr = img[off[..., None] + np.arange(8)].copy().view("<u8")[..., 0].astype(np.int64)
             off = ptr - int(image_addr) + so
+    return off.reshape(tuple(shape))
+
+
+def nc_offsets(image, image_addr, base_addr, shape, strides, pointer_axis=-1, ptr_offset=0, row_offset=0):
+    """Byte offsets into `image` of every element of an NC view in the device view form, C order.
+
+    Without a pointer axis, element (i_0, ..., i_{n-1}) is at base + sum_d i_d * strides[d].  With
+    pointer axis P it is at row + row_offset + sum_{d > P} i_d * strides[d], where row is the
+    8-byte pointer at entry i_P * strides[P] + ptr_offset of the table at
+    base + sum_{d < P} i_d * strides[d].  This is NCOffsetsPolicy::advance (layout.hh:468-476)
+    with the reference's per-axis byte offsets folded in: those of the axes before P (all of
+    them without a pointer axis) into base, offsets[P] into ptr_offset (table entries) and those
+    of the axes after P into row_offset; strides[P] is 1 for the reference's own views.
+    """
+    img = np.frombuffer(np.ascontiguousarray(image).tobytes(), dtype=np.uint8)
+    off = np.array(int(base_addr) - int(image_addr), dtype=np.int64)
+    for d, n in enumerate(shape):
+        if d == pointer_axis:
+            ent = off[..., None] + (np.arange(n, dtype=np.int64) * int(strides[d]) + int(ptr_offset)) * 8
+            assert ent.size == 0 or (ent.min() >= 0 and ent.max() + 8 <= img.size), "pointer read outside the image"
+            ptr = img[ent[..., None] + np.arange(8)].copy().view("<u8")[..., 0].astype(np.int64)
+            off = ptr - int(image_addr) + int(row_offset)
+        else:
+            off = off[..., None] + np.arange(n, dtype=np.int64) * int(strides[d])
     return off.reshape(tuple(shape))
 
 
