@@ -27,6 +27,8 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBPS = 8000.0
+# FP64 error evaluations of every workload (outside the timed regions); --no-errors clears it
+WITH_ERRORS = True
 SHAPE = (512, 512, 512)
 # the fused step's kernel: float32 march stencil with the axis-0 sum and the max in its epilogue
 MARCH_KERNEL = "nca_stencil_march_float32_axis0_max"
@@ -199,6 +201,8 @@ def main():
 
     # ---- errors vs FP64 (outside the timed region) ----
     errors = None
+    global WITH_ERRORS
+    WITH_ERRORS = not args.no_errors
     if not args.no_errors:
         fused_step()
         bf16_step()
@@ -399,26 +403,28 @@ def bench_c0(torch, capi, tapes, ti, dev, sptr, time_loop):
             e = A * B + nca.exp(-(A * A)) - B * 0.5
         return e.sum()  # a host scalar: waits for the result
 
-    # errors against an FP64 evaluation of the same inputs (the fused call's out and sum)
-    fused()
-    torch.cuda.synchronize()
-    a64, b64 = a.double(), b.double()
-    ref64 = a64 * b64 + torch.exp(-a64 * a64) - 0.5 * b64
-    diff = (out.double() - ref64).abs()
-    # a*b + exp(-a*a) - 0.5*b cancels to near zero for some inputs, where ulp and relative error
-    # say nothing about the evaluation: errors are given relative to the terms' magnitudes (the
-    # bound the tests state is 2 eps of that), and in ulp where the result keeps at least a
-    # quarter of that magnitude
-    scale = (a64 * b64).abs() + torch.exp(-a64 * a64) + 0.5 * b64.abs()
-    keep = ref64.abs() >= 0.25 * scale
-    eps = float(np.finfo(np.float32).eps)
-    errors = {"expr_max_err_in_eps_of_term_magnitude": round(float((diff / scale).max()) / eps, 4),
-              "expr_max_ulp_without_cancellation": round(float((diff[keep] / f32_ulp(torch, ref64[keep])).max()), 3),
-              "expr_max_rel_err_without_cancellation": float((diff[keep] / ref64[keep].abs()).max()),
-              "sum_vs_fp64_of_exact_values": sum_errors(torch, s.item(), ref64.sum().item(), ref64.abs().sum().item()),
-              "sum_vs_fp64_of_stored_values": sum_errors(torch, s.item(), out.double().sum().item(),
-                                                         out.double().abs().sum().item())}
-    del a64, b64, ref64, diff, scale, keep
+    errors = None
+    if WITH_ERRORS:
+        # errors against an FP64 evaluation of the same inputs (the fused call's out and sum)
+        fused()
+        torch.cuda.synchronize()
+        a64, b64 = a.double(), b.double()
+        ref64 = a64 * b64 + torch.exp(-a64 * a64) - 0.5 * b64
+        diff = (out.double() - ref64).abs()
+        # a*b + exp(-a*a) - 0.5*b cancels to near zero for some inputs, where ulp and relative error
+        # say nothing about the evaluation: errors are given relative to the terms' magnitudes (the
+        # bound the tests state is 2 eps of that), and in ulp where the result keeps at least a
+        # quarter of that magnitude
+        scale = (a64 * b64).abs() + torch.exp(-a64 * a64) + 0.5 * b64.abs()
+        keep = ref64.abs() >= 0.25 * scale
+        eps = float(np.finfo(np.float32).eps)
+        errors = {"expr_max_err_in_eps_of_term_magnitude": round(float((diff / scale).max()) / eps, 4),
+                  "expr_max_ulp_without_cancellation": round(float((diff[keep] / f32_ulp(torch, ref64[keep])).max()), 3),
+                  "expr_max_rel_err_without_cancellation": float((diff[keep] / ref64[keep].abs()).max()),
+                  "sum_vs_fp64_of_exact_values": sum_errors(torch, s.item(), ref64.sum().item(), ref64.abs().sum().item()),
+                  "sum_vs_fp64_of_stored_values": sum_errors(torch, s.item(), out.double().sum().item(),
+                                                             out.double().abs().sum().item())}
+        del a64, b64, ref64, diff, scale, keep
     cold_f = cold_device_ms(torch, capi, fused)
     cold_o = cold_device_ms(torch, capi, only)
     pe, _ = time_loop(py_eager, 30, 5)
@@ -477,22 +483,24 @@ def bench_c2(torch, capi, tapes, ti, dev, sptr, time_loop):
     for name, kms in recs:
         per.setdefault(name, []).append(kms)
     n = F * ny * nx
-    # counts and sums against a float64 scatter of the same frames by the same map
-    fwd()
-    torch.cuda.synchronize()
-    idx = bm.view(-1).long()
-    valid = (idx >= 0) & (idx < nb)
-    s64 = torch.zeros((F, nb), dtype=torch.float64, device=dev)
-    c64 = torch.zeros((F, nb), dtype=torch.int64, device=dev)
-    for f0 in range(0, F, 32):
-        blk = fr[f0:f0 + 32].reshape(fr[f0:f0 + 32].shape[0], -1)[:, valid]
-        s64[f0:f0 + 32].index_add_(1, idx[valid], blk.double())
-        c64[f0:f0 + 32].index_add_(1, idx[valid], (blk != 0).long())
-    err = (sums.double() - s64).abs()
-    c2_errors = {"counts_bit_exact_vs_fp64_scatter": bool(torch.equal(counts.long(), c64)),
-                 "sums_max_rel_err_vs_fp64": float((err / s64.abs().clamp_min(1e-300)).max()),
-                 "sums_max_ulp": round(float((err / f32_ulp(torch, s64)).max()), 3)}
-    del s64, c64, err
+    c2_errors = None
+    if WITH_ERRORS:
+        # counts and sums against a float64 scatter of the same frames by the same map
+        fwd()
+        torch.cuda.synchronize()
+        idx = bm.view(-1).long()
+        valid = (idx >= 0) & (idx < nb)
+        s64 = torch.zeros((F, nb), dtype=torch.float64, device=dev)
+        c64 = torch.zeros((F, nb), dtype=torch.int64, device=dev)
+        for f0 in range(0, F, 32):
+            blk = fr[f0:f0 + 32].reshape(fr[f0:f0 + 32].shape[0], -1)[:, valid]
+            s64[f0:f0 + 32].index_add_(1, idx[valid], blk.double())
+            c64[f0:f0 + 32].index_add_(1, idx[valid], (blk != 0).long())
+        err = (sums.double() - s64).abs()
+        c2_errors = {"counts_bit_exact_vs_fp64_scatter": bool(torch.equal(counts.long(), c64)),
+                     "sums_max_rel_err_vs_fp64": float((err / s64.abs().clamp_min(1e-300)).max()),
+                     "sums_max_ulp": round(float((err / f32_ulp(torch, s64)).max()), 3)}
+        del s64, c64, err
     fwd_bytes = 4 * n + 4 * ny * nx + 2 * 4 * F * nb
     bwd_bytes = 4 * n + 4 * ny * nx + 4 * F * nb
     return {"fwd_ms": round(float(np.mean(fm)), 4), "fwd_GBps": gbps(fwd_bytes, np.mean(fm)),
@@ -555,7 +563,7 @@ def bench_c3(torch, capi, tapes, ti, dev, sptr, time_loop):
         cold_vg = cold_device_ms(torch, capi, lambda: A.laplacian_c0_value_and_grad(xd, yd))
         with torch.no_grad():
             loss_eager = A.laplacian_c0_loss(x, y).clone()
-        errs = c3_errors(torch, xd, yd, loss_eager, vg)
+        errs = c3_errors(torch, xd, yd, loss_eager, vg) if WITH_ERRORS else None
         gfm, _ = graph_loop(torch, fwd, 10, 10)  # forward only (launched from this thread)
         # forward + backward without torch's autograd (the same kernels, launched from this
         # thread), 10 steps per hipGraph replay
@@ -781,21 +789,23 @@ def bench_c4(torch, capi, tapes, ti, dev, sptr, time_loop):
     nbytes = 12 * total
     same_max = bool(torch.equal(maxs, rb.maxs))
     rel = float(((sums.double() - rb.sums.double()).abs() / (rb.sums.double().abs() + 1.0)).max())
-    # the batch's sums against float64 sums of its stored outputs, and its max against the stored
-    # outputs' max (exact), per array
-    rb.launch(scalars=(0.5,), stream=sptr)
-    torch.cuda.synchronize()
-    worst_rel = worst_abs = 0.0
-    max_exact = True
-    for i in range(len(sizes)):
-        oi = o[i].double()
-        s64, sa = oi.sum().item(), oi.abs().sum().item()
-        d = abs(float(rb.sums[i]) - s64)
-        worst_rel = max(worst_rel, d / abs(s64) if s64 != 0 else d)
-        worst_abs = max(worst_abs, d / sa if sa else d)
-        max_exact = max_exact and float(rb.maxs[i]) == float(o[i].max())
-    c4_errors = {"sum_max_rel_err_vs_fp64": worst_rel, "sum_max_err_over_sum_abs": worst_abs,
-                 "max_equals_max_of_stored": bool(max_exact)}
+    c4_errors = None
+    if WITH_ERRORS:
+        # the batch's sums against float64 sums of its stored outputs, and its max against the stored
+        # outputs' max (exact), per array
+        rb.launch(scalars=(0.5,), stream=sptr)
+        torch.cuda.synchronize()
+        worst_rel = worst_abs = 0.0
+        max_exact = True
+        for i in range(len(sizes)):
+            oi = o[i].double()
+            s64, sa = oi.sum().item(), oi.abs().sum().item()
+            d = abs(float(rb.sums[i]) - s64)
+            worst_rel = max(worst_rel, d / abs(s64) if s64 != 0 else d)
+            worst_abs = max(worst_abs, d / sa if sa else d)
+            max_exact = max_exact and float(rb.maxs[i]) == float(o[i].max())
+        c4_errors = {"sum_max_rel_err_vs_fp64": worst_rel, "sum_max_err_over_sum_abs": worst_abs,
+                     "max_equals_max_of_stored": bool(max_exact)}
     rb.close()
     return {"n_arrays": len(sizes), "total_elements": total, "total_GB_moved": round(nbytes / 1e9, 3),
             "batched_default_ms": round(float(np.mean(dm)), 3), "default_config": cfg_default,
