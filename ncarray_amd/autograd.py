@@ -281,8 +281,8 @@ def laplacian_c0_backward(x, y, g=None, method=None):
     transposed stencil turns w into dx; the vectorised march kernels' shape and alignment rules
     apply).  "compose": the Laplacian, the derivative expression and the transposed stencil as
     three kernels (any shape).  The default is the fastest measured at 256^3: "train" for
-    4-byte types (61.8 us against 45.5 + 31.8 us for "vjp"), "vjp" for 2-byte types where it
-    applies (34.6 + 27.1 us against 64.4 us: the one-pass kernel's arithmetic and LDS traffic
+    4-byte types (58.3-59.2 us against 45.5 + 32.5 us for "vjp"), "vjp" for 2-byte types where it
+    applies (34.9 + 26.9 us against 62.5 us: the one-pass kernel's arithmetic and LDS traffic
     are those of float32), else "train" for 3-D C-contiguous arrays, else "compose"."""
     s = _stream()
     xcode, ycode = _code(x), _code(y)

@@ -1158,12 +1158,20 @@ static std::string gen_march_train_vec(const StencilProgram& p, const Tape& adj,
                                        "lin", "a", aroots);
   const int XW = TXs + 16, XH = TYs + 4, RWV = TXs + 8, RH = TYs + 2;
   const int XV = XW / 4, WV = RWV / 4, TV = TXs / 4;
+  // w vectors: the tile's own TV x TY first, then the one-point ring around it.  Without an
+  // offset that moves in both y and x, the transposed stencil reads no corner of the ring, so
+  // the ring is its two rows and two columns of vectors only (64 x 16 tiles: 256 + 64 vectors,
+  // one pass of every wave and one more of one wave, instead of a sixth partly filled pass)
+  bool diag = false;
+  for (auto& o : p.offsets)
+    if (o[1] != 0 && o[2] != 0) diag = true;
   const std::string kname =
       std::string("nca_stencil_train_") + dtype_info(p.src_dtype).name + (fz.fuse ? "_sum" : "") + "_v4";
   std::ostringstream os;
   os << "#define TX " << TXs << "\n#define TY " << TYs << "\n#define XW " << XW << "\n#define XH " << XH << "\n#define RWV "
      << RWV << "\n#define RH " << RH << "\n#define XV " << XV << "\n#define WV " << WV << "\n#define TV " << TV << "\n"
-     << "#define NXL ((XV * XH + 255) / 256)\n#define NWL ((WV * RH + 255) / 256)\n#define NTL ((TV * TY + 255) / 256)\n";
+     << "#define NWI " << (diag ? "(WV * RH)" : "(TV * TY + 2 * TV + 2 * TY)") << "\n"
+     << "#define NXL ((XV * XH + 255) / 256)\n#define NWL ((NWI + 255) / 256)\n#define NTL ((TV * TY + 255) / 256)\n";
   os << "__device__ __forceinline__ int nca_mod3(int z) { const int r = z % 3; return r < 0 ? r + 3 : r; }\n";
   os << "extern \"C\" __global__ __launch_bounds__(256) void " << kname << "(const " << S << "* __restrict__ src";
   for (int j = 0; j < n_x; ++j) os << ", const " << ctype_of(vj.extra_dtypes[j]) << "* __restrict__ e" << j;
@@ -1191,8 +1199,17 @@ static std::string gen_march_train_vec(const StencilProgram& p, const Tape& adj,
      << "    const int i = tid + 256 * m, r = i / XV, c = i - r * XV, gy = y0 - 2 + r, gx = x0 - 8 + 4 * c;\n"
      << "    xo[m] = (i < XV * XH && gy >= 0 && gy < NYi && gx >= 0 && gx < NXi) ? gy * NXi + gx : -1;\n  }\n";
   os << "  int go[NWL], xb[NWL], wb[NWL];\n  unsigned own = 0u;\n#pragma unroll\n  for (int m = 0; m < NWL; ++m) {\n"
-     << "    const int i = tid + 256 * m, r = i / WV, c = i - r * WV, gy = y0 - 1 + r, gx = x0 - 4 + 4 * c;\n"
-     << "    const bool in = i < WV * RH && gy >= 0 && gy < NYi && gx >= 0 && gx < NXi;\n"
+     << "    const int i = tid + 256 * m;\n";
+  if (diag) {
+    os << "    const int r = i / WV, c = i - r * WV;\n";
+  } else {
+    os << "    int r, c;\n"
+       << "    if (i < TV * TY) { r = 1 + i / TV; c = 1 + i % TV; }\n"
+       << "    else if (i < TV * TY + 2 * TV) { const int k = i - TV * TY; r = k < TV ? 0 : TY + 1; c = 1 + (k < TV ? k : k - TV); }\n"
+       << "    else { const int k = i - TV * TY - 2 * TV; r = 1 + (k < TY ? k : k - TY); c = k < TY ? 0 : TV + 1; }\n";
+  }
+  os << "    const int gy = y0 - 1 + r, gx = x0 - 4 + 4 * c;\n"
+     << "    const bool in = i < NWI && gy >= 0 && gy < NYi && gx >= 0 && gx < NXi;\n"
      << "    go[m] = in ? gy * NXi + gx : -1;\n    xb[m] = (r + 1) * XW + 4 + 4 * c;\n    wb[m] = r * RWV + 4 * c;\n"
      << "    if (in && r >= 1 && r <= TY && c >= 1 && c <= TV) own |= 1u << m;\n  }\n";
   os << "  int to[NTL], tb[NTL];\n#pragma unroll\n  for (int m = 0; m < NTL; ++m) {\n"
@@ -1278,7 +1295,7 @@ static std::string gen_march_train_vec(const StencilProgram& p, const Tape& adj,
      << "    const int sw = nca_mod3(zw), sm = nca_mod3(zw - 1), sp = nca_mod3(zw + 1);\n"
      << "    const size_t zoff = (size_t)(zw > 0 ? zw : 0) * plane;\n"
      << "#pragma unroll\n    for (int m = 0; m < NWL; ++m) {\n"
-     << "      const int i = tid + 256 * m;\n      if (i < WV * RH) {\n"
+     << "      const int i = tid + 256 * m;\n      if (i < NWI) {\n"
      << "        lvec wv = {};\n"
      << "        if (zin && go[m] >= 0) {\n";
   emit_reads(xnb, "xs", "sm", "sw", "sp", "xb[m]", XW, "X");

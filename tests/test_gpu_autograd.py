@@ -415,6 +415,32 @@ def test_stencil_expr_train_vector_matches_scalar(dev, dtype, shape, monkeypatch
     assert abs(l2.item() - loss64(xn, yn)) <= 8 * np.finfo(np.float32).eps * mass
 
 
+@pytest.mark.parametrize("tile", ["32x4", "64x16", "128x8", "32x32", "256x2"])
+def test_stencil_expr_train_vector_tiles(dev, tile, monkeypatch):
+    """The vector form's w vectors (the tile's own, then its ring of rows and columns) at tile
+    shapes whose vectors fill different numbers of passes: dx and dy bit-identical to the scalar
+    form, on a shape with partial tiles in x and y."""
+    A, capi, tapes = _mods()
+    from ncarray_amd import torch_interop as ti
+    g = torch.Generator(device=dev).manual_seed(77)
+    shape = (7, 45, 136)
+    x = torch.randn(shape, generator=g, device=dev)
+    y = torch.randn(shape, generator=g, device=dev)
+    lap, _ = A.laplacian_stencil(capi.FLOAT32, capi.FLOAT32)
+    tape = A.c0_tape(capi.FLOAT32, capi.FLOAT32, capi.FLOAT32)
+    monkeypatch.setenv("NCA_HIP_TRAIN_TILE", tile)
+    res = []
+    for scalar in ("1", "0"):
+        monkeypatch.setenv("NCA_HIP_TRAIN_SCALAR", scalar)
+        dx, dy = torch.empty_like(x), torch.empty_like(y)
+        lap.expr_train(tape, [-1, 0, -1, -1, 0], ti.view(x), ti.view(dx), extras=[ti.view(y)],
+                       grad_extras=[ti.view(dy)], scalars=(6.0,), expr_scalars=(0.5,))
+        res.append((dx, dy))
+    torch.cuda.synchronize()
+    (dx1, dy1), (dx2, dy2) = res
+    assert torch.equal(dx1, dx2) and torch.equal(dy1, dy2)
+
+
 # an asymmetric radius-1 stencil with diagonal offsets and scalar coefficients, and an
 # expression of it and two extra arrays: f = a*a*y + a*z (operands a, a, y, a, z)
 _DIAG_OFFSETS = [[-1, 1, -1], [1, -1, 1], [0, 1, 1], [1, 0, -1], [0, 0, 0], [-1, -1, 0], [0, -1, -1]]
