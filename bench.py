@@ -872,11 +872,18 @@ def cpu_info():
     try:
         with open("/proc/cpuinfo") as f:
             for line in f:
-                if line.startswith("model name"):
+                if line.startswith("model name") and "cpu_model" not in info:
                     info["cpu_model"] = line.split(":", 1)[1].strip()
+                elif line.startswith("flags") and "vector_isa" not in info:
+                    flags = set(line.split(":", 1)[1].split())
+                    info["vector_isa"] = [f for f in ("avx2", "fma", "avx512f", "avx512bw", "avx512_bf16") if f in flags]
+                if "cpu_model" in info and "vector_isa" in info:
                     break
     except OSError:
         pass
+    if info.get("lscpu_cpu_family", "").isdigit():
+        info["cpu_family_hex"] = hex(int(info["lscpu_cpu_family"]))  # 0x1a: AMD Zen 5
+    info["oracle_build"] = "gcc -O3 -march=x86-64-v3 (AVX2, FMA) -ffp-contract=off, OpenMP"
     return info
 
 
