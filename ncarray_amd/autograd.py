@@ -265,8 +265,6 @@ def _train_ok(x, y):
 
 
 def _default_method(x, y):
-    if x.element_size() == 2 and _march_vjp_ok(x, y):
-        return "vjp"
     return "train" if _train_ok(x, y) else ("vjp" if _march_vjp_ok(x, y) else "compose")
 
 
@@ -280,10 +278,9 @@ def laplacian_c0_backward(x, y, g=None, method=None):
     two-kernel form (nca_hip_stencil_expr_vjp writes w and dy in one march pass, the
     transposed stencil turns w into dx; the vectorised march kernels' shape and alignment rules
     apply).  "compose": the Laplacian, the derivative expression and the transposed stencil as
-    three kernels (any shape).  The default is the fastest measured at 256^3: "train" for
-    4-byte types (58.3-59.2 us against 45.5 + 32.5 us for "vjp"), "vjp" for 2-byte types where it
-    applies (34.9 + 26.9 us against 62.5 us: the one-pass kernel's arithmetic and LDS traffic
-    are those of float32), else "train" for 3-D C-contiguous arrays, else "compose"."""
+    three kernels (any shape).  The default is the fastest measured at 256^3: "train" for 3-D
+    C-contiguous arrays (float32 58.3-59.2 us against 45.5 + 32.5 us for "vjp"; bfloat16 55.8 us
+    against 34.9 + 26.9 us), else "vjp" where its shape and alignment rules hold, else "compose"."""
     s = _stream()
     xcode, ycode = _code(x), _code(y)
     lap, adj = laplacian_stencil(xcode, capi.FLOAT32)

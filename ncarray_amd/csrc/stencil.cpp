@@ -2564,12 +2564,13 @@ static void stencil_expr_vjp_impl(uint64_t stencil_handle, uint64_t expr_handle,
 }
 
 // Tile of the training-step march: TX x TY points per block, the fastest measured at 256^3:
-// 64 x 16, and for 2-byte sources 128 x 16 (vector form) or 64 x 8 (scalar form);
-// NCA_HIP_TRAIN_TILE = "TXxTY" overrides it, for tuning.
+// 64 x 16 for 4-byte sources (58-59 us; 32-128 x 8-16 within 2%), 64 x 8 for 2-byte sources
+// (vector form 55.8 us against 56.3 at 64 x 16 and 62.6 at 128 x 16, whose 62 KiB of LDS allow
+// two blocks per CU); NCA_HIP_TRAIN_TILE = "TXxTY" overrides it, for tuning.
 static void train_tile(int32_t src_dtype, bool vec, int& tx, int& ty) {
   const bool two_byte = itemsize(src_dtype) == 2;
-  tx = two_byte && vec ? 128 : 64;
-  ty = two_byte && !vec ? 8 : 16;
+  tx = 64;
+  ty = two_byte ? 8 : 16;
   if (const char* e = std::getenv("NCA_HIP_TRAIN_TILE")) {
     int a = 0, b = 0;
     if (std::sscanf(e, "%dx%d", &a, &b) == 2 && (a == 32 || a == 64 || a == 128 || a == 256) &&
