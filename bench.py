@@ -168,7 +168,8 @@ def main():
 
     un_ms, _ = time_loop(unfused_step, max(3, args.steps // 2), 2)
     un_mean = float(np.mean(un_ms))
-    bytes_unfused = 8 * n + (4 * n + 4 * SHAPE[1] * SHAPE[2]) + 4 * n  # per API op, BASELINE accounting
+    # per API op, each input read once and each output written once: stencil, axis-0 sum, max
+    bytes_unfused = 8 * n + (4 * n + 4 * SHAPE[1] * SHAPE[2]) + 4 * n
     stencil_only_ms, _ = time_loop(lambda: st.apply(vx, vlap, capi.BOUNDARY_ZERO, scalars=(6.0,), stream=sptr),
                                    max(3, args.steps // 2), 2)
 
