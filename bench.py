@@ -633,8 +633,10 @@ def c3_errors(torch, x, y, loss, vg):
 def bench_cp(torch, capi, tapes, ti, dev, sptr, time_loop):
     """Copies of a 512^3 float32 array (512 MiB) into dense arrays: a casting copy to BF16, a plain
     copy, a strided slice, two permuted copies and a gather of its 512 planes through an NC table of
-    plane pointers.  Kernel time from the HIP events the library records around each launch;
-    algorithmic bytes = elements x (source + destination itemsize)."""
+    plane pointers; and the two halves of a copy on their own, through the library's kernels: a
+    constant written into the 512 MiB (write only) and the full sum of a 1 GiB array, four times
+    the Infinity Cache (read only).  Kernel time from the HIP events the library records around
+    each launch; algorithmic bytes = elements x (source + destination itemsize)."""
     n = 512
     x = torch.randn((n, n, n), device=dev)
     one = lambda sd, dd: capi.Tape([capi.pack(capi.OP_LOAD_NCARR, 0)], [sd], [], capi.FLOAT32, dd)  # noqa: E731
@@ -644,6 +646,7 @@ def bench_cp(torch, capi, tapes, ti, dev, sptr, time_loop):
     ob = torch.empty((n, n, n), dtype=torch.bfloat16, device=dev)
     tbl = torch.tensor([x[i].data_ptr() for i in range(n)], dtype=torch.int64, device=dev)
     vnc = capi.make_nc_view(tbl.data_ptr(), capi.FLOAT32, (n, n, n), [1, n * 4, 4], 0)
+    fill = capi.Tape([capi.pack(capi.OP_LOAD_CONST, 0)], [], [capi.FLOAT32], capi.FLOAT32, capi.FLOAT32)
     S = capi.STREAM_DEFAULT_ASYNC
     cases = [
         ("cast_f32_to_bf16", lambda: to_bf16.launch([ti.view(x)], ti.view(ob), stream=S), n ** 3 * 6),
@@ -652,6 +655,7 @@ def bench_cp(torch, capi, tapes, ti, dev, sptr, time_loop):
         ("transpose_1_2", lambda: f32.launch([ti.view(x.transpose(1, 2))], ti.view(dense), stream=S), n ** 3 * 8),
         ("transpose_0_2", lambda: f32.launch([ti.view(x.transpose(0, 2))], ti.view(dense), stream=S), n ** 3 * 8),
         ("nc_planes_to_dense", lambda: f32.launch([vnc], ti.view(dense), stream=S), n ** 3 * 8),
+        ("fill_f32_write_only", lambda: fill.launch([], ti.view(dense), scalars=(1.5,), stream=S), n ** 3 * 4),
     ]
     res = {}
     for name, fn, nbytes in cases:
@@ -667,6 +671,29 @@ def bench_cp(torch, capi, tapes, ti, dev, sptr, time_loop):
         ms = float(np.median([k for _, k in recs]))
         res[name] = {"kernel": recs[0][0], "kernel_ms": round(ms, 4), "GBps": gbps(nbytes, ms),
                      "frac_of_8TBps": round(nbytes / (ms * 1e-3) / 1e9 / HBM_PEAK_GBPS, 4)}
+    # read only: the full sum of a 1 GiB float32 array (the reduction kernel and its combine, timed
+    # together per call)
+    big = torch.randn(1 << 28, device=dev)
+    vb = ti.view(big)
+    tot = torch.empty((), dtype=torch.float32, device=dev)
+    red = lambda: capi.call("nca_hip_reduce_full", capi.ctypes.byref(vb), capi.REDUCE_SUM, capi.ACCUM_PLAIN,  # noqa: E731
+                            tot.data_ptr(), S)
+    for _ in range(3):
+        red()
+    torch.cuda.synchronize()
+    per = []
+    for _ in range(10):
+        capi.profile_enable(True)
+        red()
+        torch.cuda.synchronize()
+        recs = capi.profile_read()
+        capi.profile_enable(False)
+        per.append(recs)
+    ms = float(np.median([sum(k for _, k in r) for r in per]))
+    nb = big.numel() * 4
+    res["sum_f32_1GiB_read_only"] = {"kernel": " + ".join(nm for nm, _ in per[0]), "kernel_ms": round(ms, 4),
+                                     "GBps": gbps(nb, ms), "frac_of_8TBps": round(nb / (ms * 1e-3) / 1e9 / HBM_PEAK_GBPS, 4)}
+    del big
     # the runtime's device-to-device copy of the same 512 MiB (hipMemcpyAsync), timed with events on
     # torch's stream: the copy ceiling the library's plain copy is compared with
     for _ in range(3):

@@ -450,7 +450,36 @@ std::string gen_elementwise(const Tape& t, const Operands& ops, bool strided, in
   os << bcast_loads(t, ops);
   os << rc.decl;
   os << "  const nca_i64 gstride = (nca_i64)gridDim.x * blockDim.x;\n";
-  if (!strided) {
+  bool reads_array = false;
+  for (int k : ops.unique)
+    if (!ops.bcast[k]) reads_array = true;
+  if (!strided && !reads_array && !fuse && with_out) {
+    // An expression that reads no array (a fill, an index generator) is a pure store stream.
+    // Stores need many in flight per lane to reach the write bandwidth (measured on MI355X for
+    // 512 MiB: one 16-byte store per lane per step 4.6 TB/s, eight about 5.5 TB/s), so each wave
+    // writes a contiguous chunk of 8 x 64 vectors per step, streaming (nontemporal) stores.
+    os << "  constexpr int V = " << vec << ";\n  constexpr int U = 8;\n";
+    os << "  const nca_i64 nv = n / V;\n  const int lane = threadIdx.x & 63;\n";
+    os << "  const nca_i64 nwv = (nca_i64)gridDim.x * (blockDim.x >> 6);\n";
+    os << "  const nca_i64 nch = (nv + U * 64 - 1) / (U * 64);\n";
+    os << "  for (nca_i64 c = (nca_i64)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); c < nch; c += nwv) {\n";
+    os << "#pragma unroll\n    for (int u = 0; u < U; ++u) {\n";
+    os << "      const nca_i64 i = (c * U + u) * 64 + lane;\n      if (i < nv) {\n";
+    os << "        nca_vec<" << D << ", V> r;\n";
+    os << "#pragma unroll\n        for (int j = 0; j < V; ++j) {\n";
+    if (t.uses_idx) os << "      const nca_i64 lin = i * V + j;\n";
+    os << body;
+    os << "      const " << D << " st = nca_store_cvt<" << D << ", " << C << ">::to(" << roots[0] << ");\n";
+    os << "          r.v[j] = st;\n        }\n";
+    os << "        if (out) nca_store_nt(reinterpret_cast<nca_vec<" << D << ", V>*>(out + i * V), r);\n";
+    os << "      }\n    }\n  }\n";
+    os << "  for (nca_i64 i = nv * V + (nca_i64)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gstride) {\n";
+    if (t.uses_idx) os << "      const nca_i64 lin = i;\n";
+    os << body;
+    os << "      const " << D << " st = nca_store_cvt<" << D << ", " << C << ">::to(" << roots[0] << ");\n";
+    os << "      if (out) out[i] = st;\n";
+    os << "  }\n";
+  } else if (!strided) {
     os << "  constexpr int V = " << vec << ";\n";
     os << "  const nca_i64 nv = n / V;\n";
     os << "  for (nca_i64 i = (nca_i64)blockIdx.x * blockDim.x + threadIdx.x; i < nv; i += gstride) {\n";
@@ -923,7 +952,12 @@ void expr_launch_impl(uint64_t handle, const nca_view* ins, int n_in, const nca_
     const int v = e ? std::atoi(e) : 4;
     return v >= 1 && v <= 64 ? v : 4;
   }();
-  const int max_blocks = fuse ? dev.n_cus * 4 : dev.n_cus * (strided ? 16 : contig_per_cu);
+  bool reads_array = false;
+  for (int k : ops.unique)
+    if (!ops.bcast[k]) reads_array = true;
+  // the store-only form (no array read): two blocks per CU, as for the fastest measured writes
+  const int max_blocks =
+      fuse ? dev.n_cus * 4 : dev.n_cus * (strided ? 16 : (!reads_array && with_out ? 2 : contig_per_cu));
   const int blocks = grid_blocks(items, 256, max_blocks);
   if (fuse) {
     Workspace& w = workspace(s, static_cast<size_t>(blocks) * 8 * 3);

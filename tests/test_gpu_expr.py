@@ -122,6 +122,29 @@ def test_iota(dev):
     assert (o.cpu().ravel() == torch.arange(105) * 2 + 1).all()
 
 
+@pytest.mark.parametrize("n", [1, 3, 4, 255, 2051, 8 * 64 * 4 * 3 + 5, (1 << 22) + 7])
+def test_generators_and_fills_at_chunk_edges(dev, n):
+    """Expressions that read no array (an index generator, a constant fill) take the store-only
+    form (each wave writes chunks of 8 x 64 vectors, the remainder element by element): every
+    element written once and exactly, in float32, int64 and BF16 outputs."""
+    capi, tapes, ti = _capi()
+    P = capi.pack
+    gen = [P(capi.OP_IDX), P(capi.OP_LOAD_CONST, 0), P(capi.OP_MUL), P(capi.OP_LOAD_CONST, 1), P(capi.OP_ADD)]
+    o = torch.full((n + 64,), -7.0, dtype=torch.float32, device=dev)
+    capi.Tape(gen, [], [capi.FLOAT32, capi.FLOAT32], capi.FLOAT32, capi.FLOAT32).launch([], ti.view(o[:n]), scalars=(0.5, 3.0))
+    oi = torch.full((n + 64,), -7, dtype=torch.int64, device=dev)
+    capi.Tape(gen, [], [capi.INT64, capi.INT64], capi.INT64, capi.INT64).launch([], ti.view(oi[:n]), scalars=(3, -2))
+    ob = torch.full((n + 64,), -7.0, dtype=torch.bfloat16, device=dev)
+    capi.Tape([P(capi.OP_LOAD_CONST, 0)], [], [capi.FLOAT32], capi.FLOAT32, capi.BFLOAT16).launch(
+        [], ti.view(ob[:n]), scalars=(1.25,))
+    torch.cuda.synchronize()
+    idx = torch.arange(n, dtype=torch.float64)
+    assert torch.equal(o[:n].cpu(), (idx * 0.5 + 3.0).float())
+    assert (o[n:].cpu() == -7.0).all()
+    assert torch.equal(oi[:n].cpu(), torch.arange(n, dtype=torch.int64) * 3 - 2) and (oi[n:].cpu() == -7).all()
+    assert (ob[:n].cpu().float() == 1.25).all() and (ob[n:].cpu().float() == -7.0).all()
+
+
 def test_div_by_zero_semantics(dev):
     """interface.hh:150-156: finite/0 -> NaN, inf/0 -> inf (kept), nan/0 -> nan."""
     capi, tapes, ti = _capi()
