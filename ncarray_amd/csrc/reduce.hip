@@ -300,17 +300,17 @@ __global__ __launch_bounds__(256) void k_reduce_full(const char* __restrict__ ba
     const nca_i64 nv = n / E;
     typedef nca_vec<T, E> VT;
     nca_i64 i = tid;
-    // two vectors in flight per iteration
+    // two vectors in flight per iteration; streaming loads (every element is read once)
     for (; i + gs < nv; i += 2 * gs) {
-      const VT v0 = reinterpret_cast<const VT*>(x)[i];
-      const VT v1 = reinterpret_cast<const VT*>(x)[i + gs];
+      const VT v0 = nca_load_nt(reinterpret_cast<const VT*>(x) + i);
+      const VT v1 = nca_load_nt(reinterpret_cast<const VT*>(x) + i + gs);
 #pragma unroll
       for (int j = 0; j < E; ++j) a.fold(nca_cvt<V, T>::from(v0.v[j]), i * E + j);
 #pragma unroll
       for (int j = 0; j < E; ++j) a.fold(nca_cvt<V, T>::from(v1.v[j]), (i + gs) * E + j);
     }
     for (; i < nv; i += gs) {
-      const VT v0 = reinterpret_cast<const VT*>(x)[i];
+      const VT v0 = nca_load_nt(reinterpret_cast<const VT*>(x) + i);
 #pragma unroll
       for (int j = 0; j < E; ++j) a.fold(nca_cvt<V, T>::from(v0.v[j]), i * E + j);
     }
@@ -357,10 +357,10 @@ __global__ __launch_bounds__(256) void k_reduce_cols(const typename RT<DT>::T* _
   const T* base = x + ko * R * KI + ki0;
   nca_i64 r = r0;
   for (; r + 3 < r1; r += 4) {
-    const VT v0 = *reinterpret_cast<const VT*>(base + (r + 0) * KI);
-    const VT v1 = *reinterpret_cast<const VT*>(base + (r + 1) * KI);
-    const VT v2 = *reinterpret_cast<const VT*>(base + (r + 2) * KI);
-    const VT v3 = *reinterpret_cast<const VT*>(base + (r + 3) * KI);
+    const VT v0 = nca_load_nt(reinterpret_cast<const VT*>(base + (r + 0) * KI));
+    const VT v1 = nca_load_nt(reinterpret_cast<const VT*>(base + (r + 1) * KI));
+    const VT v2 = nca_load_nt(reinterpret_cast<const VT*>(base + (r + 2) * KI));
+    const VT v3 = nca_load_nt(reinterpret_cast<const VT*>(base + (r + 3) * KI));
     const nca_i64 lin = (ko * R + r) * KI + ki0;
 #pragma unroll
     for (int j = 0; j < E; ++j) {
@@ -371,7 +371,7 @@ __global__ __launch_bounds__(256) void k_reduce_cols(const typename RT<DT>::T* _
     }
   }
   for (; r < r1; ++r) {
-    const VT v0 = *reinterpret_cast<const VT*>(base + r * KI);
+    const VT v0 = nca_load_nt(reinterpret_cast<const VT*>(base + r * KI));
     const nca_i64 lin = (ko * R + r) * KI + ki0;
 #pragma unroll
     for (int j = 0; j < E; ++j) a[j].fold(nca_cvt<V, T>::from(v0.v[j]), lin + j);
