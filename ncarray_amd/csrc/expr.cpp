@@ -878,8 +878,11 @@ static std::string gen_rows(const Tape& t, const Operands& ops, int32_t out_dtyp
      << "          if (l < nL) {\n";
   for (int k : ops.unique)
     if (!ops.bcast[k])
-      os << "            x" << k << "[q] = *reinterpret_cast<const " << ctype_of(t.arr_dtypes[k]) << "*>(rb" << k
-         << " + l * st" << k << ".v[L]);\n";
+      // streaming loads: each element is read once (measured: BF16 plane gathers of 512^3
+      // 0.142 -> 0.137 ms; the tiled transpose form measured up to 2% slower with them and keeps
+      // cached loads)
+      os << "            x" << k << "[q] = nca_ldnt(reinterpret_cast<const " << ctype_of(t.arr_dtypes[k]) << "*>(rb" << k
+         << " + l * st" << k << ".v[L]));\n";
   os << "          }\n        }\n";
   os << "#pragma unroll\n        for (int q = 0; q < U; ++q) {\n          const nca_i64 l = l0 + 64 * q;\n"
      << "          if (l < nL) {\n";

@@ -295,6 +295,32 @@ __device__ __forceinline__ void nca_store_nt(nca_vec<T, N>* p, const nca_vec<T, 
   }
 }
 
+// Streaming (non-temporal) load of one value of any 1-, 2-, 4-, 8- or 16-byte type (storage
+// types such as nca_bf16 included), for data this kernel reads once.
+template <typename T>
+__device__ __forceinline__ T nca_ldnt(const T* p) {
+  T v;
+  if constexpr (sizeof(T) == 16) {
+    const nca_u32x4 w = __builtin_nontemporal_load(reinterpret_cast<const nca_u32x4*>(p));
+    __builtin_memcpy(&v, &w, 16);
+  } else if constexpr (sizeof(T) == 8) {
+    const unsigned long long w = __builtin_nontemporal_load(reinterpret_cast<const unsigned long long*>(p));
+    __builtin_memcpy(&v, &w, 8);
+  } else if constexpr (sizeof(T) == 4) {
+    const unsigned w = __builtin_nontemporal_load(reinterpret_cast<const unsigned*>(p));
+    __builtin_memcpy(&v, &w, 4);
+  } else if constexpr (sizeof(T) == 2) {
+    const unsigned short w = __builtin_nontemporal_load(reinterpret_cast<const unsigned short*>(p));
+    __builtin_memcpy(&v, &w, 2);
+  } else if constexpr (sizeof(T) == 1) {
+    const unsigned char w = __builtin_nontemporal_load(reinterpret_cast<const unsigned char*>(p));
+    __builtin_memcpy(&v, &w, 1);
+  } else {
+    v = *p;
+  }
+  return v;
+}
+
 // Streaming (non-temporal) load of a whole vector: for data this kernel reads once.
 template <typename T, int N>
 __device__ __forceinline__ nca_vec<T, N> nca_load_nt(const nca_vec<T, N>* p) {
