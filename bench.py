@@ -586,17 +586,23 @@ def bench_c3(torch, capi, tapes, ti, dev, sptr, time_loop):
                     "device_value_and_grad_ms": round(vms, 4), "device_value_and_grad_GBps": gbps(fb_bytes, vms),
                     "device_value_and_grad_frac_of_8TBps": frac(fb_bytes, vms),
                     "device_value_and_grad_kernels": sorted({n for n, _ in vrecs}),
+                    # the one pass's own traffic: read x, y, write dx, dy
+                    "value_and_grad_pass_bytes": 4 * item * n3,
+                    "device_value_and_grad_pass_frac_of_8TBps": frac(4 * item * n3, vms),
+                    "graph_value_and_grad_pass_frac_of_8TBps": frac(4 * item * n3, gfbm),
                     "hbm_cold_fwd_ms": round(cold_fwd, 4), "hbm_cold_fwd_frac_of_8TBps": frac(fwd_bytes, cold_fwd),
                     "hbm_cold_value_and_grad_ms": round(cold_vg, 4),
                     "hbm_cold_value_and_grad_frac_of_8TBps": frac(fb_bytes, cold_vg),
                     "loss": float(loss_eager), "loss_value_and_grad": float(vg[0]),
                     "errors_vs_fp64": errs}
-    res["note"] = ("bytes are algorithmic (each input read once, each output written once); the backward is "
-                   "one march pass (the Laplacian recomputed, both gradients of the C0 expression written) and "
-                   "the transposed stencil; fwd_bwd_ms is host-bound (torch autograd and Python per step); "
+    res["note"] = ("bytes are algorithmic (each input read once, each output written once): forward = read x, "
+                   "y; fwd_bwd = the forward's bytes + the backward's (read x, y, write dx, dy), also for "
+                   "laplacian_c0_value_and_grad, whose one pass computes the loss and both gradients (its own "
+                   "traffic, value_and_grad_pass_bytes, is 2/3 of that: the *_pass_frac_* fields); the backward "
+                   "is one training-step march (the Laplacian recomputed, dx = Laplacian^T of dloss/da from LDS, "
+                   "dy); fwd_bwd_ms is host-bound (torch autograd and Python per step); "
                    "graph_fwd replays a hipGraph of 10 forwards (no Python launch overhead), graph_fwd_bwd one of 10 "
-                   "forward+backward steps (laplacian_c0_value_and_grad: one march pass computes the loss and both "
-                   "gradients of the C0 expression, then the transposed stencil); device_fwd_bwd sums the kernel "
+                   "laplacian_c0_value_and_grad steps; device_fwd_bwd sums the kernel "
                    "durations of a torch-autograd step (forward kernel, then the backward), device_value_and_grad "
                    "those of a laplacian_c0_value_and_grad step (HIP events per launch); 134 MB working set: "
                    "cache-resident, except hbm_cold_* (device time after reading 512 MiB flushed the "
