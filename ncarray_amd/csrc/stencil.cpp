@@ -675,8 +675,10 @@ static std::string gen_march_q(const StencilProgram& p, const StencilFuse& fz, i
     for (int r = 0; r < RPT; ++r) conv_row("rq[" + I(q) + "][" + I(r) + "]", "cq[" + I(q) + "][" + I(r) + "]");
   };
   const std::string zsrc = "reinterpret_cast<const " + S + "*>(nca_zero_words)";
+  // planes past the run's last input plane (ze - 1 + RZ) are not loaded: the steps that would
+  // prefetch them read the zero buffer or the array's first vector (cached) instead
   auto gload = [&](int q, const std::string& pz, const std::string& off) {
-    os << "    {\n      const bool zin = (unsigned)(" << pz << ") < (unsigned)NZi;\n";
+    os << "    {\n      const bool zin = (unsigned)(" << pz << ") < (unsigned)NZi && (" << pz << ") < ze + " << RZ << ";\n";
     if (stab)
       os << "      const " << S << "* __restrict__ pl = zin ? splane(" << pz << ") : reinterpret_cast<const " << S
          << "*>(nca_zero_words);\n";
