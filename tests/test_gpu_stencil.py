@@ -211,6 +211,38 @@ def test_laplacian_bf16_storage(dev, shape):
     assert err.max() <= 2e-7
 
 
+@pytest.mark.parametrize("src", ["float32", "bfloat16"])
+@pytest.mark.parametrize("nz", [12, 18, 30, 40])
+def test_march_short_z_runs(dev, src, nz):
+    """512 x 512 planes are 128 tiles each, so one resident wave of blocks splits z into runs of
+    a few planes: every run starts and ends inside the array (its halo planes come from the
+    neighbouring runs and it fetches no plane past its last input plane).  Laplacian bit-exact
+    against the oracle (BF16: rounded once), max exact, column sums against FP64."""
+    capi, tapes, ti = _c()
+    shape = (nz, 512, 512)
+    x = np.random.default_rng(nz).standard_normal(shape).astype(np.float32)
+    code = capi.FLOAT32 if src == "float32" else capi.BFLOAT16
+    xd = torch.from_numpy(x).to(dev)
+    if src == "bfloat16":
+        xd = xd.to(torch.bfloat16)
+    xr = xd.float().cpu().numpy()
+    st = capi.Stencil(tapes.LAPLACIAN_OFFSETS, tapes.laplacian(), code, capi.FLOAT32, code, [capi.FLOAT32])
+    lap = torch.empty_like(xd)
+    col = torch.empty(shape[1:], dtype=torch.float32, device=dev)
+    mx = torch.empty((), dtype=xd.dtype, device=dev)
+    st.apply_reduce(ti.view(xd), ti.view(lap), capi.BOUNDARY_ZERO, axis0=ti.view(col), fuse=capi.FUSE_MAX,
+                    mode=capi.ACCUM_COMPENSATED, max_out=mx.data_ptr(), scalars=(6.0,))
+    ref = O.stencil_f32(xr, shape, tapes.LAPLACIAN_OFFSETS, tapes.laplacian(), [6.0], True)
+    if src == "bfloat16":
+        ref = O.bf16_round(ref)
+    got = lap.float().cpu().numpy()
+    assert np.array_equal(got.view(np.uint32), ref.view(np.uint32))
+    assert mx.float().item() == float(got.max())
+    g64 = got.astype(np.float64)
+    err = np.abs(col.cpu().numpy() - g64.sum(0)) / np.maximum(np.abs(g64).sum(0), 1e-30)
+    assert err.max() <= 2e-7
+
+
 def _sum_tape(capi, n):
     P = capi.pack
     t = [P(capi.OP_LOAD_NCARR, 0)]
