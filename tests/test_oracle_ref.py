@@ -76,7 +76,7 @@ def test_full_reductions_match_reference(dtype, n, special):
     s = O.sum_fold(x)
     if x.dtype.kind == "f":
         assert same_bits_f32(s, ref[R.SUM][0]) if dtype == np.float32 else same_f(s, ref[R.SUM][0])
-    else:  # the sum in sum_type (the driver hands it over as int64 bits)
+    else:  # the sum in sum_type (oracle/ref_driver.cc hands it over as int64 bits)
         st = np.iinfo(O.SUM_TYPE[code])
         assert st.min <= s <= st.max and (s - ref[R.SUM][1]) % 2 ** 64 == 0
     mx, imx = O.max_fold(x)
